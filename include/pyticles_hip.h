@@ -1,0 +1,186 @@
+/*
+ * pyticles_hip.h -- C ABI of the MI355X (gfx950) smooth-particle library.
+ *
+ * Plain C types only: pointers, sizes, a POD parameter struct and int status
+ * codes (0 = OK).  Every call that fails sets a message readable through
+ * sph_last_error().  Host arrays are C-order float64 (int64 for pair
+ * indices), borrowed only for the duration of the call.  All particle
+ * arrays are 2-D: r[n][2], v[n][2], gradv[n][2][2], P[n][2][2], jq[n][2].
+ *
+ * The reference (andrewcharles/pyticles) has no C interface: its operator
+ * API is a set of Python objects sharing NumPy arrays.  Each entry point
+ * below names the reference function whose work it replaces; the Python
+ * package pyticles_amd re-exposes them under the reference's names.
+ */
+#ifndef PYTICLES_HIP_H
+#define PYTICLES_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SPH_OK 0
+#define SPH_ERR_ARG 1      /* invalid argument or size                         */
+#define SPH_ERR_HIP 2      /* HIP runtime failure                              */
+#define SPH_ERR_NOMEM 3    /* device allocation or arena budget exhausted     */
+#define SPH_ERR_CAP 4      /* caller-supplied capacity too small (pairs)      */
+#define SPH_ERR_STATE 5    /* call not valid in the current state             */
+#define SPH_ERR_NUMERIC 6  /* non-finite value or EOS pole (rho*b >= 1)       */
+
+/* rate terms (bit mask) */
+#define SPH_PRESSURE 1u    /* repulsive vdW pressure  (forces.py:321-368)      */
+#define SPH_COHESION 2u    /* cohesive pressure       (forces.py:371-405)      */
+#define SPH_VISCOSITY 4u   /* viscous stress          (spam_complete_force.py:158-165) */
+#define SPH_CONDUCTION 8u  /* heat conduction         (c_forces.pyx:185-239)   */
+#define SPH_ALL_TERMS 15u
+
+/* precision forms */
+#define SPH_FP64 0         /* FP64 everywhere                                  */
+#define SPH_FP32C 1        /* FP32 pair arithmetic, Kahan-compensated sums,    */
+                           /* FP64 integration, energy equation and totals    */
+
+typedef struct sph_ctx sph_ctx;
+typedef struct sph_box sph_box;
+typedef struct sph_arena sph_arena;
+
+/* Physical and numerical parameters of one periodic 2-D box.  Names follow
+ * the reference: h/hl = p.h/p.hlr (particles.py:331-341), rc/tol = VerletList
+ * cutoff/tolerance (neighbour_list.py:153-157), a/b/kB = ADASH/BDASH/KBDASH
+ * (properties.py:18-20), eta/zeta (spam_complete_force.py:43-44), kcond =
+ * thermalk (f_properties.py:14). */
+typedef struct {
+    double Lx, Ly;
+    double h, hl;
+    double rc, tol;
+    double a, b, kB;
+    double eta, zeta, kcond;
+    double dt;
+    unsigned terms;    /* SPH_* rate terms used by sph_step                    */
+    int precision;     /* SPH_FP64 or SPH_FP32C                                */
+} sph_params;
+
+/* ---- context ------------------------------------------------------------ */
+/* One context owns one device and one HIP stream.  Not shared across threads
+ * without external locking (the reference is single-threaded CPython). */
+int sph_ctx_create(int device, sph_ctx **out);
+int sph_ctx_destroy(sph_ctx *ctx);
+/* Message of the last failure on this thread (never NULL). */
+const char *sph_last_error(void);
+/* Library version string. */
+const char *sph_version(void);
+/* Per-pass GPU timing with HIP events (off by default; on disables graph
+ * replay so every pass can be bracketed). */
+int sph_ctx_set_timing(sph_ctx *ctx, int on);
+
+/* ---- one box ------------------------------------------------------------ */
+/* Device-resident state of n particles.  Replaces the arrays of
+ * SmoothParticleSystem (particles.py:325-344) for the hot path. */
+int sph_box_create(sph_ctx *ctx, int64_t n, const sph_params *prm, sph_box **out);
+int sph_box_destroy(sph_box *box);
+int sph_box_set_params(sph_box *box, const sph_params *prm);
+/* Upload state (any pointer may be NULL to keep the current value).
+ * Replaces scatter_state (particles.py:512-526). */
+int sph_box_upload(sph_box *box, const double *r, const double *v,
+                   const double *m, const double *u);
+/* Download state and derived fields in the caller's particle order (any
+ * pointer may be NULL).  rho..P hold the values of the last density /
+ * gradient / rate evaluation.  Replaces the write-backs of
+ * spam_complete_force.py:171-183 and f_properties.py:138-147. */
+int sph_box_download(sph_box *box, double *r, double *v, double *u,
+                     double *rho, double *rho_lr, double *T, double *p,
+                     double *pco, double *acc, double *udot, double *gradv,
+                     double *jq, double *P);
+
+/* ---- (a) neighbour list ------------------------------------------------- */
+/* Periodic cell-list build of every unordered pair with
+ * rsq < rc^2 + tol^2, i<j, written lexicographically into iap[cap][2].
+ * Replaces VerletList.build (neighbour_list.py:160-189).  *nip_out gets the
+ * true count; returns SPH_ERR_CAP (after filling cap pairs) if it exceeds
+ * cap. */
+int sph_nlist_build(sph_box *box, int64_t cap, int64_t *iap, int64_t *nip_out);
+/* Per-pair dr = r_j - r_i (minimum image), r, r^2, dv = v_j - v_i for the
+ * given list.  Replaces NeighbourList.separations (neighbour_list.py:63-83)
+ * and pairsep (pairsep.pyx:25-81).  Outputs may be NULL. */
+int sph_separations(sph_box *box, int64_t nip, const int64_t *iap,
+                    double *drij, double *rij, double *rsq, double *dv);
+/* Minimum image of np displacements in place (dr[np][2]).  Replaces
+ * NeighbourList.minimum_image (neighbour_list.py:105-123). */
+int sph_minimum_image(sph_ctx *ctx, int64_t np, double *dr, double Lx, double Ly);
+
+/* ---- (b) kernel, density, EOS ------------------------------------------- */
+/* Lucy kernel W and dW/dx for np pairs in 1-3 dimensions (dx[np][dim]).
+ * Replaces spkernel.lucy_kernel (spkernel.py:86-118). */
+int sph_kernel_lucy(sph_ctx *ctx, int64_t np, const double *r, const double *dx,
+                    double h, int dim, double *w, double *dwdx);
+/* Summation density (short and long h) and EOS (T, p, pco).  Replaces
+ * properties.spam_properties (properties.py:63-120) and the density / EOS
+ * part of f_properties.spam_properties (f_properties.py:100-120). */
+int sph_density(sph_box *box);
+/* Velocity gradient and heat flux (needs sph_density first).  Replaces
+ * splib.calc_grad_v / calc_heat_flux_3d (f_properties.py:111,130). */
+int sph_gradients(sph_box *box);
+/* Rates of change for the given terms at the current state (density and,
+ * if needed, gradients included); results in acc / udot.  Replaces
+ * c_forces.SpamForce / CohesiveSpamForce / SpamConduction .apply
+ * (c_forces.pyx:62-239) and sphforce3d.calc_sphforce3d
+ * (spam_complete_force.py:158-165). */
+int sph_rates(sph_box *box, unsigned terms);
+/* u = kB*T - a*rho from a density pass (properties.py:44-46,119).  T may be
+ * NULL to use T0 for every particle. */
+int sph_init_thermal(sph_box *box, const double *T, double T0);
+
+/* ---- time stepping ------------------------------------------------------ */
+/* nsteps leapfrog steps (kick-drift-kick, consecutive half kicks fused) of
+ * the full hot path: cell build, density+EOS, gradients, rates, kick/drift
+ * with periodic wrap.  Replaces SmoothParticleSystem.update
+ * (particles.py:459-494) for the periodic case. */
+int sph_step(sph_box *box, int64_t nsteps);
+/* Totals at the current (synchronised) state: sum m v^2/2, sum m u, sum. */
+int sph_energy(sph_box *box, double *ke, double *ue, double *etot);
+/* Accumulated GPU time per pass in ms: [0] bin+sort, [1] density,
+ * [2] gradients, [3] rates+integrate, [4] pair export; and [5] number of
+ * steps.  n = number of entries to read (<= 6).  reset != 0 clears them.
+ * Mirrors the p.timing dictionary (particles.py:164-169,359). */
+int sph_box_timing(sph_box *box, double *out, int n, int reset);
+
+/* ---- batching and serving ----------------------------------------------- */
+/* An arena holds many independent boxes in one device-resident particle
+ * pool; one step launch advances every running box by one step.  No
+ * counterpart in the reference (it runs one system per process). */
+int sph_arena_create(sph_ctx *ctx, int64_t particle_capacity, int precision,
+                     sph_arena **out);
+int sph_arena_destroy(sph_arena *ar);
+/* Admit a box of n particles (r[n][2], v[n][2], m[n]) at temperature T0
+ * (u = kB*T0 - a*rho is set on device) that will run nsteps steps.
+ * Returns SPH_ERR_NOMEM if it does not fit; *box_id receives a handle. */
+int sph_arena_add(sph_arena *ar, int64_t n, const sph_params *prm,
+                  const double *r, const double *v, const double *m, double T0,
+                  int64_t nsteps, int *box_id);
+/* Advance every running box by nsteps arena steps (replays the captured
+ * step graph).  Boxes that finish stay frozen until removed. */
+int sph_arena_step(sph_arena *ar, int64_t nsteps);
+/* Per-box progress: steps done (or -1 while initialising) and done flag. */
+int sph_arena_poll(sph_arena *ar, int box_id, int64_t *steps_done, int *done);
+/* States (0 free, 1 initialising, 2 running, 3 done), step index and health
+ * bits of box ids 0..nmax-1 with one device read. */
+int sph_arena_poll_all(sph_arena *ar, int nmax, int *state, int64_t *k, int *health);
+/* Copy a box's state out (same layout as sph_box_download; any NULL). */
+int sph_arena_result(sph_arena *ar, int box_id, double *r, double *v, double *u,
+                     double *rho, double *e0, double *e1);
+/* Release a box's slots. */
+int sph_arena_remove(sph_arena *ar, int box_id);
+/* Occupancy: particles in use, capacity, running boxes, total boxes. */
+int sph_arena_stats(sph_arena *ar, int64_t *used, int64_t *capacity,
+                    int *running, int *boxes);
+/* Replay the step as a captured HIP graph (default on) or launch directly. */
+int sph_arena_set_graph(sph_arena *ar, int on);
+int sph_box_set_graph(sph_box *box, int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYTICLES_HIP_H */

@@ -1,0 +1,1139 @@
+// sph_capi.hip -- host runtime and C ABI (include/pyticles_hip.h).
+//
+// One sph_ctx = one device + one HIP stream.  All particle state lives in an
+// Arena (device-resident structure of arrays, see sph_common.hpp); a sph_box
+// is an arena holding a single box.  Every pass is launched on the context
+// stream; host synchronisation happens only where a result must come back.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pyticles_hip.h"
+#include "sph_common.hpp"
+#include "sph_kernels.hpp"
+#include "sph_sort.hpp"
+
+using namespace sph;
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(SPH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(SPH_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                       " bytes failed: " + hipGetErrorString(e));
+    }
+    return SPH_OK;
+}
+
+template <class T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+int bits_for(int nkeys)
+{
+    int b = 1;
+    while ((1ll << b) < (long long)nkeys) ++b;
+    return b;
+}
+
+}  // namespace
+
+struct sph_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int timing = 0;
+    // scratch for the stateless helpers (kernel, minimum image)
+    double *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+
+    int scratch(size_t bytes)
+    {
+        if (bytes <= scratch_bytes) return SPH_OK;
+        dfree(d_scratch);
+        scratch_bytes = 0;
+        int rc = dalloc((char **)&d_scratch, bytes);
+        if (rc) return rc;
+        scratch_bytes = bytes;
+        return SPH_OK;
+    }
+};
+
+namespace {
+
+enum Timer { T_SORT = 0, T_DENS, T_GRAD, T_RATES, T_PAIRS, T_N };
+
+struct Arena {
+    sph_ctx *ctx = nullptr;
+    int precision = SPH_FP64;
+    int cap = 0;   // slot capacity
+    int hwm = 0;   // slots in use (end of the last box)
+
+    // device state (cap slots)
+    double2 *r = nullptr, *v = nullptr, *r_s = nullptr, *v_s = nullptr;
+    double *u = nullptr, *m = nullptr, *u_s = nullptr, *m_s = nullptr;
+    int *id = nullptr, *id_s = nullptr, *pcell = nullptr, *slot_box = nullptr;
+    double2 *vp_s = nullptr, *acc_s = nullptr;
+    double *up_s = nullptr, *udot_s = nullptr;
+    double2 *rho = nullptr, *Q = nullptr, *acc = nullptr, *jq = nullptr;
+    double *T = nullptr, *udot = nullptr, *e = nullptr;
+    double4 *S = nullptr, *gradv = nullptr;
+    float4 *rf = nullptr, *S32 = nullptr;
+    float2 *vf = nullptr, *rho32 = nullptr, *Q32 = nullptr;
+    float *T32 = nullptr;
+    unsigned *keys_in = nullptr, *k_out = nullptr, *v_out = nullptr, *k_alt = nullptr, *v_alt = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int *cell_start = nullptr;
+    int cell_cap = 0;
+
+    // box / segment tables
+    std::vector<BoxDesc> boxes;
+    std::vector<int> box_live;
+    std::vector<SegDesc> segs;
+    BoxDesc *d_boxes = nullptr;
+    BoxDyn *d_dyn = nullptr;
+    SegDesc *d_segs = nullptr;
+    int box_cap = 0, seg_cap = 0;
+    int nkeys = 1;
+    BoxDyn *h_dyn = nullptr;  // pinned mirror for polling
+
+    // staging
+    double *d_stage = nullptr;
+    size_t stage_bytes = 0;
+    double *d_e3 = nullptr;
+
+    // pair export buffers
+    int *d_inv = nullptr;
+    unsigned *d_cnt = nullptr, *d_off = nullptr, *d_grand = nullptr;
+    int *d_jbuf = nullptr;
+    long long *d_iap = nullptr;
+    size_t jbuf_cap = 0, iap_cap = 0;
+    void *scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+
+    // timing
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_marks;
+    double t_acc[T_N] = {0, 0, 0, 0, 0};
+    double steps_done = 0;
+
+    // graph of one arena step
+    hipGraphExec_t graph = nullptr;
+    int graph_n = -1, graph_keys = -1;
+    int use_graph = 1;
+
+    bool fp32() const { return precision == SPH_FP32C; }
+
+    ArenaView view() const
+    {
+        ArenaView A;
+        memset(&A, 0, sizeof(A));
+        A.r = r; A.v = v; A.u = u; A.m = m; A.id = id;
+        A.r_s = r_s; A.v_s = v_s; A.u_s = u_s; A.m_s = m_s; A.id_s = id_s; A.pcell = pcell;
+        A.vp_s = vp_s; A.up_s = up_s; A.acc_s = acc_s; A.udot_s = udot_s;
+        A.slot_box = slot_box;
+        A.rho = rho; A.T = T; A.S = S; A.Q = Q; A.acc = acc; A.udot = udot;
+        A.gradv = gradv; A.jq = jq; A.e = e;
+        A.rf = rf; A.vf = vf; A.rho32 = rho32; A.T32 = T32; A.S32 = S32; A.Q32 = Q32;
+        A.cell_start = cell_start;
+        A.keys = k_out; A.vals = v_out; A.keys_in = keys_in; A.vals_in = nullptr;
+        A.boxes = d_boxes; A.dyn = d_dyn; A.segs = d_segs;
+        A.nseg = (int)segs.size();
+        A.n = hwm;
+        A.nkeys = nkeys;
+        return A;
+    }
+
+    int alloc(int capacity, int prec)
+    {
+        cap = capacity;
+        precision = prec;
+        int rc = 0;
+        rc |= dalloc(&r, cap); rc |= dalloc(&v, cap); rc |= dalloc(&r_s, cap); rc |= dalloc(&v_s, cap);
+        rc |= dalloc(&u, cap); rc |= dalloc(&m, cap); rc |= dalloc(&u_s, cap); rc |= dalloc(&m_s, cap);
+        rc |= dalloc(&id, cap); rc |= dalloc(&id_s, cap); rc |= dalloc(&pcell, cap);
+        rc |= dalloc(&slot_box, cap);
+        rc |= dalloc(&vp_s, cap); rc |= dalloc(&up_s, cap); rc |= dalloc(&acc_s, cap); rc |= dalloc(&udot_s, cap);
+        rc |= dalloc(&rho, cap); rc |= dalloc(&Q, cap); rc |= dalloc(&acc, cap); rc |= dalloc(&jq, cap);
+        rc |= dalloc(&T, cap); rc |= dalloc(&udot, cap); rc |= dalloc(&e, cap);
+        rc |= dalloc(&S, cap); rc |= dalloc(&gradv, cap);
+        if (prec == SPH_FP32C) {
+            rc |= dalloc(&rf, cap); rc |= dalloc(&S32, cap); rc |= dalloc(&vf, cap);
+            rc |= dalloc(&rho32, cap); rc |= dalloc(&Q32, cap); rc |= dalloc(&T32, cap);
+        }
+        rc |= dalloc(&keys_in, cap); rc |= dalloc(&k_out, cap); rc |= dalloc(&v_out, cap);
+        rc |= dalloc(&k_alt, cap); rc |= dalloc(&v_alt, cap);
+        sort_tmp_bytes = radix_sort_temp_bytes(cap);
+        rc |= dalloc((char **)&sort_tmp, sort_tmp_bytes);
+        rc |= dalloc(&d_e3, 3);
+        if (rc) return SPH_ERR_NOMEM;
+        // zero-fill the state so that unused slots hold finite values
+        size_t z2 = sizeof(double2) * cap, z1 = sizeof(double) * cap;
+        if (hipMemset(r, 0, z2) || hipMemset(v, 0, z2) || hipMemset(u, 0, z1) || hipMemset(m, 0, z1) ||
+            hipMemset(id, 0, sizeof(int) * cap) || hipMemset(rho, 0, z2) || hipMemset(T, 0, z1) ||
+            hipMemset(S, 0, sizeof(double4) * cap) || hipMemset(Q, 0, z2) || hipMemset(acc, 0, z2) ||
+            hipMemset(udot, 0, z1) || hipMemset(gradv, 0, sizeof(double4) * cap) ||
+            hipMemset(jq, 0, z2) || hipMemset(e, 0, z1) || hipMemset(acc_s, 0, z2) ||
+            hipMemset(udot_s, 0, z1))
+            return fail(SPH_ERR_HIP, "hipMemset failed");
+        // the context stream does not synchronise with the null stream
+        HIPCHK(hipDeviceSynchronize());
+        segs.clear();
+        segs.push_back(SegDesc{0, cap, -1, 0});
+        return ensure_tables(4);
+    }
+
+    void release()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        dfree(r); dfree(v); dfree(r_s); dfree(v_s); dfree(u); dfree(m); dfree(u_s); dfree(m_s);
+        dfree(id); dfree(id_s); dfree(pcell); dfree(slot_box);
+        dfree(vp_s); dfree(up_s); dfree(acc_s); dfree(udot_s); dfree(rho); dfree(Q); dfree(acc);
+        dfree(jq); dfree(T); dfree(udot); dfree(e); dfree(S); dfree(gradv);
+        dfree(rf); dfree(S32); dfree(vf); dfree(rho32); dfree(Q32); dfree(T32);
+        dfree(keys_in); dfree(k_out); dfree(v_out); dfree(k_alt); dfree(v_alt);
+        if (sort_tmp) (void)hipFree(sort_tmp);
+        sort_tmp = nullptr;
+        dfree(cell_start); dfree(d_boxes); dfree(d_dyn); dfree(d_segs); dfree(d_stage); dfree(d_e3);
+        dfree(d_inv); dfree(d_cnt); dfree(d_off); dfree(d_grand); dfree(d_jbuf); dfree(d_iap);
+        if (scan_tmp) (void)hipFree(scan_tmp);
+        scan_tmp = nullptr;
+        if (h_dyn) (void)hipHostFree(h_dyn);
+        h_dyn = nullptr;
+        for (auto &ev : ev_pool) (void)hipEventDestroy(ev);
+        ev_pool.clear();
+    }
+
+    int ensure_tables(int nbox)
+    {
+        if (nbox > box_cap) {
+            int nc = std::max(nbox, box_cap * 2);
+            BoxDesc *nb = nullptr;
+            BoxDyn *nd = nullptr;
+            if (dalloc(&nb, nc) || dalloc(&nd, nc)) return SPH_ERR_NOMEM;
+            HIPCHK(hipMemsetAsync(nd, 0, sizeof(BoxDyn) * nc, ctx->stream));
+            if (box_cap > 0) {
+                HIPCHK(hipMemcpyAsync(nd, d_dyn, sizeof(BoxDyn) * box_cap, hipMemcpyDeviceToDevice, ctx->stream));
+            }
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            dfree(d_boxes);
+            dfree(d_dyn);
+            d_boxes = nb;
+            d_dyn = nd;
+            BoxDyn *nh = nullptr;
+            HIPCHK(hipHostMalloc((void **)&nh, sizeof(BoxDyn) * nc, hipHostMallocDefault));
+            if (h_dyn) (void)hipHostFree(h_dyn);
+            h_dyn = nh;
+            box_cap = nc;
+            invalidate_graph();
+        }
+        return SPH_OK;
+    }
+
+    void invalidate_graph()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_n = -1;
+    }
+
+    // Recompute key bases in slot order, the high-water mark, and upload the
+    // box and segment tables.
+    int relayout()
+    {
+        int key = 0;
+        hwm = 0;
+        for (auto &sg : segs) {
+            sg.key = key;
+            if (sg.box >= 0) {
+                BoxDesc &B = boxes[sg.box];
+                B.key_base = key;
+                B.part_off = sg.off;
+                key += B.ncx * B.ncy;
+                hwm = sg.off + sg.len;
+            } else {
+                key += 1;
+            }
+        }
+        nkeys = key;
+        if (nkeys + 1 > cell_cap) {
+            int nc = std::max(nkeys + 1, cell_cap * 2);
+            dfree(cell_start);
+            if (dalloc(&cell_start, nc)) return SPH_ERR_NOMEM;
+            cell_cap = nc;
+            invalidate_graph();
+        }
+        if ((int)segs.size() > seg_cap) {
+            int nc = std::max((int)segs.size(), seg_cap * 2);
+            dfree(d_segs);
+            if (dalloc(&d_segs, nc)) return SPH_ERR_NOMEM;
+            seg_cap = nc;
+            invalidate_graph();
+        }
+        if (ensure_tables((int)boxes.size())) return SPH_ERR_NOMEM;
+        if (!boxes.empty())
+            HIPCHK(hipMemcpyAsync(d_boxes, boxes.data(), sizeof(BoxDesc) * boxes.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_segs, segs.data(), sizeof(SegDesc) * segs.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+        // host-side vectors must outlive the async copies
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (graph && (graph_n != hwm || graph_keys != nkeys)) invalidate_graph();
+        // the segment count is baked into captured launches
+        invalidate_graph();
+        return SPH_OK;
+    }
+
+    int stage(size_t bytes)
+    {
+        if (bytes <= stage_bytes) return SPH_OK;
+        dfree(d_stage);
+        stage_bytes = 0;
+        if (dalloc((char **)&d_stage, bytes)) return SPH_ERR_NOMEM;
+        stage_bytes = bytes;
+        return SPH_OK;
+    }
+
+    // ---- timing ----------------------------------------------------------
+    int mark(int timer, bool begin)
+    {
+        if (!ctx->timing) return SPH_OK;
+        if (ev_used >= ev_pool.size()) {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            ev_pool.push_back(ev);
+        }
+        size_t idx = ev_used++;
+        HIPCHK(hipEventRecord(ev_pool[idx], ctx->stream));
+        if (begin) ev_marks.push_back({timer, {idx, (size_t)-1}});
+        else if (!ev_marks.empty()) ev_marks.back().second.second = idx;
+        return SPH_OK;
+    }
+
+    int collect_timing()
+    {
+        if (!ctx->timing || ev_marks.empty()) {
+            ev_used = 0;
+            ev_marks.clear();
+            return SPH_OK;
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (auto &mk : ev_marks) {
+            if (mk.second.second == (size_t)-1) continue;
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev_pool[mk.second.first], ev_pool[mk.second.second]));
+            t_acc[mk.first] += ms;
+        }
+        ev_marks.clear();
+        ev_used = 0;
+        return SPH_OK;
+    }
+
+    // ---- passes ----------------------------------------------------------
+    int build(int state_mask)
+    {
+        hipStream_t st = ctx->stream;
+        ArenaView A = view();
+        if (hwm == 0) return SPH_OK;
+        launch_keys(A, state_mask, st);
+        radix_sort_pairs(keys_in, k_out, v_out, k_alt, v_alt, hwm, bits_for(nkeys), sort_tmp, st);
+        launch_permute(A, fp32() ? 1 : 0, st);
+        launch_cell_start(A, st);
+        return SPH_OK;
+    }
+
+    // One arena step: every INIT box gets u from T0, every RUN box one
+    // leapfrog step; then per-box energy capture and state advance.
+    void enqueue_step()
+    {
+        hipStream_t st = ctx->stream;
+        ArenaView A = view();
+        const int f = fp32() ? 1 : 0;
+        mark(T_SORT, true);
+        build(MASK_STEP);
+        mark(T_SORT, false);
+        mark(T_DENS, true);
+        launch_density(A, f, MASK_STEP, st);
+        mark(T_DENS, false);
+        mark(T_GRAD, true);
+        launch_grad(A, f, MASK_RUN, 0, st);
+        mark(T_GRAD, false);
+        mark(T_RATES, true);
+        launch_rates(A, f, 1, st);
+        mark(T_RATES, false);
+        launch_box_energy(A, (int)boxes.size(), st);
+        launch_advance(d_dyn, (int)boxes.size(), st);
+    }
+
+    int step(int64_t nsteps)
+    {
+        if (hwm == 0) return SPH_OK;
+        const bool graphs = use_graph && !ctx->timing;
+        for (int64_t i = 0; i < nsteps; ++i) {
+            if (graphs) {
+                if (!graph) {
+                    hipGraph_t g;
+                    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                    enqueue_step();
+                    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+                    hipError_t e1 = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    if (e1 != hipSuccess) {
+                        graph = nullptr;
+                        return fail(SPH_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e1));
+                    }
+                    graph_n = hwm;
+                    graph_keys = nkeys;
+                }
+                HIPCHK(hipGraphLaunch(graph, ctx->stream));
+            } else {
+                enqueue_step();
+            }
+        }
+        HIPCHK(hipGetLastError());
+        steps_done += (double)nsteps;
+        return collect_timing();
+    }
+
+    int read_dyn()
+    {
+        if (boxes.empty()) return SPH_OK;
+        HIPCHK(hipMemcpyAsync(h_dyn, d_dyn, sizeof(BoxDyn) * boxes.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return SPH_OK;
+    }
+
+    int write_dyn(int b, const BoxDyn &D)
+    {
+        h_dyn[b] = D;
+        HIPCHK(hipMemcpyAsync(d_dyn + b, h_dyn + b, sizeof(BoxDyn), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return SPH_OK;
+    }
+
+    // ---- boxes -------------------------------------------------------------
+    int add_box(int64_t n, const sph_params *prm, int *box_out);
+    int remove_box(int b);
+    int upload(int b, int field, const double *host, int ncomp);
+    int download(int b, int field, double *host, int ncomp);
+};
+
+int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
+{
+    if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->h > 0) || !(p->hl > 0) || !(p->dt >= 0) || !(p->kB > 0) ||
+        !(p->rc >= 0) || !(p->tol >= 0) || !std::isfinite(p->Lx) || !std::isfinite(p->Ly))
+        return fail(SPH_ERR_ARG, "invalid sph_params: need Lx, Ly, h, hl, kB > 0, rc, tol, dt >= 0");
+    if (p->precision != SPH_FP64 && p->precision != SPH_FP32C)
+        return fail(SPH_ERR_ARG, "invalid precision");
+    memset(&B, 0, sizeof(B));
+    const double PI = 3.14159265358979323846;
+    B.Lx = p->Lx; B.Ly = p->Ly; B.hLx = p->Lx / 2.; B.hLy = p->Ly / 2.;
+    B.R2list = p->rc * p->rc + p->tol * p->tol;  // neighbour_list.py:156-157,178
+    const double Rc = std::sqrt(B.R2list) * (1.0 + 1e-9);
+    int ncx = (int)std::floor(p->Lx / Rc), ncy = (int)std::floor(p->Ly / Rc);
+    if (Rc <= 0) { ncx = 1; ncy = 1; }
+    if (ncx < 3) ncx = 1;
+    if (ncy < 3) ncy = 1;
+    if ((long long)ncx * ncy > (1ll << 26))
+        return fail(SPH_ERR_ARG, "box has too many cells for the cell list (> 2^26)");
+    B.ncx = ncx; B.ncy = ncy; B.sx = ncx >= 3 ? 1 : 0; B.sy = ncy >= 3 ? 1 : 0;
+    B.cwx = p->Lx / ncx; B.cwy = p->Ly / ncy; B.icwx = ncx / p->Lx; B.icwy = ncy / p->Ly;
+    B.h = p->h; B.hl = p->hl; B.ih = 1.0 / p->h; B.ihl = 1.0 / p->hl;
+    B.qh = 5.0 / (PI * p->h * p->h);        // spkernel.py:98
+    B.qhl = 5.0 / (PI * p->hl * p->hl);
+    B.fh = -12.0 * B.qh / (p->h * p->h);    // spkernel.py:112-113, factored
+    B.fhl = -12.0 * B.qhl / (p->hl * p->hl);
+    B.h2 = p->h * p->h; B.hl2 = p->hl * p->hl;
+    const double hm2 = std::max(B.h2, B.hl2);
+    B.cut2 = std::min(B.R2list, hm2);
+    B.a = p->a; B.b = p->b; B.kB = p->kB; B.eta = p->eta; B.zeta = p->zeta; B.kcond = p->kcond;
+    B.dt = p->dt;
+    B.n = (int)n;
+    B.terms = (int)(p->terms & SPH_ALL_TERMS);
+    B.precision = p->precision;
+    return SPH_OK;
+}
+
+int Arena::add_box(int64_t n, const sph_params *prm, int *box_out)
+{
+    if (n <= 0 || n > cap) return fail(SPH_ERR_ARG, "box size out of range");
+    BoxDesc B;
+    if (int rc = fill_desc(prm, n, B)) return rc;
+    if (prm->precision != precision) return fail(SPH_ERR_ARG, "box precision differs from the arena's");
+    // first fit over free segments
+    int si = -1;
+    for (size_t i = 0; i < segs.size(); ++i)
+        if (segs[i].box < 0 && segs[i].len >= n) { si = (int)i; break; }
+    if (si < 0) return fail(SPH_ERR_NOMEM, "arena has no free slot range of " + std::to_string(n));
+    int b = -1;
+    for (size_t i = 0; i < box_live.size(); ++i)
+        if (!box_live[i]) { b = (int)i; break; }
+    if (b < 0) {
+        b = (int)boxes.size();
+        boxes.push_back(B);
+        box_live.push_back(1);
+    } else {
+        boxes[b] = B;
+        box_live[b] = 1;
+    }
+    SegDesc fs = segs[si];
+    SegDesc bs{fs.off, (int)n, b, 0};
+    std::vector<SegDesc> ns(segs.begin(), segs.begin() + si);
+    ns.push_back(bs);
+    if (fs.len > n) ns.push_back(SegDesc{fs.off + (int)n, fs.len - (int)n, -1, 0});
+    ns.insert(ns.end(), segs.begin() + si + 1, segs.end());
+    segs.swap(ns);
+    if (int rc = relayout()) return rc;
+    BoxDyn D;
+    memset(&D, 0, sizeof(D));
+    D.state = BOX_DONE;
+    if (int rc = write_dyn(b, D)) return rc;
+    launch_init_ids(view(), boxes[b].part_off, (int)n, ctx->stream);
+    HIPCHK(hipGetLastError());
+    *box_out = b;
+    return SPH_OK;
+}
+
+int Arena::remove_box(int b)
+{
+    if (b < 0 || b >= (int)boxes.size() || !box_live[b]) return fail(SPH_ERR_ARG, "no such box");
+    for (auto &sg : segs)
+        if (sg.box == b) sg.box = -1;
+    // merge adjacent free segments
+    std::vector<SegDesc> ns;
+    for (auto &sg : segs) {
+        if (!ns.empty() && ns.back().box < 0 && sg.box < 0) ns.back().len += sg.len;
+        else ns.push_back(sg);
+    }
+    segs.swap(ns);
+    box_live[b] = 0;
+    boxes[b].n = 0;
+    BoxDyn D;
+    memset(&D, 0, sizeof(D));
+    D.state = BOX_FREE;
+    if (int rc = write_dyn(b, D)) return rc;
+    return relayout();
+}
+
+int Arena::upload(int b, int field, const double *host, int ncomp)
+{
+    const int n = boxes[b].n;
+    const size_t bytes = sizeof(double) * (size_t)n * ncomp;
+    if (stage(bytes)) return SPH_ERR_NOMEM;
+    HIPCHK(hipMemcpyAsync(d_stage, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    launch_upload(view(), boxes[b].part_off, n, field, d_stage, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+int Arena::download(int b, int field, double *host, int ncomp)
+{
+    const int n = boxes[b].n;
+    const size_t bytes = sizeof(double) * (size_t)n * ncomp;
+    if (stage(bytes)) return SPH_ERR_NOMEM;
+    launch_download(view(), b, n, field, d_stage, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, d_stage, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+int check_health(Arena &A, int b)
+{
+    if (int rc = A.read_dyn()) return rc;
+    const int h = A.h_dyn[b].health;
+    if (h == 0) return SPH_OK;
+    std::string msg = "numerical failure in box " + std::to_string(b) + ":";
+    if (h & HEALTH_NONFINITE) msg += " non-finite density/rates/state;";
+    if (h & HEALTH_EOS_POLE) msg += " vdW pole rho*b >= 1 reached;";
+    if (h & HEALTH_SKIN) msg += " cell skin exceeded;";
+    return fail(SPH_ERR_NUMERIC, msg);
+}
+
+}  // namespace
+
+struct sph_box {
+    Arena ar;
+    int b = 0;
+    sph_params prm;
+};
+
+struct sph_arena {
+    Arena ar;
+};
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+const char *sph_last_error(void) { return g_err.c_str(); }
+
+const char *sph_version(void) { return "pyticles_hip 0.1 (gfx950)"; }
+
+int sph_ctx_create(int device, sph_ctx **out)
+{
+    if (!out) return fail(SPH_ERR_ARG, "null out pointer");
+    *out = nullptr;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(SPH_ERR_ARG, "device " + std::to_string(device) + " not present (" + std::to_string(ndev) + " devices)");
+    HIPCHK(hipSetDevice(device));
+    sph_ctx *c = new sph_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(SPH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return SPH_OK;
+}
+
+int sph_ctx_destroy(sph_ctx *ctx)
+{
+    if (!ctx) return SPH_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    dfree(ctx->d_scratch);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SPH_OK;
+}
+
+int sph_ctx_set_timing(sph_ctx *ctx, int on)
+{
+    if (!ctx) return fail(SPH_ERR_ARG, "null context");
+    ctx->timing = on ? 1 : 0;
+    return SPH_OK;
+}
+
+// ---- one box ---------------------------------------------------------------
+
+int sph_box_create(sph_ctx *ctx, int64_t n, const sph_params *prm, sph_box **out)
+{
+    if (!ctx || !prm || !out) return fail(SPH_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (n <= 0 || n > (1ll << 30)) return fail(SPH_ERR_ARG, "n out of range");
+    HIPCHK(hipSetDevice(ctx->device));
+    sph_box *bx = new sph_box();
+    bx->ar.ctx = ctx;
+    bx->prm = *prm;
+    int rc = bx->ar.alloc((int)n, prm->precision);
+    if (!rc) rc = bx->ar.add_box(n, prm, &bx->b);
+    if (rc) {
+        std::string keep = g_err;
+        bx->ar.release();
+        delete bx;
+        g_err = keep;
+        return rc;
+    }
+    *out = bx;
+    return SPH_OK;
+}
+
+int sph_box_destroy(sph_box *box)
+{
+    if (!box) return SPH_OK;
+    (void)hipSetDevice(box->ar.ctx->device);
+    (void)hipStreamSynchronize(box->ar.ctx->stream);
+    box->ar.release();
+    delete box;
+    return SPH_OK;
+}
+
+int sph_box_set_params(sph_box *box, const sph_params *prm)
+{
+    if (!box || !prm) return fail(SPH_ERR_ARG, "null argument");
+    if (prm->precision != box->ar.precision) return fail(SPH_ERR_ARG, "precision cannot change");
+    BoxDesc B;
+    if (int rc = fill_desc(prm, box->ar.boxes[box->b].n, B)) return rc;
+    box->ar.boxes[box->b] = B;
+    box->prm = *prm;
+    return box->ar.relayout();
+}
+
+static int check_positions(const sph_box *box, const double *r)
+{
+    const int n = box->ar.boxes[box->b].n;
+    const double Lx = box->prm.Lx, Ly = box->prm.Ly;
+    for (int i = 0; i < n; ++i) {
+        const double x = r[2 * i], y = r[2 * i + 1];
+        if (!(x >= 0.0 && x < Lx && y >= 0.0 && y < Ly))
+            return fail(SPH_ERR_ARG, "position of particle " + std::to_string(i) +
+                                         " lies outside [0, L); wrap positions into the periodic box first");
+    }
+    return SPH_OK;
+}
+
+int sph_box_upload(sph_box *box, const double *r, const double *v, const double *m, const double *u)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    HIPCHK(hipSetDevice(box->ar.ctx->device));
+    if (r) {
+        if (int rc = check_positions(box, r)) return rc;
+        if (int rc = box->ar.upload(box->b, 0, r, 2)) return rc;
+    }
+    if (v) if (int rc = box->ar.upload(box->b, 1, v, 2)) return rc;
+    if (m) if (int rc = box->ar.upload(box->b, 2, m, 1)) return rc;
+    if (u) if (int rc = box->ar.upload(box->b, 3, u, 1)) return rc;
+    return SPH_OK;
+}
+
+int sph_box_download(sph_box *box, double *r, double *v, double *u, double *rho, double *rho_lr,
+                     double *T, double *p, double *pco, double *acc, double *udot, double *gradv,
+                     double *jq, double *P)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    HIPCHK(hipSetDevice(box->ar.ctx->device));
+    struct F { double *dst; int field, nc; } fs[] = {
+        {r, 0, 2}, {v, 1, 2}, {u, 3, 1}, {rho, 4, 1}, {rho_lr, 5, 1}, {T, 6, 1}, {p, 7, 1},
+        {pco, 8, 1}, {acc, 9, 2}, {udot, 10, 1}, {gradv, 11, 4}, {jq, 12, 2}, {P, 13, 4}};
+    for (auto &f : fs)
+        if (f.dst)
+            if (int rc = box->ar.download(box->b, f.field, f.dst, f.nc)) return rc;
+    return SPH_OK;
+}
+
+// ---- (a) neighbour list ---------------------------------------------------
+
+int sph_nlist_build(sph_box *box, int64_t cap, int64_t *iap, int64_t *nip_out)
+{
+    if (!box || !nip_out || cap < 0 || (cap > 0 && !iap)) return fail(SPH_ERR_ARG, "invalid argument");
+    Arena &A = box->ar;
+    hipStream_t st = A.ctx->stream;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    const int n = A.boxes[box->b].n;
+    A.mark(T_PAIRS, true);
+    A.build(MASK_ALL);
+    if (!A.d_inv) {
+        if (dalloc(&A.d_inv, A.cap) || dalloc(&A.d_cnt, A.cap) || dalloc(&A.d_off, A.cap) ||
+            dalloc(&A.d_grand, 1))
+            return SPH_ERR_NOMEM;
+        A.scan_tmp_bytes = exscan_temp_bytes(A.cap);
+        if (dalloc((char **)&A.scan_tmp, A.scan_tmp_bytes)) return SPH_ERR_NOMEM;
+    }
+    ArenaView V = A.view();
+    launch_inverse(A.id_s, A.boxes[box->b].part_off, n, A.d_inv, st);
+    launch_pairs(V, box->b, n, A.d_inv, A.d_cnt, nullptr, nullptr, 0, st);
+    exclusive_scan_u32(A.d_cnt, A.d_off, n, A.d_grand, A.scan_tmp, st);
+    unsigned grand = 0;
+    HIPCHK(hipMemcpyAsync(&grand, A.d_grand, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int64_t nip = grand;
+    if ((size_t)nip > A.jbuf_cap) {
+        dfree(A.d_jbuf);
+        A.jbuf_cap = 0;
+        if (dalloc(&A.d_jbuf, (size_t)nip + 1)) return SPH_ERR_NOMEM;
+        A.jbuf_cap = (size_t)nip + 1;
+    }
+    const int64_t nout = std::min<int64_t>(nip, cap);
+    if ((size_t)nout > A.iap_cap) {
+        dfree(A.d_iap);
+        A.iap_cap = 0;
+        if (dalloc(&A.d_iap, 2 * (size_t)nout + 2)) return SPH_ERR_NOMEM;
+        A.iap_cap = (size_t)nout;
+    }
+    launch_pairs(V, box->b, n, A.d_inv, A.d_cnt, A.d_off, A.d_jbuf, 1, st);
+    launch_pairs_emit(n, A.d_off, A.d_cnt, A.d_jbuf, A.d_iap, (long long)nout, st);
+    HIPCHK(hipGetLastError());
+    if (nout > 0)
+        HIPCHK(hipMemcpyAsync(iap, A.d_iap, sizeof(long long) * 2 * nout, hipMemcpyDeviceToHost, st));
+    A.mark(T_PAIRS, false);
+    HIPCHK(hipStreamSynchronize(st));
+    if (int rc = A.collect_timing()) return rc;
+    *nip_out = nip;
+    if (nip > cap)
+        return fail(SPH_ERR_CAP, "pair capacity " + std::to_string(cap) + " < " + std::to_string(nip) + " pairs");
+    return SPH_OK;
+}
+
+int sph_separations(sph_box *box, int64_t nip, const int64_t *iap, double *drij, double *rij,
+                    double *rsq, double *dv)
+{
+    if (!box || nip < 0 || (nip > 0 && !iap)) return fail(SPH_ERR_ARG, "invalid argument");
+    if (nip == 0) return SPH_OK;
+    Arena &A = box->ar;
+    hipStream_t st = A.ctx->stream;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    const int n = A.boxes[box->b].n;
+    for (int64_t k = 0; k < 2 * nip; ++k)
+        if (iap[k] < 0 || iap[k] >= n) return fail(SPH_ERR_ARG, "pair index out of range");
+    if (!A.d_inv && dalloc(&A.d_inv, A.cap)) return SPH_ERR_NOMEM;
+    // device layout: iap (2*nip int64) then up to 6*nip doubles of outputs
+    const size_t bytes = sizeof(long long) * 2 * nip + sizeof(double) * 6 * nip;
+    if (A.stage(bytes)) return SPH_ERR_NOMEM;
+    long long *d_iap = (long long *)A.d_stage;
+    double *o = (double *)(d_iap + 2 * nip);
+    double *d_drij = drij ? o : nullptr;
+    double *d_dv = dv ? o + 2 * nip : nullptr;
+    double *d_rij = rij ? o + 4 * nip : nullptr;
+    double *d_rsq = rsq ? o + 5 * nip : nullptr;
+    HIPCHK(hipMemcpyAsync(d_iap, iap, sizeof(long long) * 2 * nip, hipMemcpyHostToDevice, st));
+    launch_inverse(A.id, A.boxes[box->b].part_off, n, A.d_inv, st);
+    launch_separations(A.view(), box->b, A.d_inv, nip, d_iap, d_drij, d_rij, d_rsq, d_dv, st);
+    HIPCHK(hipGetLastError());
+    if (drij) HIPCHK(hipMemcpyAsync(drij, d_drij, sizeof(double) * 2 * nip, hipMemcpyDeviceToHost, st));
+    if (dv) HIPCHK(hipMemcpyAsync(dv, d_dv, sizeof(double) * 2 * nip, hipMemcpyDeviceToHost, st));
+    if (rij) HIPCHK(hipMemcpyAsync(rij, d_rij, sizeof(double) * nip, hipMemcpyDeviceToHost, st));
+    if (rsq) HIPCHK(hipMemcpyAsync(rsq, d_rsq, sizeof(double) * nip, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return SPH_OK;
+}
+
+int sph_minimum_image(sph_ctx *ctx, int64_t np, double *dr, double Lx, double Ly)
+{
+    if (!ctx || np < 0 || (np > 0 && !dr)) return fail(SPH_ERR_ARG, "invalid argument");
+    if (np == 0) return SPH_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(double) * 2 * np;
+    if (ctx->scratch(bytes)) return SPH_ERR_NOMEM;
+    HIPCHK(hipMemcpyAsync(ctx->d_scratch, dr, bytes, hipMemcpyHostToDevice, ctx->stream));
+    launch_min_image(np, ctx->d_scratch, Lx, Ly, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dr, ctx->d_scratch, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+// ---- (b) kernel, density, EOS ----------------------------------------------
+
+int sph_kernel_lucy(sph_ctx *ctx, int64_t np, const double *r, const double *dx, double h, int dim,
+                    double *w, double *dwdx)
+{
+    if (!ctx || np < 0 || dim < 1 || dim > 3 || !(h > 0) || (np > 0 && (!r || !dx || !w || !dwdx)))
+        return fail(SPH_ERR_ARG, "invalid argument (dim must be 1..3, h > 0)");
+    if (np == 0) return SPH_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t nb = sizeof(double) * np;
+    if (ctx->scratch(nb * (2 + 2 * dim))) return SPH_ERR_NOMEM;
+    double *d_r = ctx->d_scratch, *d_dx = d_r + np, *d_w = d_dx + np * dim, *d_g = d_w + np;
+    HIPCHK(hipMemcpyAsync(d_r, r, nb, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_dx, dx, nb * dim, hipMemcpyHostToDevice, ctx->stream));
+    launch_lucy(np, d_r, d_dx, h, dim, d_w, d_g, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(w, d_w, nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dwdx, d_g, nb * dim, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+// density (+ optional gradients / rates) of the current state, committed to
+// the canonical order so downloads see matching fields.
+static int eval_box(sph_box *box, int grad, int rates)
+{
+    Arena &A = box->ar;
+    hipStream_t st = A.ctx->stream;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    const int f = A.fp32() ? 1 : 0;
+    ArenaView V = A.view();
+    A.mark(T_SORT, true);
+    A.build(MASK_ALL);
+    A.mark(T_SORT, false);
+    A.mark(T_DENS, true);
+    launch_density(V, f, MASK_ALL, st);
+    A.mark(T_DENS, false);
+    if (grad) {
+        A.mark(T_GRAD, true);
+        launch_grad(V, f, MASK_ALL, 1, st);
+        A.mark(T_GRAD, false);
+    }
+    A.mark(T_RATES, true);
+    launch_rates(V, f, rates ? 0 : 1, st);  // mode 1 on an idle box = commit
+    A.mark(T_RATES, false);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    if (int rc = A.collect_timing()) return rc;
+    return check_health(A, box->b);
+}
+
+int sph_density(sph_box *box)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    return eval_box(box, 0, 0);
+}
+
+int sph_gradients(sph_box *box)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    return eval_box(box, 1, 0);
+}
+
+int sph_rates(sph_box *box, unsigned terms)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    Arena &A = box->ar;
+    BoxDesc &B = A.boxes[box->b];
+    const int keep = B.terms;
+    B.terms = (int)(terms & SPH_ALL_TERMS);
+    if (int rc = A.relayout()) return rc;
+    int rc = eval_box(box, (terms & (SPH_VISCOSITY | SPH_CONDUCTION)) ? 1 : 0, 1);
+    std::string msg = g_err;
+    B.terms = keep;
+    int rc2 = A.relayout();
+    if (rc) {
+        g_err = msg;
+        return rc;
+    }
+    return rc2;
+}
+
+int sph_init_thermal(sph_box *box, const double *T, double T0)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    Arena &A = box->ar;
+    const int n = A.boxes[box->b].n;
+    if (int rc = eval_box(box, 0, 0)) return rc;
+    double *dT = nullptr;
+    if (T) {
+        if (A.stage(sizeof(double) * n)) return SPH_ERR_NOMEM;
+        HIPCHK(hipMemcpyAsync(A.d_stage, T, sizeof(double) * n, hipMemcpyHostToDevice, A.ctx->stream));
+        dT = A.d_stage;
+    }
+    launch_set_u(A.view(), box->b, n, dT, T0, A.ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(A.ctx->stream));
+    return SPH_OK;
+}
+
+// ---- time stepping -----------------------------------------------------------
+
+int sph_step(sph_box *box, int64_t nsteps)
+{
+    if (!box || nsteps < 0) return fail(SPH_ERR_ARG, "invalid argument");
+    if (nsteps == 0) return SPH_OK;
+    Arena &A = box->ar;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    BoxDyn D;
+    memset(&D, 0, sizeof(D));
+    D.state = BOX_RUN;
+    D.k = 0;
+    D.nsteps = nsteps;
+    if (int rc = A.write_dyn(box->b, D)) return rc;
+    // nsteps leapfrog steps take nsteps + 1 rate evaluations
+    if (int rc = A.step(nsteps + 1)) return rc;
+    A.steps_done -= 1.0;
+    return check_health(A, box->b);
+}
+
+int sph_energy(sph_box *box, double *ke, double *ue, double *etot)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    Arena &A = box->ar;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    double out[3];
+    launch_energy_now(A.view(), box->b, A.d_e3, A.ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, A.d_e3, sizeof(out), hipMemcpyDeviceToHost, A.ctx->stream));
+    HIPCHK(hipStreamSynchronize(A.ctx->stream));
+    if (ke) *ke = out[0];
+    if (ue) *ue = out[1];
+    if (etot) *etot = out[2];
+    return SPH_OK;
+}
+
+int sph_box_timing(sph_box *box, double *out, int n, int reset)
+{
+    if (!box || n < 0 || n > 6 || (n > 0 && !out)) return fail(SPH_ERR_ARG, "invalid argument");
+    Arena &A = box->ar;
+    for (int i = 0; i < n; ++i) out[i] = i < T_N ? A.t_acc[i] : A.steps_done;
+    if (reset) {
+        for (double &t : A.t_acc) t = 0;
+        A.steps_done = 0;
+    }
+    return SPH_OK;
+}
+
+// ---- batching and serving ------------------------------------------------------
+
+int sph_arena_create(sph_ctx *ctx, int64_t particle_capacity, int precision, sph_arena **out)
+{
+    if (!ctx || !out || particle_capacity <= 0 || particle_capacity > (1ll << 30))
+        return fail(SPH_ERR_ARG, "invalid argument");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(ctx->device));
+    sph_arena *a = new sph_arena();
+    a->ar.ctx = ctx;
+    int rc = a->ar.alloc((int)particle_capacity, precision);
+    if (!rc) rc = a->ar.relayout();
+    if (rc) {
+        std::string keep = g_err;
+        a->ar.release();
+        delete a;
+        g_err = keep;
+        return rc;
+    }
+    *out = a;
+    return SPH_OK;
+}
+
+int sph_arena_destroy(sph_arena *ar)
+{
+    if (!ar) return SPH_OK;
+    (void)hipSetDevice(ar->ar.ctx->device);
+    (void)hipStreamSynchronize(ar->ar.ctx->stream);
+    ar->ar.release();
+    delete ar;
+    return SPH_OK;
+}
+
+int sph_arena_add(sph_arena *ar, int64_t n, const sph_params *prm, const double *r, const double *v,
+                  const double *m, double T0, int64_t nsteps, int *box_id)
+{
+    if (!ar || !prm || !r || !v || !m || !box_id || nsteps < 0) return fail(SPH_ERR_ARG, "invalid argument");
+    Arena &A = ar->ar;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    for (int64_t i = 0; i < n; ++i) {
+        const double x = r[2 * i], y = r[2 * i + 1];
+        if (!(x >= 0.0 && x < prm->Lx && y >= 0.0 && y < prm->Ly))
+            return fail(SPH_ERR_ARG, "position outside [0, L)");
+    }
+    int b = -1;
+    if (int rc = A.add_box(n, prm, &b)) return rc;
+    A.boxes[b].T0 = T0;
+    if (int rc = A.relayout()) return rc;
+    if (int rc = A.upload(b, 0, r, 2)) return rc;
+    if (int rc = A.upload(b, 1, v, 2)) return rc;
+    if (int rc = A.upload(b, 2, m, 1)) return rc;
+    BoxDyn D;
+    memset(&D, 0, sizeof(D));
+    D.state = BOX_INIT;
+    D.nsteps = nsteps;
+    if (int rc = A.write_dyn(b, D)) return rc;
+    *box_id = b;
+    return SPH_OK;
+}
+
+int sph_arena_step(sph_arena *ar, int64_t nsteps)
+{
+    if (!ar || nsteps < 0) return fail(SPH_ERR_ARG, "invalid argument");
+    HIPCHK(hipSetDevice(ar->ar.ctx->device));
+    return ar->ar.step(nsteps);
+}
+
+int sph_arena_poll(sph_arena *ar, int box_id, int64_t *steps_done, int *done)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    Arena &A = ar->ar;
+    if (box_id < 0 || box_id >= (int)A.boxes.size() || !A.box_live[box_id]) return fail(SPH_ERR_ARG, "no such box");
+    if (int rc = A.read_dyn()) return rc;
+    const BoxDyn &D = A.h_dyn[box_id];
+    if (steps_done) *steps_done = D.state == BOX_INIT ? -1 : (D.state == BOX_DONE ? D.nsteps : D.k);
+    if (done) *done = D.state == BOX_DONE ? 1 : 0;
+    if (D.health) return check_health(A, box_id);
+    return SPH_OK;
+}
+
+int sph_arena_poll_all(sph_arena *ar, int nmax, int *state, int64_t *k, int *health)
+{
+    if (!ar || nmax < 0) return fail(SPH_ERR_ARG, "invalid argument");
+    Arena &A = ar->ar;
+    if (int rc = A.read_dyn()) return rc;
+    const int nb = std::min(nmax, (int)A.boxes.size());
+    for (int b = 0; b < nb; ++b) {
+        if (state) state[b] = A.box_live[b] ? A.h_dyn[b].state : BOX_FREE;
+        if (k) k[b] = A.h_dyn[b].k;
+        if (health) health[b] = A.h_dyn[b].health;
+    }
+    return (int)A.boxes.size() > nmax ? fail(SPH_ERR_CAP, "more boxes than nmax") : SPH_OK;
+}
+
+int sph_arena_result(sph_arena *ar, int box_id, double *r, double *v, double *u, double *rho, double *e0,
+                     double *e1)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    Arena &A = ar->ar;
+    if (box_id < 0 || box_id >= (int)A.boxes.size() || !A.box_live[box_id]) return fail(SPH_ERR_ARG, "no such box");
+    HIPCHK(hipSetDevice(A.ctx->device));
+    if (r) if (int rc = A.download(box_id, 0, r, 2)) return rc;
+    if (v) if (int rc = A.download(box_id, 1, v, 2)) return rc;
+    if (u) if (int rc = A.download(box_id, 3, u, 1)) return rc;
+    if (rho) if (int rc = A.download(box_id, 4, rho, 1)) return rc;
+    if (e0 || e1) {
+        if (int rc = A.read_dyn()) return rc;
+        if (e0) *e0 = A.h_dyn[box_id].E0;
+        if (e1) *e1 = A.h_dyn[box_id].E1;
+    }
+    return SPH_OK;
+}
+
+int sph_arena_remove(sph_arena *ar, int box_id)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    HIPCHK(hipSetDevice(ar->ar.ctx->device));
+    return ar->ar.remove_box(box_id);
+}
+
+int sph_arena_stats(sph_arena *ar, int64_t *used, int64_t *capacity, int *running, int *boxes)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    Arena &A = ar->ar;
+    int64_t u = 0;
+    int nb = 0;
+    for (size_t b = 0; b < A.boxes.size(); ++b)
+        if (A.box_live[b]) {
+            u += A.boxes[b].n;
+            ++nb;
+        }
+    if (used) *used = u;
+    if (capacity) *capacity = A.cap;
+    if (boxes) *boxes = nb;
+    if (running) {
+        if (int rc = A.read_dyn()) return rc;
+        int nr = 0;
+        for (size_t b = 0; b < A.boxes.size(); ++b)
+            if (A.box_live[b] && (A.h_dyn[b].state == BOX_RUN || A.h_dyn[b].state == BOX_INIT)) ++nr;
+        *running = nr;
+    }
+    return SPH_OK;
+}
+
+int sph_arena_set_graph(sph_arena *ar, int on)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    ar->ar.use_graph = on ? 1 : 0;
+    ar->ar.invalidate_graph();
+    return SPH_OK;
+}
+
+int sph_box_set_graph(sph_box *box, int on)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    box->ar.use_graph = on ? 1 : 0;
+    box->ar.invalidate_graph();
+    return SPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1041 @@
+// sph_kernels.hip -- gfx950 kernels of the smooth-particle hot path.
+//
+// Every pass is a per-particle gather: work-item s owns sorted slot s and
+// walks the 3x3 periodic cell stencil around the cell it was binned into,
+// so each pair is evaluated from both ends and no float atomics are needed.
+// Slots are sorted by (box, cell), so the 64 lanes of a wavefront cover a
+// few adjacent cells and their candidate reads hit the same cache lines.
+//
+// Passes (reference functions they restate in brackets):
+//   P0 bin + stable radix sort + permute     [VerletList.build, neighbour_list.py:160-189]
+//   P1 density (h and h_l) + vdW EOS          [properties.py:63-120, f_properties.py:100-120]
+//   P2 velocity gradient + heat flux          [f_properties.py:111,130 (Fortran absent)]
+//   P3 pressure/cohesion/viscous/conduction   [forces.py:338-405, c_forces.pyx:96-237,
+//      rates + fused leapfrog kick/drift       spam_complete_force.py:158-165]
+#include <math.h>
+#include "sph_common.hpp"
+#include "sph_kernels.hpp"
+
+namespace sph {
+
+static constexpr int NT = 256;  // work-items per block (4 waves)
+
+// gfx950 dispatches block i to XCD (i mod 8).  Remap so that each XCD gets a
+// contiguous run of blocks: neighbouring slots (neighbouring cells) then share
+// that XCD's L2 instead of being spread over all eight.
+__device__ __forceinline__ int xcd_block()
+{
+    const int G = gridDim.x, pid = blockIdx.x;
+    const int x = pid & 7, local = pid >> 3;
+    const int per = G >> 3, rem = G & 7;
+    return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
+}
+
+__device__ __forceinline__ int wrapi(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
+
+// Minimum image exactly as neighbour_list.py:110-123 (all '>' tests, then '<').
+__device__ __forceinline__ void min_image(double &dx, double &dy, const BoxDesc &B)
+{
+    dx = dx > B.hLx ? dx - B.Lx : dx;
+    dy = dy > B.hLy ? dy - B.Ly : dy;
+    dx = dx < -B.hLx ? dx + B.Lx : dx;
+    dy = dy < -B.hLy ? dy + B.Ly : dy;
+}
+
+// rsq = dx*dx + dy*dy with no fma, so the strict neighbour test is bit-exact
+// against a plain IEEE evaluation of the reference's expression.
+__device__ __forceinline__ double rsq_exact(double dx, double dy)
+{
+#pragma clang fp contract(off)
+    const double xx = dx * dx;
+    const double yy = dy * dy;
+    return xx + yy;
+}
+
+__device__ __forceinline__ double wrap_coord(double x, double L)
+{
+    if (x >= L) x -= L;
+    else if (x < 0.0) x += L;
+    return x;
+}
+
+// Calls f(t0, t1, ox, oy) for the slot range of every stencil cell.
+template <class F>
+__device__ __forceinline__ void for_each_cell(const BoxDesc &B, const int *__restrict__ cs, int cell, F &&f)
+{
+    const int cx = cell % B.ncx, cy = cell / B.ncx;
+    for (int oy = -B.sy; oy <= B.sy; ++oy) {
+        const int yy = wrapi(cy + oy, B.ncy);
+        const int *row = cs + B.key_base + yy * B.ncx;
+        for (int ox = -B.sx; ox <= B.sx; ++ox) {
+            const int xx = wrapi(cx + ox, B.ncx);
+            f(row[xx], row[xx + 1], ox, oy);
+        }
+    }
+}
+
+__device__ __forceinline__ bool state_in(int st, int mask) { return ((1 << st) & mask) != 0; }
+
+__device__ __forceinline__ void flag_health(BoxDyn *dyn, int b, int bits) { atomicOr(&dyn[b].health, bits); }
+
+// ===========================================================================
+// P0  binning, permutation, cell starts
+// ===========================================================================
+
+// key = box key base + cell (row major); boxes outside state_mask and free
+// ranges keep one constant key, so the stable sort leaves them in place.
+__global__ __launch_bounds__(NT) void k_keys(ArenaView A, int state_mask)
+{
+    const int s = blockIdx.x * NT + threadIdx.x;
+    if (s >= A.n) return;
+    int lo = 0, hi = A.nseg - 1;
+    while (lo < hi) {  // last segment with off <= s
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.segs[mid].off <= s) lo = mid;
+        else hi = mid - 1;
+    }
+    const SegDesc sg = A.segs[lo];
+    int key = sg.key, box = -1;
+    if (sg.box >= 0) {
+        box = sg.box;
+        if (state_in(A.dyn[box].state, state_mask)) {
+            const BoxDesc &B = A.boxes[box];
+            const double2 r = A.r[s];
+            int cx = (int)(r.x * B.icwx), cy = (int)(r.y * B.icwy);
+            cx = cx < 0 ? 0 : (cx >= B.ncx ? B.ncx - 1 : cx);
+            cy = cy < 0 ? 0 : (cy >= B.ncy ? B.ncy - 1 : cy);
+            key = B.key_base + cy * B.ncx + cx;
+        }
+    }
+    A.slot_box[s] = box;
+    A.keys_in[s] = (unsigned)key;
+}
+
+// Gather the state into sorted order; FP32 mirrors for the compensated form.
+__global__ __launch_bounds__(NT) void k_permute(ArenaView A, int fp32)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int src = (int)A.vals[s];
+    const double2 r = A.r[src], v = A.v[src];
+    const double m = A.m[src], u = A.u[src];
+    A.r_s[s] = r;
+    A.v_s[s] = v;
+    A.u_s[s] = u;
+    A.m_s[s] = m;
+    A.id_s[s] = A.id[src];
+    const int b = A.slot_box[s];
+    int cell = 0;
+    double2 vp = v;
+    double up = u;
+    if (b >= 0) {
+        const BoxDesc &B = A.boxes[b];
+        const BoxDyn &D = A.dyn[b];
+        const double2 a = A.acc[src];
+        const double ud = A.udot[src];
+        if (D.state == BOX_RUN) {
+            if (D.k > 0) {  // predictor to the integer time: v_{k-1/2} + dt/2 a_{k-1}
+                const double hdt = 0.5 * B.dt;
+                vp = make_double2(v.x + hdt * a.x, v.y + hdt * a.y);
+                up = u + hdt * ud;
+            }
+        } else {
+            A.acc_s[s] = a;
+            A.udot_s[s] = ud;
+        }
+        cell = (int)A.keys[s] - B.key_base;
+        if (cell < 0 || cell >= B.ncx * B.ncy) cell = 0;  // frozen boxes
+        if (fp32) {
+            const int cx = cell % B.ncx, cy = cell / B.ncx;
+            double xl = r.x - cx * B.cwx, yl = r.y - cy * B.cwy;
+            A.rf[s] = make_float4((float)xl, (float)yl, (float)m, __int_as_float(cx | (cy << 16)));
+            A.vf[s] = make_float2((float)vp.x, (float)vp.y);
+        }
+    }
+    A.vp_s[s] = vp;
+    A.up_s[s] = up;
+    A.pcell[s] = cell;
+}
+
+// cell_start[k] = first sorted slot with key >= k, for k in [0, nkeys].
+__global__ __launch_bounds__(NT) void k_cell_start(ArenaView A)
+{
+    const int s = blockIdx.x * NT + threadIdx.x;
+    if (s > A.n) return;
+    if (A.n == 0) {
+        for (int c = 0; c <= A.nkeys; ++c) A.cell_start[c] = 0;
+        return;
+    }
+    if (s == A.n) {
+        for (int c = (int)A.keys[A.n - 1] + 1; c <= A.nkeys; ++c) A.cell_start[c] = A.n;
+        return;
+    }
+    const int k = (int)A.keys[s];
+    const int kp = s == 0 ? -1 : (int)A.keys[s - 1];
+    for (int c = kp + 1; c <= k; ++c) A.cell_start[c] = s;
+}
+
+// ===========================================================================
+// FP64 form
+// ===========================================================================
+
+__device__ __forceinline__ double vdw_p(double rho, double T, const BoxDesc &B)
+{
+    return (rho * B.kB * T) / (1.0 - rho * B.b);  // properties.py:41
+}
+
+// P1: rho_i = W(0,h) + sum_j m_j W(r_ij, h)  (self term not times m_i,
+// properties.py:77-78), rho_lr likewise with h_l; then T, p, pco.
+__global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    const int st = A.dyn[b].state;
+    if (!state_in(st, state_mask)) return;
+    const BoxDesc &B = A.boxes[b];
+    const double2 ri = A.r_s[s];
+    double rho = B.qh, rhol = B.qhl;
+    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int, int) {
+        for (int t = t0; t < t1; ++t) {
+            const double2 rj = A.r_s[t];
+            double dx = rj.x - ri.x, dy = rj.y - ri.y;
+            min_image(dx, dy, B);
+            const double r2 = rsq_exact(dx, dy);
+            if (r2 < B.cut2 && t != s) {
+                const double mj = A.m_s[t];
+                const double rr = sqrt(r2);
+                if (r2 < B.hl2) {
+                    const double q = rr * B.ihl, w = 1.0 - q;
+                    rhol += mj * (B.qhl * (1.0 + 3.0 * q) * (w * w * w));
+                }
+                if (r2 < B.h2) {
+                    const double q = rr * B.ih, w = 1.0 - q;
+                    rho += mj * (B.qh * (1.0 + 3.0 * q) * (w * w * w));
+                }
+            }
+        }
+    });
+    A.m[s] = A.m_s[s];
+    A.id[s] = A.id_s[s];
+    double ui = A.up_s[s];
+    if (st == BOX_INIT) {  // u = kB T0 - a rho (properties.py:44-46,119)
+        ui = B.kB * B.T0 - B.a * rho;
+        A.u_s[s] = ui;
+        A.up_s[s] = ui;
+    }
+    double T = (ui + B.a * rho) / B.kB;  // properties.py:48-49
+    T = T < 0.0 ? 0.0 : T;               // spam_complete_force.py:152
+    const double p = vdw_p(rho, T, B);
+    const double pco = -B.a * rhol * rhol;
+    A.rho[s] = make_double2(rho, rhol);
+    A.T[s] = T;
+    const double ps = (B.terms & 1u) ? (p + 0.0) / (rho * rho) : 0.0;
+    const double cs = (B.terms & 2u) ? pco / (rhol * rhol) : 0.0;
+    A.S[s] = make_double4(ps, 0.0, ps, cs);
+    A.Q[s] = make_double2(0.0, 0.0);
+    if (!(rho * B.b < 1.0)) flag_health(A.dyn, b, HEALTH_EOS_POLE);
+    if (!isfinite(rho) || !isfinite(rhol) || !isfinite(ui)) flag_health(A.dyn, b, HEALTH_NONFINITE);
+}
+
+// P2: grad v and grad T with grad_i W_ij = -dwdx_ij (dwdx from r_j - r_i);
+// viscous stress Pi = -eta (G + G^T - tr G I) - zeta tr G I (2-D) and heat
+// flux q = -k grad T.  diag != 0 also stores grad v and q for download.
+__global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int diag)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    if (!state_in(A.dyn[b].state, state_mask)) return;
+    const BoxDesc &B = A.boxes[b];
+    if (!diag && !(B.terms & 12u)) return;
+    const double2 ri = A.r_s[s], vi = A.vp_s[s];
+    const double Ti = A.T[s];
+    const double cut = B.R2list < B.h2 ? B.R2list : B.h2;
+    double g00 = 0, g01 = 0, g10 = 0, g11 = 0, gt0 = 0, gt1 = 0;
+    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int, int) {
+        for (int t = t0; t < t1; ++t) {
+            const double2 rj = A.r_s[t];
+            double dx = rj.x - ri.x, dy = rj.y - ri.y;
+            min_image(dx, dy, B);
+            const double r2 = rsq_exact(dx, dy);
+            if (r2 < cut && t != s) {
+                const double rr = sqrt(r2);
+                const double q = rr * B.ih, w = 1.0 - q;
+                const double F = B.fh * w * w;
+                const double gx = -F * dx, gy = -F * dy;
+                const double vj = A.m_s[t] / A.rho[t].x;
+                const double2 vv = A.vp_s[t];
+                const double dvx = vv.x - vi.x, dvy = vv.y - vi.y;
+                const double dT = A.T[t] - Ti;
+                g00 += vj * dvx * gx;
+                g01 += vj * dvx * gy;
+                g10 += vj * dvy * gx;
+                g11 += vj * dvy * gy;
+                gt0 += vj * dT * gx;
+                gt1 += vj * dT * gy;
+            }
+        }
+    });
+    const double rho = A.rho[s].x, r2i = rho * rho;
+    double pxx = 0, pxy = 0, pyy = 0;
+    if (B.terms & 4u) {
+        const double tr = g00 + g11;
+        pxx = -B.eta * (2.0 * g00 - tr) - B.zeta * tr;
+        pyy = -B.eta * (2.0 * g11 - tr) - B.zeta * tr;
+        pxy = -B.eta * (g01 + g10);
+        const double pr = (B.terms & 1u) ? vdw_p(rho, Ti, B) : 0.0;
+        const double4 S0 = A.S[s];
+        A.S[s] = make_double4((pr + pxx) / r2i, pxy / r2i, (pr + pyy) / r2i, S0.w);
+    }
+    double qx = 0, qy = 0;
+    if (B.terms & 8u) {
+        qx = -B.kcond * gt0;
+        qy = -B.kcond * gt1;
+        A.Q[s] = make_double2(qx / r2i, qy / r2i);
+    }
+    if (diag) {
+        A.gradv[s] = make_double4(g00, g01, g10, g11);
+        A.jq[s] = make_double2(qx, qy);
+    }
+}
+
+// P3: rates and the fused leapfrog.
+//   f   = (S_i + S_j) . dwdx_h + (Sco_i + Sco_j) dwdx_hl
+//   a_i = sum_j m_j f;   udot_i = sum_j m_j (dv . f / 2 + (Q_i + Q_j) . dwdx_h)
+// mode 0: rates only (store acc/udot, state unchanged);  mode 1: step --
+// running boxes kick by dt/2 at k = 0 and k = nsteps, by dt otherwise, and
+// drift (with periodic wrap) while k < nsteps; other boxes copy through.
+__global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int mode)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    const BoxDyn &D = A.dyn[b];
+    const double2 ri = A.r_s[s], vh = A.v_s[s], vi = A.vp_s[s];
+    const double uh = A.u_s[s];
+    if (mode == 1 && D.state != BOX_RUN) {
+        A.r[s] = ri;
+        A.v[s] = vh;
+        A.u[s] = uh;
+        A.acc[s] = A.acc_s[s];
+        A.udot[s] = A.udot_s[s];
+        return;
+    }
+    const BoxDesc &B = A.boxes[b];
+    const bool cond = (B.terms & 8u) != 0;
+    const double4 Si = A.S[s];
+    const double2 Qi = cond ? A.Q[s] : make_double2(0.0, 0.0);
+    double ax = 0, ay = 0, ud = 0;
+    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int, int) {
+        for (int t = t0; t < t1; ++t) {
+            const double2 rj = A.r_s[t];
+            double dx = rj.x - ri.x, dy = rj.y - ri.y;
+            min_image(dx, dy, B);
+            const double r2 = rsq_exact(dx, dy);
+            if (r2 < B.cut2 && t != s) {
+                const double rr = sqrt(r2);
+                const double4 Sj = A.S[t];
+                double fx = 0.0, fy = 0.0, c = 0.0;
+                if (r2 < B.h2) {
+                    const double q = rr * B.ih, w = 1.0 - q;
+                    const double F = B.fh * w * w;
+                    const double wx = F * dx, wy = F * dy;
+                    const double axx = Si.x + Sj.x, axy = Si.y + Sj.y, ayy = Si.z + Sj.z;
+                    fx = axx * wx + axy * wy;
+                    fy = axy * wx + ayy * wy;
+                    if (cond) {
+                        const double2 Qj = A.Q[t];
+                        c = (Qi.x + Qj.x) * wx + (Qi.y + Qj.y) * wy;
+                    }
+                }
+                if (r2 < B.hl2) {
+                    const double q = rr * B.ihl, w = 1.0 - q;
+                    const double Fl = B.fhl * w * w;
+                    const double cs = Si.w + Sj.w;
+                    fx = fx + cs * (Fl * dx);
+                    fy = fy + cs * (Fl * dy);
+                }
+                const double mj = A.m_s[t];
+                const double2 vv = A.vp_s[t];
+                const double e = 0.5 * ((vv.x - vi.x) * fx + (vv.y - vi.y) * fy);
+                ax += mj * fx;
+                ay += mj * fy;
+                ud += mj * (e + c);
+            }
+        }
+    });
+    if (!isfinite(ax) || !isfinite(ay) || !isfinite(ud)) flag_health(A.dyn, b, HEALTH_NONFINITE);
+    A.acc[s] = make_double2(ax, ay);
+    A.udot[s] = ud;
+    if (mode == 0) {
+        A.r[s] = ri;
+        A.v[s] = vh;
+        A.u[s] = uh;
+        return;
+    }
+    const int64_t k = D.k, K = D.nsteps;
+    const double kick = (k == 0 || k == K) ? 0.5 * B.dt : B.dt;
+    const double vx = vh.x + kick * ax, vy = vh.y + kick * ay;
+    const double un = uh + kick * ud;
+    const double mi = A.m_s[s];
+    if (k == 0) A.e[s] = mi * (0.5 * (vh.x * vh.x + vh.y * vh.y) + uh);
+    if (k == K) A.e[s] = mi * (0.5 * (vx * vx + vy * vy) + un);
+    double2 rn = ri;
+    if (k < K) {
+        rn.x = wrap_coord(ri.x + B.dt * vx, B.Lx);
+        rn.y = wrap_coord(ri.y + B.dt * vy, B.Ly);
+    }
+    A.r[s] = rn;
+    A.v[s] = make_double2(vx, vy);
+    A.u[s] = un;
+}
+
+// ===========================================================================
+// FP32-compensated form: pair arithmetic in FP32 on cell-relative
+// coordinates, Kahan-compensated sums, FP64 EOS / integration / energy.
+// ===========================================================================
+
+struct Kahan {
+    float s, c;
+    __device__ __forceinline__ Kahan() : s(0.f), c(0.f) {}
+    __device__ __forceinline__ explicit Kahan(float x) : s(x), c(0.f) {}
+    __device__ __forceinline__ void add(float x)
+    {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
+struct BoxF {
+    float cwx, cwy, Lx, Ly, hLx, hLy;
+    float cut2, h2, hl2, cuts;
+    float ih, ihl, qh, qhl, fh, fhl;
+};
+
+__device__ __forceinline__ BoxF boxf(const BoxDesc &B)
+{
+    BoxF f;
+    f.cwx = (float)B.cwx; f.cwy = (float)B.cwy;
+    f.Lx = (float)B.Lx; f.Ly = (float)B.Ly; f.hLx = (float)B.hLx; f.hLy = (float)B.hLy;
+    f.cut2 = (float)B.cut2; f.h2 = (float)B.h2; f.hl2 = (float)B.hl2;
+    f.cuts = (float)(B.R2list < B.h2 ? B.R2list : B.h2);
+    f.ih = (float)B.ih; f.ihl = (float)B.ihl; f.qh = (float)B.qh; f.qhl = (float)B.qhl;
+    f.fh = (float)B.fh; f.fhl = (float)B.fhl;
+    return f;
+}
+
+// Separation of candidate rj from ri (both cell-relative) in stencil cell
+// (ox, oy); boxes with a single cell along an axis use the minimum image.
+__device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, float ox, float oy,
+                                      const BoxDesc &B, const BoxF &F, float &dx, float &dy)
+{
+    dx = (rj.x - ri.x) + ox * F.cwx;
+    dy = (rj.y - ri.y) + oy * F.cwy;
+    if (B.sx == 0) {
+        dx = dx > F.hLx ? dx - F.Lx : dx;
+        dx = dx < -F.hLx ? dx + F.Lx : dx;
+    }
+    if (B.sy == 0) {
+        dy = dy > F.hLy ? dy - F.Ly : dy;
+        dy = dy < -F.hLy ? dy + F.Ly : dy;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    const int st = A.dyn[b].state;
+    if (!state_in(st, state_mask)) return;
+    const BoxDesc &B = A.boxes[b];
+    const BoxF F = boxf(B);
+    const float4 ri = A.rf[s];
+    Kahan rho(F.qh), rhol(F.qhl);
+    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int ox, int oy) {
+        for (int t = t0; t < t1; ++t) {
+            const float4 rj = A.rf[t];
+            float dx, dy;
+            sep32(ri, rj, (float)ox, (float)oy, B, F, dx, dy);
+            const float r2 = dx * dx + dy * dy;
+            if (r2 < F.cut2 && t != s) {
+                const float rr = sqrtf(r2);
+                if (r2 < F.hl2) {
+                    const float q = rr * F.ihl, w = 1.f - q;
+                    rhol.add(rj.z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
+                }
+                if (r2 < F.h2) {
+                    const float q = rr * F.ih, w = 1.f - q;
+                    rho.add(rj.z * (F.qh * (1.f + 3.f * q) * (w * w * w)));
+                }
+            }
+        }
+    });
+    A.m[s] = A.m_s[s];
+    A.id[s] = A.id_s[s];
+    const double rd = rho.value(), rl = rhol.value();
+    double ui = A.up_s[s];
+    if (st == BOX_INIT) {
+        ui = B.kB * B.T0 - B.a * rd;
+        A.u_s[s] = ui;
+        A.up_s[s] = ui;
+    }
+    double T = (ui + B.a * rd) / B.kB;
+    T = T < 0.0 ? 0.0 : T;
+    const double p = vdw_p(rd, T, B);
+    const double pco = -B.a * rl * rl;
+    A.rho[s] = make_double2(rd, rl);
+    A.T[s] = T;
+    A.rho32[s] = make_float2((float)rd, (float)rl);
+    A.T32[s] = (float)T;
+    const double ps = (B.terms & 1u) ? p / (rd * rd) : 0.0;
+    const double cs = (B.terms & 2u) ? pco / (rl * rl) : 0.0;
+    A.S[s] = make_double4(ps, 0.0, ps, cs);
+    A.S32[s] = make_float4((float)ps, 0.f, (float)ps, (float)cs);
+    A.Q[s] = make_double2(0.0, 0.0);
+    A.Q32[s] = make_float2(0.f, 0.f);
+    if (!(rd * B.b < 1.0)) flag_health(A.dyn, b, HEALTH_EOS_POLE);
+    if (!isfinite(rd) || !isfinite(rl) || !isfinite(ui)) flag_health(A.dyn, b, HEALTH_NONFINITE);
+}
+
+__global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int diag)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    if (!state_in(A.dyn[b].state, state_mask)) return;
+    const BoxDesc &B = A.boxes[b];
+    if (!diag && !(B.terms & 12u)) return;
+    const BoxF F = boxf(B);
+    const float4 ri = A.rf[s];
+    const float2 vi = A.vf[s];
+    const float Ti = A.T32[s];
+    Kahan g00, g01, g10, g11, gt0, gt1;
+    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int ox, int oy) {
+        for (int t = t0; t < t1; ++t) {
+            const float4 rj = A.rf[t];
+            float dx, dy;
+            sep32(ri, rj, (float)ox, (float)oy, B, F, dx, dy);
+            const float r2 = dx * dx + dy * dy;
+            if (r2 < F.cuts && t != s) {
+                const float rr = sqrtf(r2);
+                const float q = rr * F.ih, w = 1.f - q;
+                const float Fk = F.fh * w * w;
+                const float gx = -Fk * dx, gy = -Fk * dy;
+                const float vj = rj.z / A.rho32[t].x;
+                const float2 vv = A.vf[t];
+                const float dvx = vv.x - vi.x, dvy = vv.y - vi.y;
+                const float dT = A.T32[t] - Ti;
+                g00.add(vj * dvx * gx);
+                g01.add(vj * dvx * gy);
+                g10.add(vj * dvy * gx);
+                g11.add(vj * dvy * gy);
+                gt0.add(vj * dT * gx);
+                gt1.add(vj * dT * gy);
+            }
+        }
+    });
+    const double G00 = g00.value(), G01 = g01.value(), G10 = g10.value(), G11 = g11.value();
+    const double rho = A.rho[s].x, r2i = rho * rho;
+    double qx = 0, qy = 0;
+    if (B.terms & 4u) {
+        const double tr = G00 + G11;
+        const double pxx = -B.eta * (2.0 * G00 - tr) - B.zeta * tr;
+        const double pyy = -B.eta * (2.0 * G11 - tr) - B.zeta * tr;
+        const double pxy = -B.eta * (G01 + G10);
+        const double pr = (B.terms & 1u) ? vdw_p(rho, A.T[s], B) : 0.0;
+        const double4 S0 = A.S[s];
+        const double4 Sn = make_double4((pr + pxx) / r2i, pxy / r2i, (pr + pyy) / r2i, S0.w);
+        A.S[s] = Sn;
+        A.S32[s] = make_float4((float)Sn.x, (float)Sn.y, (float)Sn.z, (float)Sn.w);
+    }
+    if (B.terms & 8u) {
+        qx = -B.kcond * gt0.value();
+        qy = -B.kcond * gt1.value();
+        A.Q[s] = make_double2(qx / r2i, qy / r2i);
+        A.Q32[s] = make_float2((float)(qx / r2i), (float)(qy / r2i));
+    }
+    if (diag) {
+        A.gradv[s] = make_double4(G00, G01, G10, G11);
+        A.jq[s] = make_double2(qx, qy);
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int mode)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    const BoxDyn &D = A.dyn[b];
+    const double2 rid = A.r_s[s], vid = A.v_s[s];
+    const double ui = A.u_s[s];
+    if (mode == 1 && D.state != BOX_RUN) {
+        A.r[s] = rid;
+        A.v[s] = vid;
+        A.u[s] = ui;
+        A.acc[s] = A.acc_s[s];
+        A.udot[s] = A.udot_s[s];
+        return;
+    }
+    const BoxDesc &B = A.boxes[b];
+    const BoxF F = boxf(B);
+    const bool cond = (B.terms & 8u) != 0;
+    const float4 ri = A.rf[s];
+    const float2 vi = A.vf[s];
+    const float4 Si = A.S32[s];
+    const float2 Qi = cond ? A.Q32[s] : make_float2(0.f, 0.f);
+    Kahan ax, ay, ud;
+    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int ox, int oy) {
+        for (int t = t0; t < t1; ++t) {
+            const float4 rj = A.rf[t];
+            float dx, dy;
+            sep32(ri, rj, (float)ox, (float)oy, B, F, dx, dy);
+            const float r2 = dx * dx + dy * dy;
+            if (r2 < F.cut2 && t != s) {
+                const float rr = sqrtf(r2);
+                const float4 Sj = A.S32[t];
+                float fx = 0.f, fy = 0.f, c = 0.f;
+                if (r2 < F.h2) {
+                    const float q = rr * F.ih, w = 1.f - q;
+                    const float Fk = F.fh * w * w;
+                    const float wx = Fk * dx, wy = Fk * dy;
+                    const float axx = Si.x + Sj.x, axy = Si.y + Sj.y, ayy = Si.z + Sj.z;
+                    fx = axx * wx + axy * wy;
+                    fy = axy * wx + ayy * wy;
+                    if (cond) {
+                        const float2 Qj = A.Q32[t];
+                        c = (Qi.x + Qj.x) * wx + (Qi.y + Qj.y) * wy;
+                    }
+                }
+                if (r2 < F.hl2) {
+                    const float q = rr * F.ihl, w = 1.f - q;
+                    const float Fl = F.fhl * w * w;
+                    const float cs = Si.w + Sj.w;
+                    fx = fx + cs * (Fl * dx);
+                    fy = fy + cs * (Fl * dy);
+                }
+                const float mj = rj.z;
+                const float2 vv = A.vf[t];
+                const float e = 0.5f * ((vv.x - vi.x) * fx + (vv.y - vi.y) * fy);
+                ax.add(mj * fx);
+                ay.add(mj * fy);
+                ud.add(mj * (e + c));
+            }
+        }
+    });
+    const double axd = ax.value(), ayd = ay.value(), udd = ud.value();
+    if (!isfinite(axd) || !isfinite(ayd) || !isfinite(udd)) flag_health(A.dyn, b, HEALTH_NONFINITE);
+    A.acc[s] = make_double2(axd, ayd);
+    A.udot[s] = udd;
+    if (mode == 0) {
+        A.r[s] = rid;
+        A.v[s] = vid;
+        A.u[s] = ui;
+        return;
+    }
+    const int64_t k = D.k, K = D.nsteps;
+    const double kick = (k == 0 || k == K) ? 0.5 * B.dt : B.dt;
+    const double vx = vid.x + kick * axd, vy = vid.y + kick * ayd;
+    const double un = ui + kick * udd;
+    const double mi = A.m_s[s];
+    if (k == 0) A.e[s] = mi * (0.5 * (vid.x * vid.x + vid.y * vid.y) + ui);
+    if (k == K) A.e[s] = mi * (0.5 * (vx * vx + vy * vy) + un);
+    double2 rn = rid;
+    if (k < K) {
+        rn.x = wrap_coord(rid.x + B.dt * vx, B.Lx);
+        rn.y = wrap_coord(rid.y + B.dt * vy, B.Ly);
+    }
+    A.r[s] = rn;
+    A.v[s] = make_double2(vx, vy);
+    A.u[s] = un;
+}
+
+// ===========================================================================
+// per-box bookkeeping
+// ===========================================================================
+
+// Total energy sum m (v^2/2 + u) of boxes at their first (k = 0) or last
+// (k = nsteps) synchronised state; one block per box, fixed summation order.
+__global__ __launch_bounds__(NT) void k_box_energy(ArenaView A, int nbox)
+{
+    const int b = blockIdx.x;
+    if (b >= nbox) return;
+    BoxDyn &D = A.dyn[b];
+    if (D.state != BOX_RUN || !(D.k == 0 || D.k == D.nsteps)) return;
+    const BoxDesc &B = A.boxes[b];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < B.n; i += NT) acc += A.e[B.part_off + i];
+    __shared__ double red[NT];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (D.k == 0) D.E0 = red[0];
+        else D.E1 = red[0];
+    }
+}
+
+// INIT -> RUN (k = 0); RUN: k++ until k == nsteps, then DONE.
+__global__ void k_advance(BoxDyn *dyn, int nbox)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    BoxDyn &D = dyn[b];
+    if (D.state == BOX_INIT) {
+        D.state = D.nsteps > 0 ? BOX_RUN : BOX_DONE;
+        D.k = 0;
+    } else if (D.state == BOX_RUN) {
+        if (D.k >= D.nsteps) D.state = BOX_DONE;
+        else D.k += 1;
+    }
+}
+
+// ===========================================================================
+// transfers between caller order (local ids) and slots
+// ===========================================================================
+
+// field codes: 0 r, 1 v, 2 m, 3 u (upload); download adds 4 rho, 5 rho_lr,
+// 6 T, 7 p, 8 pco, 9 acc, 10 udot, 11 gradv, 12 jq, 13 P
+__global__ __launch_bounds__(NT) void k_upload(ArenaView A, int off, int n, int field,
+                                                const double *__restrict__ src)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    const int s = off + i;
+    const int k = A.id[s];
+    switch (field) {
+    case 0: A.r[s] = make_double2(src[2 * k], src[2 * k + 1]); break;
+    case 1: A.v[s] = make_double2(src[2 * k], src[2 * k + 1]); break;
+    case 2: A.m[s] = src[k]; break;
+    case 3: A.u[s] = src[k]; break;
+    default: break;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_init_ids(ArenaView A, int off, int n)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    A.id[off + i] = i;
+    A.id_s[off + i] = i;
+}
+
+__global__ __launch_bounds__(NT) void k_download(ArenaView A, int box, int field,
+                                                  double *__restrict__ dst)
+{
+    const BoxDesc &B = A.boxes[box];
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= B.n) return;
+    const int s = B.part_off + i;
+    const int k = A.id[s];
+    switch (field) {
+    case 0: { const double2 x = A.r[s]; dst[2 * k] = x.x; dst[2 * k + 1] = x.y; } break;
+    case 1: { const double2 x = A.v[s]; dst[2 * k] = x.x; dst[2 * k + 1] = x.y; } break;
+    case 2: dst[k] = A.m[s]; break;
+    case 3: dst[k] = A.u[s]; break;
+    case 4: dst[k] = A.rho[s].x; break;
+    case 5: dst[k] = A.rho[s].y; break;
+    case 6: dst[k] = A.T[s]; break;
+    case 7: dst[k] = vdw_p(A.rho[s].x, A.T[s], B); break;
+    case 8: { const double rl = A.rho[s].y; dst[k] = -B.a * rl * rl; } break;
+    case 9: { const double2 x = A.acc[s]; dst[2 * k] = x.x; dst[2 * k + 1] = x.y; } break;
+    case 10: dst[k] = A.udot[s]; break;
+    case 11: { const double4 g = A.gradv[s];
+               dst[4 * k] = g.x; dst[4 * k + 1] = g.y; dst[4 * k + 2] = g.z; dst[4 * k + 3] = g.w; } break;
+    case 12: { const double2 q = A.jq[s]; dst[2 * k] = q.x; dst[2 * k + 1] = q.y; } break;
+    case 13: {  // P = p_rev + p_rev_lr + pi_irr (spam_complete_force.py:178)
+        const double rho = A.rho[s].x, rl = A.rho[s].y, r2 = rho * rho;
+        const double4 S = A.S[s];
+        const double pco = (B.terms & 2u) ? -B.a * rl * rl : 0.0;
+        dst[4 * k] = S.x * r2 + pco;
+        dst[4 * k + 1] = S.y * r2;
+        dst[4 * k + 2] = S.y * r2;
+        dst[4 * k + 3] = S.z * r2 + pco;
+    } break;
+    default: break;
+    }
+}
+
+// ===========================================================================
+// (a) pair export, separations, kernel and minimum image on demand
+// ===========================================================================
+
+// inv[local id] = sorted slot
+__global__ __launch_bounds__(NT) void k_inverse(const int *__restrict__ ids, int off, int n,
+                                                 int *__restrict__ inv)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    inv[ids[off + i]] = off + i;
+}
+
+// Pairs (i, j), i < j, rsq < R2list: count per i (pass 0) or write the j's
+// of i into its segment, sorted ascending (pass 1).  i runs over local ids.
+__global__ __launch_bounds__(NT) void k_pairs(ArenaView A, int box, const int *__restrict__ inv,
+                                               unsigned *__restrict__ count,
+                                               const unsigned *__restrict__ offs,
+                                               int *__restrict__ jbuf, int pass)
+{
+    const BoxDesc &B = A.boxes[box];
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= B.n) return;
+    const int s = inv[i];
+    const double2 ri = A.r_s[s];
+    unsigned c = 0;
+    const unsigned o = pass ? offs[i] : 0u;
+    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int, int) {
+        for (int t = t0; t < t1; ++t) {
+            const int j = A.id_s[t];
+            if (j <= i) continue;
+            const double2 rj = A.r_s[t];
+            double dx = rj.x - ri.x, dy = rj.y - ri.y;
+            min_image(dx, dy, B);
+            if (rsq_exact(dx, dy) < B.R2list) {
+                if (pass) jbuf[o + c] = j;
+                ++c;
+            }
+        }
+    });
+    if (!pass) {
+        count[i] = c;
+        return;
+    }
+    int *seg = jbuf + o;  // insertion sort of this particle's partners
+    for (unsigned a = 1; a < c; ++a) {
+        const int x = seg[a];
+        int q = (int)a - 1;
+        while (q >= 0 && seg[q] > x) {
+            seg[q + 1] = seg[q];
+            --q;
+        }
+        seg[q + 1] = x;
+    }
+}
+
+// Expand (offsets, j-lists) into int64 iap[k][2] (first cap pairs).
+__global__ __launch_bounds__(NT) void k_pairs_emit(int n, const unsigned *__restrict__ offs,
+                                                    const unsigned *__restrict__ count,
+                                                    const int *__restrict__ jbuf,
+                                                    long long *__restrict__ iap, long long cap)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    const unsigned o = offs[i], c = count[i];
+    for (unsigned q = 0; q < c; ++q) {
+        const long long k = (long long)o + q;
+        if (k < cap) {
+            iap[2 * k] = i;
+            iap[2 * k + 1] = jbuf[o + q];
+        }
+    }
+}
+
+// neighbour_list.py:63-83 on a given pair list (local ids, canonical state).
+__global__ __launch_bounds__(NT) void k_separations(ArenaView A, int box, const int *__restrict__ inv,
+                                                     long long np, const long long *__restrict__ iap,
+                                                     double *__restrict__ drij, double *__restrict__ rij,
+                                                     double *__restrict__ rsq, double *__restrict__ dv)
+{
+    const long long k = (long long)blockIdx.x * NT + threadIdx.x;
+    if (k >= np) return;
+    const BoxDesc &B = A.boxes[box];
+    const int si = inv[iap[2 * k]], sj = inv[iap[2 * k + 1]];
+    const double2 ri = A.r[si], rj = A.r[sj];
+    double dx = rj.x - ri.x, dy = rj.y - ri.y;
+    min_image(dx, dy, B);
+    const double q = rsq_exact(dx, dy);
+    if (drij) { drij[2 * k] = dx; drij[2 * k + 1] = dy; }
+    if (rsq) rsq[k] = q;
+    if (rij) rij[k] = sqrt(q);
+    if (dv) {
+        const double2 vi = A.v[si], vj = A.v[sj];
+        dv[2 * k] = vj.x - vi.x;
+        dv[2 * k + 1] = vj.y - vi.y;
+    }
+}
+
+// spkernel.lucy_kernel (spkernel.py:86-118) for np pairs, dim 1..3; the
+// gradient uses the factored form dW/dr / r = -12 q (1 - r/h)^2 / h^2.
+__global__ __launch_bounds__(NT) void k_lucy(long long np, const double *__restrict__ r,
+                                              const double *__restrict__ dx, double h, int dim,
+                                              double *__restrict__ w, double *__restrict__ dwdx)
+{
+    const long long k = (long long)blockIdx.x * NT + threadIdx.x;
+    if (k >= np) return;
+    const double PI = 3.14159265358979323846;
+    const double q = dim == 1 ? 5.0 / (4.0 * h) : (dim == 2 ? 5.0 / (PI * h * h) : 105.0 / (PI * 16.0 * (h * h * h)));
+    const double rr = fabs(r[k]);
+    double W = 0.0, F = 0.0;
+    if (rr < h) {
+        const double s = rr / h, t = 1.0 - s;
+        W = q * (1.0 + 3.0 * s) * (t * t * t);
+        // dwdx is 0 at r == 0 whatever dx says (spkernel.py:108-109)
+        F = rr == 0.0 ? 0.0 : -12.0 * q * (t * t) / (h * h);
+    }
+    w[k] = W;
+    for (int c = 0; c < dim; ++c) dwdx[dim * k + c] = F * dx[dim * k + c];
+}
+
+__global__ __launch_bounds__(NT) void k_min_image(long long np, double *__restrict__ dr, double Lx, double Ly)
+{
+    const long long k = (long long)blockIdx.x * NT + threadIdx.x;
+    if (k >= np) return;
+    double x = dr[2 * k], y = dr[2 * k + 1];
+    x = x > Lx / 2. ? x - Lx : x;
+    y = y > Ly / 2. ? y - Ly : y;
+    x = x < -Lx / 2. ? x + Lx : x;
+    y = y < -Ly / 2. ? y + Ly : y;
+    dr[2 * k] = x;
+    dr[2 * k + 1] = y;
+}
+
+// u = kB T - a rho for one box (T in caller order, or T0 when T is null).
+__global__ __launch_bounds__(NT) void k_set_u(ArenaView A, int box, const double *__restrict__ T,
+                                               double T0)
+{
+    const BoxDesc &B = A.boxes[box];
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= B.n) return;
+    const int s = B.part_off + i;
+    const double t = T ? T[A.id[s]] : T0;
+    const double u = B.kB * t - B.a * A.rho[s].x;
+    A.u[s] = u;
+    A.u_s[s] = u;
+    A.up_s[s] = u;
+}
+
+// out3 = (sum m v^2/2, sum m u, total) over one box's canonical state.
+__global__ __launch_bounds__(NT) void k_energy_now(ArenaView A, int box, double *__restrict__ out3)
+{
+    const BoxDesc &B = A.boxes[box];
+    double ke = 0.0, ue = 0.0;
+    for (int i = threadIdx.x; i < B.n; i += NT) {
+        const int s = B.part_off + i;
+        const double2 v = A.v[s];
+        const double m = A.m[s];
+        ke += m * (0.5 * (v.x * v.x + v.y * v.y));
+        ue += m * A.u[s];
+    }
+    __shared__ double rk[NT], ru[NT];
+    rk[threadIdx.x] = ke;
+    ru[threadIdx.x] = ue;
+    __syncthreads();
+    for (int off = NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            rk[threadIdx.x] += rk[threadIdx.x + off];
+            ru[threadIdx.x] += ru[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out3[0] = rk[0];
+        out3[1] = ru[0];
+        out3[2] = rk[0] + ru[0];
+    }
+}
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+
+static inline dim3 grid_for(long long n) { return dim3((unsigned)((n + NT - 1) / NT > 0 ? (n + NT - 1) / NT : 1)); }
+
+void launch_keys(const ArenaView &A, int state_mask, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_keys, grid_for(A.n), dim3(NT), 0, st, A, state_mask);
+}
+void launch_permute(const ArenaView &A, int fp32, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_permute, grid_for(A.n), dim3(NT), 0, st, A, fp32);
+}
+void launch_cell_start(const ArenaView &A, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cell_start, grid_for((long long)A.n + 1), dim3(NT), 0, st, A);
+}
+void launch_density(const ArenaView &A, int fp32, int state_mask, hipStream_t st)
+{
+    if (fp32) hipLaunchKernelGGL(k_density32, grid_for(A.n), dim3(NT), 0, st, A, state_mask);
+    else hipLaunchKernelGGL(k_density64, grid_for(A.n), dim3(NT), 0, st, A, state_mask);
+}
+void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, hipStream_t st)
+{
+    if (fp32) hipLaunchKernelGGL(k_grad32, grid_for(A.n), dim3(NT), 0, st, A, state_mask, diag);
+    else hipLaunchKernelGGL(k_grad64, grid_for(A.n), dim3(NT), 0, st, A, state_mask, diag);
+}
+void launch_rates(const ArenaView &A, int fp32, int mode, hipStream_t st)
+{
+    if (fp32) hipLaunchKernelGGL(k_rates32, grid_for(A.n), dim3(NT), 0, st, A, mode);
+    else hipLaunchKernelGGL(k_rates64, grid_for(A.n), dim3(NT), 0, st, A, mode);
+}
+void launch_box_energy(const ArenaView &A, int nbox, hipStream_t st)
+{
+    if (nbox > 0) hipLaunchKernelGGL(k_box_energy, dim3(nbox), dim3(NT), 0, st, A, nbox);
+}
+void launch_advance(BoxDyn *dyn, int nbox, hipStream_t st)
+{
+    if (nbox > 0) hipLaunchKernelGGL(k_advance, grid_for(nbox), dim3(NT), 0, st, dyn, nbox);
+}
+void launch_upload(const ArenaView &A, int off, int n, int field, const double *src, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_upload, grid_for(n), dim3(NT), 0, st, A, off, n, field, src);
+}
+void launch_init_ids(const ArenaView &A, int off, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_init_ids, grid_for(n), dim3(NT), 0, st, A, off, n);
+}
+void launch_download(const ArenaView &A, int box, int n, int field, double *dst, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_download, grid_for(n), dim3(NT), 0, st, A, box, field, dst);
+}
+void launch_inverse(const int *ids, int off, int n, int *inv, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_inverse, grid_for(n), dim3(NT), 0, st, ids, off, n, inv);
+}
+void launch_pairs(const ArenaView &A, int box, int n, const int *inv, unsigned *count,
+                  const unsigned *offs, int *jbuf, int pass, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pairs, grid_for(n), dim3(NT), 0, st, A, box, inv, count, offs, jbuf, pass);
+}
+void launch_pairs_emit(int n, const unsigned *offs, const unsigned *count, const int *jbuf,
+                       long long *iap, long long cap, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pairs_emit, grid_for(n), dim3(NT), 0, st, n, offs, count, jbuf, iap, cap);
+}
+void launch_separations(const ArenaView &A, int box, const int *inv, long long np,
+                        const long long *iap, double *drij, double *rij, double *rsq, double *dv,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL(k_separations, grid_for(np), dim3(NT), 0, st, A, box, inv, np, iap, drij, rij, rsq, dv);
+}
+void launch_lucy(long long np, const double *r, const double *dx, double h, int dim, double *w,
+                 double *dwdx, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lucy, grid_for(np), dim3(NT), 0, st, np, r, dx, h, dim, w, dwdx);
+}
+void launch_min_image(long long np, double *dr, double Lx, double Ly, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_min_image, grid_for(np), dim3(NT), 0, st, np, dr, Lx, Ly);
+}
+void launch_set_u(const ArenaView &A, int box, int n, const double *T, double T0, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_set_u, grid_for(n), dim3(NT), 0, st, A, box, T, T0);
+}
+void launch_energy_now(const ArenaView &A, int box, double *out3, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_energy_now, dim3(1), dim3(NT), 0, st, A, box, out3);
+}
+
+}  // namespace sph

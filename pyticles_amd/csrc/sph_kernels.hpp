@@ -1,0 +1,39 @@
+// sph_kernels.hpp -- host launchers of the gfx950 kernels (sph_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "sph_common.hpp"
+
+namespace sph {
+
+// state masks for the box state filter of the passes
+constexpr int MASK_STEP = (1 << BOX_INIT) | (1 << BOX_RUN);
+constexpr int MASK_RUN = (1 << BOX_RUN);
+constexpr int MASK_INIT = (1 << BOX_INIT);
+constexpr int MASK_ALL = (1 << BOX_INIT) | (1 << BOX_RUN) | (1 << BOX_DONE);
+
+void launch_keys(const ArenaView &A, int state_mask, hipStream_t st);
+void launch_permute(const ArenaView &A, int fp32, hipStream_t st);
+void launch_cell_start(const ArenaView &A, hipStream_t st);
+void launch_density(const ArenaView &A, int fp32, int state_mask, hipStream_t st);
+void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, hipStream_t st);
+void launch_rates(const ArenaView &A, int fp32, int mode, hipStream_t st);
+void launch_box_energy(const ArenaView &A, int nbox, hipStream_t st);
+void launch_advance(BoxDyn *dyn, int nbox, hipStream_t st);
+void launch_upload(const ArenaView &A, int off, int n, int field, const double *src, hipStream_t st);
+void launch_init_ids(const ArenaView &A, int off, int n, hipStream_t st);
+void launch_download(const ArenaView &A, int box, int n, int field, double *dst, hipStream_t st);
+void launch_inverse(const int *ids, int off, int n, int *inv, hipStream_t st);
+void launch_pairs(const ArenaView &A, int box, int n, const int *inv, unsigned *count,
+                  const unsigned *offs, int *jbuf, int pass, hipStream_t st);
+void launch_pairs_emit(int n, const unsigned *offs, const unsigned *count, const int *jbuf,
+                       long long *iap, long long cap, hipStream_t st);
+void launch_separations(const ArenaView &A, int box, const int *inv, long long np,
+                        const long long *iap, double *drij, double *rij, double *rsq, double *dv,
+                        hipStream_t st);
+void launch_lucy(long long np, const double *r, const double *dx, double h, int dim, double *w,
+                 double *dwdx, hipStream_t st);
+void launch_min_image(long long np, double *dr, double Lx, double Ly, hipStream_t st);
+void launch_set_u(const ArenaView &A, int box, int n, const double *T, double T0, hipStream_t st);
+void launch_energy_now(const ArenaView &A, int box, double *out3, hipStream_t st);
+
+}  // namespace sph

@@ -1,0 +1,377 @@
+"""CPU checks that pin the FP64 oracle (oracle/oracle.c) before it is trusted
+as the checker of the HIP path: reference known answers, golden vectors from
+the line-by-line restatement (tests/golden/literal.py), closed forms,
+brute force and conservation laws.  Each check is chosen so that a dropped
+term, a sign or index slip or a transposed operand fails it."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from oracle import oracle as O
+from pyticles_amd import configs
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _prm(L, **kw):
+    p = configs.params(L)
+    p.update(kw)
+    return p
+
+
+def _jittered(n_side, L=None, seed=0, jit=0.1):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = float(n_side) if L is None else L
+    return configs.lattice(n_side * n_side, L, rng, jitter=jit), rng
+
+
+# ---------------------------------------------------------------- a1 / a2
+def test_min_image_reference_semantics():
+    # strict comparisons at exactly L/2 (neighbour_list.py:111-122)
+    dr = np.array([[6.0, -6.0], [5.0, -5.0], [5.0000001, 0.0], [-5.0000001, 4.9], [0.0, 10.0 - 1e-9]])
+    out = O.min_image(dr, 10.0, 10.0)
+    assert out[0].tolist() == [-4.0, 4.0]
+    assert out[1].tolist() == [5.0, -5.0]
+    assert out[2, 0] == 5.0000001 - 10.0
+    assert out[3, 0] == -5.0000001 + 10.0 and out[3, 1] == 4.9
+    assert out[4, 1] == (10.0 - 1e-9) - 10.0
+
+
+def test_reference_known_answer_unit_pair():
+    # test/neighbour_list_test.py:15-24 in 2-D: rij of pair (0,1) == 1.0
+    r = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    iap = O.pairs(r, 5.0, 5.0, 10.0 ** 2 + 2.0 * 2.0)
+    assert iap.tolist() == [[0, 1], [0, 2], [1, 2]]
+    drij, rij, rsq, dv = O.separations(r, np.zeros_like(r), 5.0, 5.0, iap)
+    assert rij[0] == 1.0
+    assert rij[2] == math.sqrt(2.0)
+
+
+@pytest.mark.parametrize("n_side,L,R2", [(20, 20.0, 17.0), (13, 13.0, 17.0), (9, 9.0, 17.0),
+                                         (7, 5.0, 17.0), (16, 16.0, 9.0), (10, 33.0, 17.0)])
+def test_cell_list_equals_brute_force(n_side, L, R2):
+    r, _ = _jittered(n_side, L=L, seed=n_side)
+    a = O.pairs(r, L, L, R2, method="cell")
+    b = O.pairs(r, L, L, R2, method="brute")
+    assert np.array_equal(a, b)
+    assert np.all(a[:, 0] < a[:, 1])
+    # lexicographic
+    key = a[:, 0] * r.shape[0] + a[:, 1]
+    assert np.all(np.diff(key) > 0)
+
+
+def test_pair_set_vectorised_independent():
+    # an independent NumPy enumeration of the same criterion
+    r, _ = _jittered(12, seed=3)
+    L = 12.0
+    n = r.shape[0]
+    i, j = np.triu_indices(n, 1)
+    d = r[j] - r[i]
+    d = np.where(d > L / 2, d - L, d)
+    d = np.where(d < -L / 2, d + L, d)
+    rsq = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]
+    keep = rsq < 17.0
+    ref = np.stack([i[keep], j[keep]], axis=1)
+    assert np.array_equal(O.pairs(r, L, L, 17.0), ref)
+
+
+def test_lattice_ties_excluded():
+    # unit lattice: 48 neighbours with r^2 < 17 (the 8 at exactly 17 excluded),
+    # 44 with r^2 < 16; nip = 24 N and 22 N
+    gold = json.load(open(os.path.join(GOLD, "lattice.json")))
+    assert gold["neighbours_rsq_lt_17"] == 48 and gold["neighbours_rsq_lt_16"] == 44
+    for ns in (12, 16):
+        r = configs.lattice(ns * ns, float(ns), None, jitter=0)
+        assert O.pairs(r, ns, ns, 17.0).shape[0] == 24 * ns * ns
+        assert O.pairs(r, ns, ns, 16.0).shape[0] == 22 * ns * ns
+
+
+# ---------------------------------------------------------------- b1 Lucy
+@pytest.mark.parametrize("h,w0", [(2.0, 0.3978873577297384), (4.0, 0.0994718394324346)])
+def test_lucy_self_value(h, w0):
+    w, g = O.lucy(np.array([0.0]), np.zeros((1, 2)), h, 2)
+    assert w[0] == pytest.approx(w0, rel=1e-15)
+    assert w[0] == pytest.approx(5.0 / (math.pi * h * h), rel=1e-15)
+    assert g.tolist() == [[0.0, 0.0]]
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_lucy_normalised(dim):
+    h = 2.0
+
+    def W(r):
+        return O.lucy(np.array([r]), np.zeros((1, dim)), h, dim)[0][0]
+
+    if dim == 1:
+        val = 2.0 * integrate.quad(W, 0, h, epsabs=1e-14)[0]
+    elif dim == 2:
+        val = integrate.quad(lambda r: 2 * math.pi * r * W(r), 0, h, epsabs=1e-14)[0]
+    else:
+        val = integrate.quad(lambda r: 4 * math.pi * r * r * W(r), 0, h, epsabs=1e-14)[0]
+    assert val == pytest.approx(1.0, abs=1e-10)
+
+
+def test_lucy_gradient_finite_difference_and_support():
+    h = 2.0
+    rng = np.random.Generator(np.random.PCG64(7))
+    rs = rng.uniform(0.05, 1.95, size=50)
+    ang = rng.uniform(0, 2 * math.pi, size=50)
+    dx = np.stack([rs * np.cos(ang), rs * np.sin(ang)], axis=1)
+    w, g = O.lucy(rs, dx, h, 2)
+    eps = 1e-6
+    wp, _ = O.lucy(rs + eps, dx, h, 2)
+    wm, _ = O.lucy(rs - eps, dx, h, 2)
+    dWdr = (wp - wm) / (2 * eps)
+    np.testing.assert_allclose(g, (dWdr / rs)[:, None] * dx, rtol=1e-6, atol=1e-9)
+    # compact support and smooth edge: W(h) = dW/dr(h) = 0
+    w, g = O.lucy(np.array([h, h + 0.5]), np.array([[h, 0.0], [h + 0.5, 0.0]]), h, 2)
+    assert w.tolist() == [0.0, 0.0] and np.all(g == 0.0)
+    w, g = O.lucy(np.array([h * (1 - 1e-6)]), np.array([[h * (1 - 1e-6), 0.0]]), h, 2)
+    assert abs(g[0, 0]) < 1e-10
+
+
+# ---------------------------------------------------------------- golden vectors
+def test_oracle_matches_literal_restatement():
+    g = dict(np.load(os.path.join(GOLD, "literal_n64.npz")))
+    L = float(g["L"])
+    iap = O.pairs(g["r"], L, L, 17.0)
+    assert np.array_equal(iap, g["iap"])
+    drij, rij, rsq, dv = O.separations(g["r"], g["v"], L, L, iap)
+    np.testing.assert_array_equal(drij, g["drij"])
+    np.testing.assert_array_equal(dv, g["dv"])
+    np.testing.assert_allclose(rij, g["rij"], rtol=0, atol=0)
+    rho, rho_lr = O.density(g["m"], iap, rij, 2.0, 4.0)
+    np.testing.assert_allclose(rho, g["rho"], rtol=1e-14)
+    np.testing.assert_allclose(rho_lr, g["rho_lr"], rtol=1e-14)
+    T, p, pco = O.eos(rho, rho_lr, g["u"], 2.0, 0.5, 1.0)
+    np.testing.assert_allclose(T, g["T"], rtol=1e-13)
+    np.testing.assert_allclose(p, g["p"], rtol=1e-13)
+    np.testing.assert_allclose(pco, g["pco"], rtol=1e-14)
+    prm = _prm(L, eta=0.0, kcond=0.0)
+    d = O.derivatives(g["r"], g["v"], g["m"], g["u"], prm, terms=O.PRESSURE | O.COHESION)
+    sc = np.max(np.abs(g["vdot"]))
+    np.testing.assert_allclose(d["acc"], g["vdot"], rtol=0, atol=1e-12 * sc)
+    np.testing.assert_allclose(d["udot"], g["udot"], rtol=0, atol=1e-12 * np.max(np.abs(g["udot"])))
+
+
+# ---------------------------------------------------------------- b3 density, b4 EOS
+def test_lattice_density_closed_form():
+    gold = json.load(open(os.path.join(GOLD, "lattice.json")))
+    ns = 16
+    r = configs.lattice(ns * ns, float(ns), None, jitter=0)
+    m = np.ones(ns * ns)
+    d = O.derivatives(r, np.zeros_like(r), m, np.zeros(ns * ns), _prm(ns), terms=0)
+    np.testing.assert_allclose(d["rho"], 1.02006720065032, rtol=1e-14)
+    np.testing.assert_allclose(d["rho_lr"], 1.0006325056255552, rtol=1e-14)
+    np.testing.assert_allclose(d["rho"], gold["rho_h2"], rtol=1e-14)
+    np.testing.assert_allclose(d["rho_lr"], gold["rho_h4"], rtol=1e-14)
+
+
+def test_density_brute_fsum_and_invariance():
+    r, rng = _jittered(10, seed=11)
+    L = 10.0
+    n = r.shape[0]
+    m = rng.uniform(0.5, 1.5, size=n)
+    d = O.derivatives(r, np.zeros_like(r), m, np.zeros(n), _prm(L), terms=0)
+    # brute force with exact summation (all pairs, min image, r < h)
+    for h, key in ((2.0, "rho"), (4.0, "rho_lr")):
+        q = 5.0 / (math.pi * h * h)
+        for i in range(0, n, 7):
+            terms = [q]
+            for j in range(n):
+                if j == i:
+                    continue
+                dx = r[j] - r[i]
+                dx = np.where(dx > L / 2, dx - L, dx)
+                dx = np.where(dx < -L / 2, dx + L, dx)
+                rr = math.sqrt(dx[0] ** 2 + dx[1] ** 2)
+                if rr < h:
+                    s = rr / h
+                    terms.append(m[j] * q * (1 + 3 * s) * (1 - s) ** 3)
+            assert d[key][i] == pytest.approx(math.fsum(terms), rel=1e-14)
+    # translation (with wrap) and permutation invariance
+    shift = np.mod(r + np.array([3.3, -1.7]), L)
+    d2 = O.derivatives(shift, np.zeros_like(r), m, np.zeros(n), _prm(L), terms=0)
+    np.testing.assert_allclose(d2["rho"], d["rho"], rtol=1e-13)
+    perm = rng.permutation(n)
+    d3 = O.derivatives(r[perm], np.zeros_like(r), m[perm], np.zeros(n), _prm(L), terms=0)
+    np.testing.assert_allclose(d3["rho_lr"], d["rho_lr"][perm], rtol=1e-14)
+
+
+def test_eos_identities():
+    # vdw_temp(rho, vdw_energy(rho, T)) == T (properties.py:190-196); p=2, pco=-2 at rho=1, T=1
+    rho = np.array([1.0, 0.7, 1.3])
+    T = np.array([1.0, 0.4, 2.5])
+    u = 1.0 * T - 2.0 * rho
+    Tb, p, pco = O.eos(rho, rho, u, 2.0, 0.5, 1.0)
+    np.testing.assert_allclose(Tb, T, rtol=1e-15)
+    assert p[0] == 2.0 and pco[0] == -2.0
+    # temperature clamp at zero (spam_complete_force.py:152)
+    Tb, p, _ = O.eos(np.array([1.0]), np.array([1.0]), np.array([-5.0]), 2.0, 0.5, 1.0)
+    assert Tb[0] == 0.0 and p[0] == 0.0
+
+
+# ---------------------------------------------------------------- c1-c4 rates
+def test_two_particle_analytic():
+    # geometry of test/force_test.py:15-25 in 2-D; every term by hand
+    L = 20.0
+    r = np.array([[5.0, 5.0], [6.0, 5.0]])
+    v = np.array([[0.3, -0.1], [-0.2, 0.4]])
+    m = np.array([1.0, 1.0])
+    u = np.array([-0.5, -0.7])
+    h, hl = 2.0, 4.0
+
+    def lucy_r(rr, hh):
+        q = 5.0 / (math.pi * hh * hh)
+        W = q * (1 + 3 * rr / hh) * (1 - rr / hh) ** 3
+        dWdr = q * (-12 * rr ** 3 / hh ** 4 + 24 * rr ** 2 / hh ** 3 - 12 * rr / hh ** 2)
+        return q, W, dWdr
+
+    q, W, dW = lucy_r(1.0, h)
+    ql, Wl, dWl = lucy_r(1.0, hl)
+    rho = q + W
+    rl = ql + Wl
+    T = (u + 2.0 * rho) / 1.0
+    p = rho * T / (1 - 0.5 * rho)
+    pco = -2.0 * rl * rl
+    d = O.derivatives(r, v, m, u, _prm(L, eta=0.0, kcond=0.0), terms=O.PRESSURE | O.COHESION)
+    np.testing.assert_allclose(d["rho"], [rho, rho], rtol=1e-15)
+    np.testing.assert_allclose(d["rho_lr"], [rl, rl], rtol=1e-15)
+    # dwdx from r_j - r_i = (+1, 0) for particle 0
+    f = (p[0] / rho ** 2 + p[1] / rho ** 2) * dW + (pco / rl ** 2 + pco / rl ** 2) * dWl
+    np.testing.assert_allclose(d["acc"][0], [f, 0.0], rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(d["acc"][1], [-f, 0.0], rtol=1e-13, atol=1e-15)
+    dv = v[1] - v[0]
+    e = 0.5 * dv[0] * f
+    np.testing.assert_allclose(d["udot"], [e, e], rtol=1e-13)
+    # repulsion when the short pressure dominates: particle 0 pushed to -x
+    assert (p[0] > -pco) == (d["acc"][0, 0] < 0)
+
+
+def _random_state(ns=10, seed=5, T=0.9):
+    r, rng = _jittered(ns, seed=seed, jit=0.15)
+    n = r.shape[0]
+    v = rng.normal(0, math.sqrt(T), size=(n, 2))
+    m = rng.uniform(0.8, 1.2, size=n)
+    u = rng.uniform(-1.5, -0.8, size=n)
+    return r, v, m, u, float(ns)
+
+
+@pytest.mark.parametrize("terms", [3, 7, 11, 15])
+def test_momentum_and_energy_conservation(terms):
+    r, v, m, u, L = _random_state()
+    d = O.derivatives(r, v, m, u, _prm(L), terms=terms)
+    mom = (m[:, None] * d["acc"]).sum(axis=0)
+    scale = np.sum(np.abs(m[:, None] * d["acc"]))
+    assert np.all(np.abs(mom) < 1e-13 * scale)
+    power = np.sum(m[:, None] * v * d["acc"]) + np.sum(m * d["udot"])
+    pscale = np.sum(np.abs(m[:, None] * v * d["acc"])) + np.sum(np.abs(m * d["udot"]))
+    assert abs(power) < 1e-13 * pscale
+
+
+def test_lattice_forces_vanish():
+    ns = 12
+    r = configs.lattice(ns * ns, float(ns), None, jitter=0)
+    n = ns * ns
+    d = O.derivatives(r, np.zeros_like(r), np.ones(n), np.full(n, -1.0), _prm(ns), terms=15)
+    assert np.max(np.abs(d["acc"])) < 1e-13
+    assert np.max(np.abs(d["udot"])) < 1e-13
+
+
+def _lattice_c(h=2.0):
+    """c = sum_j V_j (r_j - r_i)_y (grad_i W)_y over the unit lattice (V = 1/rho)."""
+    q = 5.0 / (math.pi * h * h)
+    rho = math.fsum([q] + [q * (1 + 3 * s) * (1 - s) ** 3 for s in
+                           [math.sqrt(i * i + j * j) / h for i in range(-3, 4) for j in range(-3, 4)
+                            if (i, j) != (0, 0) and i * i + j * j < h * h]])
+    terms = []
+    for i in range(-3, 4):
+        for j in range(-3, 4):
+            rr = math.sqrt(i * i + j * j)
+            if (i, j) == (0, 0) or rr >= h:
+                continue
+            F = -12 * q * (1 - rr / h) ** 2 / (h * h)
+            terms.append((1.0 / rho) * j * (-F * j))
+    return math.fsum(terms)
+
+
+def test_viscous_shear_flow():
+    # v = (gamma y, 0): bulk grad v[0][1] = gamma c, Pi_xy = -eta gamma c
+    ns, gamma, eta = 24, 0.01, 1.0
+    r = configs.lattice(ns * ns, float(ns), None, jitter=0)
+    n = ns * ns
+    v = np.zeros((n, 2))
+    v[:, 0] = gamma * (r[:, 1] - ns / 2)
+    d = O.derivatives(r, v, np.ones(n), np.full(n, -1.0), _prm(ns, eta=eta, zeta=0.0, kcond=0.0), terms=15)
+    c = _lattice_c()
+    bulk = np.abs(r[:, 1] - ns / 2) < ns / 2 - 3
+    np.testing.assert_allclose(d["gradv"][bulk, 0, 1], gamma * c, rtol=1e-11)
+    np.testing.assert_allclose(d["gradv"][bulk, 1, 0], 0.0, atol=1e-15)
+    Pxy = d["P"][bulk, 0, 1]
+    np.testing.assert_allclose(Pxy, -eta * gamma * c, rtol=1e-11)
+    assert abs(c - 1.0) < 0.05
+    # uniform velocity and rigid rotation: no viscous stress
+    v2 = np.tile([0.3, -0.2], (n, 1))
+    d2 = O.derivatives(r, v2, np.ones(n), np.full(n, -1.0), _prm(ns, kcond=0.0), terms=15)
+    assert np.max(np.abs(d2["gradv"])) < 1e-14
+    w = 0.01
+    v3 = np.stack([-w * (r[:, 1] - ns / 2), w * (r[:, 0] - ns / 2)], axis=1)
+    d3 = O.derivatives(r, v3, np.ones(n), np.full(n, -1.0), _prm(ns, zeta=0.3, kcond=0.0), terms=15)
+    inner = (np.abs(r[:, 1] - ns / 2) < ns / 2 - 3) & (np.abs(r[:, 0] - ns / 2) < ns / 2 - 3)
+    P = d3["P"][inner]
+    assert np.max(np.abs(P[:, 0, 1])) < 1e-14
+    np.testing.assert_allclose(P[:, 0, 0], P[:, 1, 1], rtol=1e-13)
+
+
+def test_conduction_hot_spot_decays():
+    ns = 20
+    r = configs.lattice(ns * ns, float(ns), None, jitter=0)
+    n = ns * ns
+    rho0 = json.load(open(os.path.join(GOLD, "lattice.json")))["rho_h2"]
+    d2c = np.sum((r - ns / 2) ** 2, axis=1)
+    T = 0.5 + 0.3 * np.exp(-d2c / 8.0)
+    u = T - 2.0 * rho0
+    prm = _prm(ns, eta=0.0, kcond=5.0)
+    d = O.derivatives(r, np.zeros_like(r), np.ones(n), u, prm, terms=O.CONDUCTION)
+    np.testing.assert_allclose(d["T"], T, rtol=1e-13)
+    ud = d["udot"]
+    assert abs(ud.sum()) < 1e-13 * np.abs(ud).sum()
+    hot = np.argmin(d2c)
+    assert ud[hot] < 0 and ud[hot] == ud.min()
+    # dVar(T)/dt = 2 sum (T - Tbar) udot / kB < 0
+    assert np.sum((T - T.mean()) * ud) < 0
+    # q = -k grad T points away from the hot spot
+    q = d["jq"]
+    out = np.sum(q * (r - ns / 2), axis=1)
+    ring = (d2c > 2) & (d2c < 30)
+    assert np.all(out[ring] > 0)
+    # uniform T: no flux
+    d0 = O.derivatives(r, np.zeros_like(r), np.ones(n), np.full(n, 0.5 - 2 * rho0), prm, terms=O.CONDUCTION)
+    assert np.max(np.abs(d0["jq"])) < 1e-13 and np.max(np.abs(d0["udot"])) < 1e-13
+
+
+# ---------------------------------------------------------------- leapfrog
+def test_leapfrog_second_order_energy():
+    s = configs.square_system(16, 0.9, 2, 0)
+    u = O.init_u(s.r, s.m, s.T, s.prm)
+    drift = []
+    for dt in (0.004, 0.002):
+        p = dict(s.prm, dt=dt)
+        _, _, _, E = O.run(s.r, s.v, s.m, u, p, int(round(0.2 / dt)), energy=True)
+        drift.append(abs(E[-1] - E[0]) / abs(E[0]))
+    assert drift[0] < 1e-4
+    assert 3.0 < drift[0] / drift[1] < 5.5
+
+
+def test_leapfrog_free_particles_exact():
+    # far-apart particles (no pairs): uniform motion with periodic wrap
+    r = np.array([[1.0, 1.0], [30.0, 30.0]])
+    v = np.array([[1.0, -2.0], [-0.5, 0.25]])
+    prm = _prm(60.0, dt=0.01)
+    r2, v2, u2 = O.run(r, v, np.ones(2), np.array([-1.0, -1.0]), prm, 150)
+    np.testing.assert_array_equal(v2, v)
+    np.testing.assert_allclose(r2, np.mod(r + 1.5 * v, 60.0), rtol=1e-12)
