@@ -58,6 +58,9 @@ typedef struct {
     double a, b, kB;
     double eta, zeta, kcond;
     double dt;
+    double skin;       /* neighbour-list skin: lists cover r_cut + skin and are
+                          rebuilt before any particle moves skin/2 (0 = every
+                          step); r_cut = min(sqrt(rc^2 + tol^2), max(h, hl))  */
     unsigned terms;    /* SPH_* rate terms used by sph_step                    */
     int precision;     /* SPH_FP64 or SPH_FP32C                                */
 } sph_params;
@@ -140,9 +143,10 @@ int sph_init_thermal(sph_box *box, const double *T, double T0);
 int sph_step(sph_box *box, int64_t nsteps);
 /* Totals at the current (synchronised) state: sum m v^2/2, sum m u, sum. */
 int sph_energy(sph_box *box, double *ke, double *ue, double *etot);
-/* Accumulated GPU time per pass in ms: [0] bin+sort, [1] density,
- * [2] gradients, [3] rates+integrate, [4] pair export; and [5] number of
- * steps.  n = number of entries to read (<= 6).  reset != 0 clears them.
+/* Accumulated GPU time per pass in ms: [0] cell build (bin, sort, lists),
+ * [1] density, [2] gradients, [3] rates, [4] pair export, [5] number of
+ * steps, [6] kick/drift.  n = number of entries to read (<= 7).  reset != 0
+ * clears them.  Recorded only while sph_ctx_set_timing is on.
  * Mirrors the p.timing dictionary (particles.py:164-169,359). */
 int sph_box_timing(sph_box *box, double *out, int n, int reset);
 

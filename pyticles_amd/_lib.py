@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpyticles_hip.so")
+# PYTICLES_HIP_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("PYTICLES_HIP_LIB") or os.path.join(HERE, "libpyticles_hip.so")
 
 SPH_OK = 0
 SPH_ERR_ARG = 1
@@ -41,7 +42,8 @@ class SphError(RuntimeError):
 
 class Params(ctypes.Structure):
     _fields_ = [(name, ctypes.c_double) for name in
-                ("Lx", "Ly", "h", "hl", "rc", "tol", "a", "b", "kB", "eta", "zeta", "kcond", "dt")] + \
+                ("Lx", "Ly", "h", "hl", "rc", "tol", "a", "b", "kB", "eta", "zeta", "kcond", "dt",
+                 "skin")] + \
                [("terms", ctypes.c_uint), ("precision", ctypes.c_int)]
 
 

@@ -107,9 +107,10 @@ class Box:
         _lib.check(_lib.lib.sph_box_set_graph(self._h, 1 if on else 0))
 
     def timing(self, reset=False):
-        out = np.zeros(6)
-        _lib.check(_lib.lib.sph_box_timing(self._h, _lib.dptr(out), 6, 1 if reset else 0))
-        return dict(zip(("sort_ms", "density_ms", "gradients_ms", "rates_ms", "pairs_ms", "steps"), out))
+        out = np.zeros(7)
+        _lib.check(_lib.lib.sph_box_timing(self._h, _lib.dptr(out), 7, 1 if reset else 0))
+        return dict(zip(("build_ms", "density_ms", "gradients_ms", "rates_ms", "pairs_ms", "steps",
+                         "integrate_ms"), out))
 
     # -- (a) neighbour list ----------------------------------------------
     def pairs(self, cap=None):

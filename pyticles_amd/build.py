@@ -31,21 +31,23 @@ def hipcc():
     return "hipcc"
 
 
-def build(verbose=True):
+def build(verbose=True, out=LIB, defines=()):
+    """Compile the library; defines are extra -D flags (tuning variants)."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(HERE, "..", "include")] + srcs + ["-o", tmp]
+    tmp = out + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-D" + d for d in defines] + \
+        ["-I" + os.path.join(HERE, "..", "include")] + srcs + ["-o", tmp]
     if verbose:
         print("[build] " + " ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
-    out = "\n".join(l for l in (res.stdout + res.stderr).splitlines()
+    log = "\n".join(l for l in (res.stdout + res.stderr).splitlines()
                     if "packed-fp32-ops" not in l and l.strip())
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + out)
-    if out and verbose:
-        print(out)
-    os.replace(tmp, LIB)
-    return LIB
+        raise RuntimeError("hipcc failed:\n" + log)
+    if log and verbose:
+        print(log)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -22,13 +22,14 @@ from . import _lib
 H, HL, RC, TOL = 2.0, 4.0, 4.0, 1.0
 A_VDW, B_VDW, KB = 2.0, 0.5, 1.0
 DT = 0.005
+SKIN = 0.5  # neighbour-list skin (lists cover r_cut + skin)
 
 
 def params(L, eta=1.0, zeta=0.0, kcond=5.0, terms=_lib.ALL_TERMS, precision=_lib.FP64,
-           h=H, hl=HL, rc=RC, tol=TOL, a=A_VDW, b=B_VDW, kB=KB, dt=DT, Ly=None):
+           h=H, hl=HL, rc=RC, tol=TOL, a=A_VDW, b=B_VDW, kB=KB, dt=DT, Ly=None, skin=SKIN):
     """sph_params for a periodic square (or Lx x Ly) box."""
     return dict(Lx=float(L), Ly=float(L if Ly is None else Ly), h=h, hl=hl, rc=rc, tol=tol,
-                a=a, b=b, kB=kB, eta=eta, zeta=zeta, kcond=kcond, dt=dt,
+                a=a, b=b, kB=kB, eta=eta, zeta=zeta, kcond=kcond, dt=dt, skin=skin,
                 terms=int(terms), precision=int(precision))
 
 

@@ -87,7 +87,7 @@ struct sph_ctx {
 
 namespace {
 
-enum Timer { T_SORT = 0, T_DENS, T_GRAD, T_RATES, T_PAIRS, T_N };
+enum Timer { T_SORT = 0, T_DENS, T_GRAD, T_RATES, T_PAIRS, T_INT, T_N };
 
 struct Arena {
     sph_ctx *ctx = nullptr;
@@ -95,14 +95,20 @@ struct Arena {
     int cap = 0;   // slot capacity
     int hwm = 0;   // slots in use (end of the last box)
 
+    int lcap = 0;  // neighbour-list entries per slot
+
     // device state (cap slots)
-    double2 *r = nullptr, *v = nullptr, *r_s = nullptr, *v_s = nullptr;
-    double *u = nullptr, *m = nullptr, *u_s = nullptr, *m_s = nullptr;
-    int *id = nullptr, *id_s = nullptr, *pcell = nullptr, *slot_box = nullptr;
-    double2 *vp_s = nullptr, *acc_s = nullptr;
-    double *up_s = nullptr, *udot_s = nullptr;
-    double2 *rho = nullptr, *Q = nullptr, *acc = nullptr, *jq = nullptr;
-    double *T = nullptr, *udot = nullptr, *e = nullptr;
+    double2 *r = nullptr, *v = nullptr, *vp = nullptr, *acc = nullptr;
+    double *u = nullptr, *m = nullptr, *up = nullptr, *udot = nullptr;
+    int *id = nullptr;
+    double2 *r_s = nullptr, *v_s = nullptr, *vp_s = nullptr, *acc_s = nullptr;
+    double *u_s = nullptr, *m_s = nullptr, *up_s = nullptr, *udot_s = nullptr;
+    int *id_s = nullptr, *pcell = nullptr, *slot_box = nullptr;
+    double2 *r_ref = nullptr;
+    unsigned *nbr = nullptr;
+    int2 *cnt = nullptr;
+    double2 *rho = nullptr, *Q = nullptr, *jq = nullptr;
+    double *T = nullptr, *e = nullptr;
     double4 *S = nullptr, *gradv = nullptr;
     float4 *rf = nullptr, *S32 = nullptr;
     float2 *vf = nullptr, *rho32 = nullptr, *Q32 = nullptr;
@@ -142,13 +148,13 @@ struct Arena {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_marks;
-    double t_acc[T_N] = {0, 0, 0, 0, 0};
+    double t_acc[T_N] = {0, 0, 0, 0, 0, 0};
     double steps_done = 0;
 
-    // graph of one arena step
-    hipGraphExec_t graph = nullptr;
-    int graph_n = -1, graph_keys = -1;
+    // captured graphs of one arena step without / with a list build
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
     int use_graph = 1;
+    int always_build = 0;  // arenas: build stage every step (per-box schedule on device)
 
     bool fp32() const { return precision == SPH_FP32C; }
 
@@ -156,34 +162,39 @@ struct Arena {
     {
         ArenaView A;
         memset(&A, 0, sizeof(A));
-        A.r = r; A.v = v; A.u = u; A.m = m; A.id = id;
-        A.r_s = r_s; A.v_s = v_s; A.u_s = u_s; A.m_s = m_s; A.id_s = id_s; A.pcell = pcell;
-        A.vp_s = vp_s; A.up_s = up_s; A.acc_s = acc_s; A.udot_s = udot_s;
-        A.slot_box = slot_box;
-        A.rho = rho; A.T = T; A.S = S; A.Q = Q; A.acc = acc; A.udot = udot;
-        A.gradv = gradv; A.jq = jq; A.e = e;
+        A.r = r; A.v = v; A.vp = vp; A.acc = acc; A.u = u; A.m = m; A.up = up; A.udot = udot; A.id = id;
+        A.r_s = r_s; A.v_s = v_s; A.vp_s = vp_s; A.acc_s = acc_s;
+        A.u_s = u_s; A.m_s = m_s; A.up_s = up_s; A.udot_s = udot_s; A.id_s = id_s;
+        A.pcell = pcell; A.r_ref = r_ref; A.slot_box = slot_box; A.nbr = nbr; A.cnt = cnt;
+        A.rho = rho; A.T = T; A.S = S; A.Q = Q; A.gradv = gradv; A.jq = jq; A.e = e;
         A.rf = rf; A.vf = vf; A.rho32 = rho32; A.T32 = T32; A.S32 = S32; A.Q32 = Q32;
         A.cell_start = cell_start;
-        A.keys = k_out; A.vals = v_out; A.keys_in = keys_in; A.vals_in = nullptr;
+        A.keys = k_out; A.vals = v_out; A.keys_in = keys_in;
         A.boxes = d_boxes; A.dyn = d_dyn; A.segs = d_segs;
         A.nseg = (int)segs.size();
         A.n = hwm;
         A.nkeys = nkeys;
+        A.cap = cap;
+        A.lcap = lcap;
         return A;
     }
 
-    int alloc(int capacity, int prec)
+    int alloc(int capacity, int prec, int list_entries)
     {
         cap = capacity;
         precision = prec;
+        lcap = list_entries;
         int rc = 0;
-        rc |= dalloc(&r, cap); rc |= dalloc(&v, cap); rc |= dalloc(&r_s, cap); rc |= dalloc(&v_s, cap);
-        rc |= dalloc(&u, cap); rc |= dalloc(&m, cap); rc |= dalloc(&u_s, cap); rc |= dalloc(&m_s, cap);
-        rc |= dalloc(&id, cap); rc |= dalloc(&id_s, cap); rc |= dalloc(&pcell, cap);
-        rc |= dalloc(&slot_box, cap);
-        rc |= dalloc(&vp_s, cap); rc |= dalloc(&up_s, cap); rc |= dalloc(&acc_s, cap); rc |= dalloc(&udot_s, cap);
-        rc |= dalloc(&rho, cap); rc |= dalloc(&Q, cap); rc |= dalloc(&acc, cap); rc |= dalloc(&jq, cap);
-        rc |= dalloc(&T, cap); rc |= dalloc(&udot, cap); rc |= dalloc(&e, cap);
+        rc |= dalloc(&r, cap); rc |= dalloc(&v, cap); rc |= dalloc(&vp, cap); rc |= dalloc(&acc, cap);
+        rc |= dalloc(&u, cap); rc |= dalloc(&m, cap); rc |= dalloc(&up, cap); rc |= dalloc(&udot, cap);
+        rc |= dalloc(&id, cap);
+        rc |= dalloc(&r_s, cap); rc |= dalloc(&v_s, cap); rc |= dalloc(&vp_s, cap); rc |= dalloc(&acc_s, cap);
+        rc |= dalloc(&u_s, cap); rc |= dalloc(&m_s, cap); rc |= dalloc(&up_s, cap); rc |= dalloc(&udot_s, cap);
+        rc |= dalloc(&id_s, cap); rc |= dalloc(&pcell, cap); rc |= dalloc(&slot_box, cap);
+        rc |= dalloc(&r_ref, cap); rc |= dalloc(&cnt, cap);
+        rc |= dalloc(&nbr, (size_t)cap * lcap);
+        rc |= dalloc(&rho, cap); rc |= dalloc(&Q, cap); rc |= dalloc(&jq, cap);
+        rc |= dalloc(&T, cap); rc |= dalloc(&e, cap);
         rc |= dalloc(&S, cap); rc |= dalloc(&gradv, cap);
         if (prec == SPH_FP32C) {
             rc |= dalloc(&rf, cap); rc |= dalloc(&S32, cap); rc |= dalloc(&vf, cap);
@@ -197,12 +208,12 @@ struct Arena {
         if (rc) return SPH_ERR_NOMEM;
         // zero-fill the state so that unused slots hold finite values
         size_t z2 = sizeof(double2) * cap, z1 = sizeof(double) * cap;
-        if (hipMemset(r, 0, z2) || hipMemset(v, 0, z2) || hipMemset(u, 0, z1) || hipMemset(m, 0, z1) ||
-            hipMemset(id, 0, sizeof(int) * cap) || hipMemset(rho, 0, z2) || hipMemset(T, 0, z1) ||
-            hipMemset(S, 0, sizeof(double4) * cap) || hipMemset(Q, 0, z2) || hipMemset(acc, 0, z2) ||
-            hipMemset(udot, 0, z1) || hipMemset(gradv, 0, sizeof(double4) * cap) ||
-            hipMemset(jq, 0, z2) || hipMemset(e, 0, z1) || hipMemset(acc_s, 0, z2) ||
-            hipMemset(udot_s, 0, z1))
+        if (hipMemset(r, 0, z2) || hipMemset(v, 0, z2) || hipMemset(vp, 0, z2) || hipMemset(u, 0, z1) ||
+            hipMemset(up, 0, z1) || hipMemset(m, 0, z1) || hipMemset(id, 0, sizeof(int) * cap) ||
+            hipMemset(rho, 0, z2) || hipMemset(T, 0, z1) || hipMemset(S, 0, sizeof(double4) * cap) ||
+            hipMemset(Q, 0, z2) || hipMemset(acc, 0, z2) || hipMemset(udot, 0, z1) ||
+            hipMemset(gradv, 0, sizeof(double4) * cap) || hipMemset(jq, 0, z2) || hipMemset(e, 0, z1) ||
+            hipMemset(r_ref, 0, z2) || hipMemset(cnt, 0, sizeof(int2) * cap))
             return fail(SPH_ERR_HIP, "hipMemset failed");
         // the context stream does not synchronise with the null stream
         HIPCHK(hipDeviceSynchronize());
@@ -213,12 +224,11 @@ struct Arena {
 
     void release()
     {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
-        dfree(r); dfree(v); dfree(r_s); dfree(v_s); dfree(u); dfree(m); dfree(u_s); dfree(m_s);
-        dfree(id); dfree(id_s); dfree(pcell); dfree(slot_box);
-        dfree(vp_s); dfree(up_s); dfree(acc_s); dfree(udot_s); dfree(rho); dfree(Q); dfree(acc);
-        dfree(jq); dfree(T); dfree(udot); dfree(e); dfree(S); dfree(gradv);
+        invalidate_graph();
+        dfree(r); dfree(v); dfree(vp); dfree(acc); dfree(u); dfree(m); dfree(up); dfree(udot); dfree(id);
+        dfree(r_s); dfree(v_s); dfree(vp_s); dfree(acc_s); dfree(u_s); dfree(m_s); dfree(up_s); dfree(udot_s);
+        dfree(id_s); dfree(pcell); dfree(slot_box); dfree(r_ref); dfree(nbr); dfree(cnt);
+        dfree(rho); dfree(Q); dfree(jq); dfree(T); dfree(e); dfree(S); dfree(gradv);
         dfree(rf); dfree(S32); dfree(vf); dfree(rho32); dfree(Q32); dfree(T32);
         dfree(keys_in); dfree(k_out); dfree(v_out); dfree(k_alt); dfree(v_alt);
         if (sort_tmp) (void)hipFree(sort_tmp);
@@ -261,9 +271,10 @@ struct Arena {
 
     void invalidate_graph()
     {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
-        graph_n = -1;
+        for (auto &g : graph) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
     }
 
     // Recompute key bases in slot order, the high-water mark, and upload the
@@ -307,8 +318,8 @@ struct Arena {
                               ctx->stream));
         // host-side vectors must outlive the async copies
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (graph && (graph_n != hwm || graph_keys != nkeys)) invalidate_graph();
-        // the segment count is baked into captured launches
+        // the slot count, key count and segment count are baked into
+        // captured launches
         invalidate_graph();
         return SPH_OK;
     }
@@ -359,28 +370,33 @@ struct Arena {
     }
 
     // ---- passes ----------------------------------------------------------
-    int build(int state_mask)
+    // Cell order and neighbour lists of every box that builds in this launch
+    // (see building() in the kernels); the others keep their order.
+    void enqueue_build(int force_mask)
     {
         hipStream_t st = ctx->stream;
         ArenaView A = view();
-        if (hwm == 0) return SPH_OK;
-        launch_keys(A, state_mask, st);
+        if (hwm == 0) return;
+        launch_keys(A, force_mask, st);
         radix_sort_pairs(keys_in, k_out, v_out, k_alt, v_alt, hwm, bits_for(nkeys), sort_tmp, st);
-        launch_permute(A, fp32() ? 1 : 0, st);
+        launch_permute(A, st);
         launch_cell_start(A, st);
-        return SPH_OK;
+        launch_build_lists(A, force_mask, fp32() ? 1 : 0, st);
     }
 
-    // One arena step: every INIT box gets u from T0, every RUN box one
-    // leapfrog step; then per-box energy capture and state advance.
-    void enqueue_step()
+    // One arena step: INIT boxes get u from T0, RUN boxes one leapfrog step
+    // (density, gradients, rates, kick/drift); then per-box energy capture
+    // and state advance.
+    void enqueue_step(bool with_build)
     {
         hipStream_t st = ctx->stream;
         ArenaView A = view();
         const int f = fp32() ? 1 : 0;
-        mark(T_SORT, true);
-        build(MASK_STEP);
-        mark(T_SORT, false);
+        if (with_build) {
+            mark(T_SORT, true);
+            enqueue_build(0);
+            mark(T_SORT, false);
+        }
         mark(T_DENS, true);
         launch_density(A, f, MASK_STEP, st);
         mark(T_DENS, false);
@@ -388,39 +404,71 @@ struct Arena {
         launch_grad(A, f, MASK_RUN, 0, st);
         mark(T_GRAD, false);
         mark(T_RATES, true);
-        launch_rates(A, f, 1, st);
+        launch_rates(A, f, MASK_RUN, st);
         mark(T_RATES, false);
+        mark(T_INT, true);
+        launch_integrate(A, f, st);
+        mark(T_INT, false);
         launch_box_energy(A, (int)boxes.size(), st);
-        launch_advance(d_dyn, (int)boxes.size(), st);
+        launch_advance(A, (int)boxes.size(), st);
     }
 
-    int step(int64_t nsteps)
+    int launch_steps(int64_t count, bool with_build)
     {
-        if (hwm == 0) return SPH_OK;
         const bool graphs = use_graph && !ctx->timing;
-        for (int64_t i = 0; i < nsteps; ++i) {
+        hipGraphExec_t &g = graph[with_build ? 1 : 0];
+        for (int64_t i = 0; i < count; ++i) {
             if (graphs) {
-                if (!graph) {
-                    hipGraph_t g;
+                if (!g) {
+                    hipGraph_t gr;
                     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                    enqueue_step();
-                    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
-                    hipError_t e1 = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(g);
+                    enqueue_step(with_build);
+                    HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
+                    hipError_t e1 = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(gr);
                     if (e1 != hipSuccess) {
-                        graph = nullptr;
+                        g = nullptr;
                         return fail(SPH_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e1));
                     }
-                    graph_n = hwm;
-                    graph_keys = nkeys;
                 }
-                HIPCHK(hipGraphLaunch(graph, ctx->stream));
+                HIPCHK(hipGraphLaunch(g, ctx->stream));
             } else {
-                enqueue_step();
+                enqueue_step(with_build);
             }
         }
         HIPCHK(hipGetLastError());
+        return SPH_OK;
+    }
+
+    // Arena stepping: the build stage runs every step and each box builds on
+    // its own schedule.
+    int step(int64_t nsteps)
+    {
+        if (hwm == 0) return SPH_OK;
+        if (int rc = launch_steps(nsteps, true)) return rc;
         steps_done += (double)nsteps;
+        return collect_timing();
+    }
+
+    // One running box: steps without the build stage until the box's next
+    // scheduled build (read back after each build).
+    int step_single(int b, int64_t evals)
+    {
+        int64_t done = 0;
+        while (done < evals) {
+            if (int rc = launch_steps(1, true)) return rc;
+            ++done;
+            if (done >= evals) break;
+            if (int rc = read_dyn()) return rc;
+            const BoxDyn &D = h_dyn[b];
+            if (D.health) break;
+            int64_t plain = D.next_rebuild - D.k;
+            if (plain < 0) plain = 0;
+            if (plain > evals - done) plain = evals - done;
+            if (int rc = launch_steps(plain, false)) return rc;
+            done += plain;
+        }
+        steps_done += (double)evals;
         return collect_timing();
     }
 
@@ -454,11 +502,20 @@ int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
         return fail(SPH_ERR_ARG, "invalid sph_params: need Lx, Ly, h, hl, kB > 0, rc, tol, dt >= 0");
     if (p->precision != SPH_FP64 && p->precision != SPH_FP32C)
         return fail(SPH_ERR_ARG, "invalid precision");
+    if (!(p->skin >= 0) || !std::isfinite(p->skin)) return fail(SPH_ERR_ARG, "invalid skin (need >= 0)");
     memset(&B, 0, sizeof(B));
     const double PI = 3.14159265358979323846;
     B.Lx = p->Lx; B.Ly = p->Ly; B.hLx = p->Lx / 2.; B.hLy = p->Ly / 2.;
     B.R2list = p->rc * p->rc + p->tol * p->tol;  // neighbour_list.py:156-157,178
-    const double Rc = std::sqrt(B.R2list) * (1.0 + 1e-9);
+    B.h2 = p->h * p->h; B.hl2 = p->hl * p->hl;
+    B.cut2 = std::min(B.R2list, std::max(B.h2, B.hl2));
+    const double rcut = std::sqrt(B.cut2);
+    B.skin = p->skin;
+    B.rnl2 = (rcut + p->skin) * (rcut + p->skin);
+    B.hsk2 = (p->h + p->skin) * (p->h + p->skin);
+    B.hskin2 = 0.25 * p->skin * p->skin;
+    // cells cover both the pair-export radius and the list radius
+    const double Rc = std::max(std::sqrt(B.R2list), rcut + p->skin) * (1.0 + 1e-9);
     int ncx = (int)std::floor(p->Lx / Rc), ncy = (int)std::floor(p->Ly / Rc);
     if (Rc <= 0) { ncx = 1; ncy = 1; }
     if (ncx < 3) ncx = 1;
@@ -472,15 +529,24 @@ int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
     B.qhl = 5.0 / (PI * p->hl * p->hl);
     B.fh = -12.0 * B.qh / (p->h * p->h);    // spkernel.py:112-113, factored
     B.fhl = -12.0 * B.qhl / (p->hl * p->hl);
-    B.h2 = p->h * p->h; B.hl2 = p->hl * p->hl;
-    const double hm2 = std::max(B.h2, B.hl2);
-    B.cut2 = std::min(B.R2list, hm2);
     B.a = p->a; B.b = p->b; B.kB = p->kB; B.eta = p->eta; B.zeta = p->zeta; B.kcond = p->kcond;
     B.dt = p->dt;
     B.n = (int)n;
     B.terms = (int)(p->terms & SPH_ALL_TERMS);
     B.precision = p->precision;
     return SPH_OK;
+}
+
+// Neighbour-list entries per slot for a box: 2.5x the mean count at the
+// box's number density plus 32, rounded up to 32 (at least 64).
+int list_entries_for(const sph_params *p, int64_t n)
+{
+    const double rcut = std::sqrt(std::min(p->rc * p->rc + p->tol * p->tol, std::max(p->h * p->h, p->hl * p->hl)));
+    const double rnl = rcut + p->skin;
+    const double dens = (double)n / (p->Lx * p->Ly);
+    const double est = 3.14159265358979323846 * rnl * rnl * dens;
+    int l = (int)std::ceil((2.5 * est + 32.0) / 32.0) * 32;
+    return std::max(64, std::min(l, 4096));
 }
 
 int Arena::add_box(int64_t n, const sph_params *prm, int *box_out)
@@ -517,7 +583,7 @@ int Arena::add_box(int64_t n, const sph_params *prm, int *box_out)
     memset(&D, 0, sizeof(D));
     D.state = BOX_DONE;
     if (int rc = write_dyn(b, D)) return rc;
-    launch_init_ids(view(), boxes[b].part_off, (int)n, ctx->stream);
+    launch_init_slots(view(), boxes[b].part_off, (int)n, ctx->stream);
     HIPCHK(hipGetLastError());
     *box_out = b;
     return SPH_OK;
@@ -576,7 +642,8 @@ int check_health(Arena &A, int b)
     std::string msg = "numerical failure in box " + std::to_string(b) + ":";
     if (h & HEALTH_NONFINITE) msg += " non-finite density/rates/state;";
     if (h & HEALTH_EOS_POLE) msg += " vdW pole rho*b >= 1 reached;";
-    if (h & HEALTH_SKIN) msg += " cell skin exceeded;";
+    if (h & HEALTH_SKIN) msg += " a particle moved more than half the neighbour-list skin between builds;";
+    if (h & HEALTH_LISTCAP) msg += " neighbour-list capacity exceeded (density too high for the list);";
     return fail(SPH_ERR_NUMERIC, msg);
 }
 
@@ -651,7 +718,7 @@ int sph_box_create(sph_ctx *ctx, int64_t n, const sph_params *prm, sph_box **out
     sph_box *bx = new sph_box();
     bx->ar.ctx = ctx;
     bx->prm = *prm;
-    int rc = bx->ar.alloc((int)n, prm->precision);
+    int rc = bx->ar.alloc((int)n, prm->precision, list_entries_for(prm, n));
     if (!rc) rc = bx->ar.add_box(n, prm, &bx->b);
     if (rc) {
         std::string keep = g_err;
@@ -737,7 +804,7 @@ int sph_nlist_build(sph_box *box, int64_t cap, int64_t *iap, int64_t *nip_out)
     HIPCHK(hipSetDevice(A.ctx->device));
     const int n = A.boxes[box->b].n;
     A.mark(T_PAIRS, true);
-    A.build(MASK_ALL);
+    A.enqueue_build(MASK_ALL);
     if (!A.d_inv) {
         if (dalloc(&A.d_inv, A.cap) || dalloc(&A.d_cnt, A.cap) || dalloc(&A.d_off, A.cap) ||
             dalloc(&A.d_grand, 1))
@@ -850,8 +917,8 @@ int sph_kernel_lucy(sph_ctx *ctx, int64_t np, const double *r, const double *dx,
     return SPH_OK;
 }
 
-// density (+ optional gradients / rates) of the current state, committed to
-// the canonical order so downloads see matching fields.
+// Build (cell order + lists) then density, optional gradients and rates of
+// the current state; every derived field ends in the new slot order.
 static int eval_box(sph_box *box, int grad, int rates)
 {
     Arena &A = box->ar;
@@ -860,7 +927,7 @@ static int eval_box(sph_box *box, int grad, int rates)
     const int f = A.fp32() ? 1 : 0;
     ArenaView V = A.view();
     A.mark(T_SORT, true);
-    A.build(MASK_ALL);
+    A.enqueue_build(MASK_ALL);
     A.mark(T_SORT, false);
     A.mark(T_DENS, true);
     launch_density(V, f, MASK_ALL, st);
@@ -870,9 +937,11 @@ static int eval_box(sph_box *box, int grad, int rates)
         launch_grad(V, f, MASK_ALL, 1, st);
         A.mark(T_GRAD, false);
     }
-    A.mark(T_RATES, true);
-    launch_rates(V, f, rates ? 0 : 1, st);  // mode 1 on an idle box = commit
-    A.mark(T_RATES, false);
+    if (rates) {
+        A.mark(T_RATES, true);
+        launch_rates(V, f, MASK_ALL, st);
+        A.mark(T_RATES, false);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     if (int rc = A.collect_timing()) return rc;
@@ -941,9 +1010,10 @@ int sph_step(sph_box *box, int64_t nsteps)
     D.state = BOX_RUN;
     D.k = 0;
     D.nsteps = nsteps;
+    D.next_rebuild = 0;
     if (int rc = A.write_dyn(box->b, D)) return rc;
     // nsteps leapfrog steps take nsteps + 1 rate evaluations
-    if (int rc = A.step(nsteps + 1)) return rc;
+    if (int rc = A.step_single(box->b, nsteps + 1)) return rc;
     A.steps_done -= 1.0;
     return check_health(A, box->b);
 }
@@ -966,9 +1036,11 @@ int sph_energy(sph_box *box, double *ke, double *ue, double *etot)
 
 int sph_box_timing(sph_box *box, double *out, int n, int reset)
 {
-    if (!box || n < 0 || n > 6 || (n > 0 && !out)) return fail(SPH_ERR_ARG, "invalid argument");
+    if (!box || n < 0 || n > 7 || (n > 0 && !out)) return fail(SPH_ERR_ARG, "invalid argument");
     Arena &A = box->ar;
-    for (int i = 0; i < n; ++i) out[i] = i < T_N ? A.t_acc[i] : A.steps_done;
+    const double vals[7] = {A.t_acc[T_SORT], A.t_acc[T_DENS], A.t_acc[T_GRAD], A.t_acc[T_RATES],
+                            A.t_acc[T_PAIRS], A.steps_done, A.t_acc[T_INT]};
+    for (int i = 0; i < n; ++i) out[i] = vals[i];
     if (reset) {
         for (double &t : A.t_acc) t = 0;
         A.steps_done = 0;
@@ -986,7 +1058,10 @@ int sph_arena_create(sph_ctx *ctx, int64_t particle_capacity, int precision, sph
     HIPCHK(hipSetDevice(ctx->device));
     sph_arena *a = new sph_arena();
     a->ar.ctx = ctx;
-    int rc = a->ar.alloc((int)particle_capacity, precision);
+    // boxes of number density up to 2.5 (the vdW pole is at 1/b = 2) with
+    // the default list radius 4.5
+    int rc = a->ar.alloc((int)particle_capacity, precision, 192);
+    a->ar.always_build = 1;
     if (!rc) rc = a->ar.relayout();
     if (rc) {
         std::string keep = g_err;

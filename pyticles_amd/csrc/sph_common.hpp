@@ -1,15 +1,21 @@
 // sph_common.hpp -- types shared by the host runtime and the gfx950 kernels.
 //
-// Device layout (structure of arrays, one slot per particle, sorted by cell
-// after every build; all arrays live in one arena shared by every box):
-//   canonical state  r, v (double2), u, m (double), id (int)  -- end of step
-//   sorted copies    r_s, v_s, u_s, m_s, id_s, pcell          -- read by passes
-//                    vp_s, up_s: velocity / energy predicted to the current
-//                    integer time from the last rates (leapfrog predictor)
+// Device layout (structure of arrays, one slot per particle; slots are
+// sorted by (box, cell) at every neighbour-list build and keep that order
+// until the next build; all arrays live in one arena shared by every box):
+//   state            r, v (double2), u, m (double), id (int); v and u are
+//                    half-step values between steps (leapfrog)
+//   predicted        vp, up: velocity / energy predicted to the current
+//                    integer time from the last rates; every pass reads these
+//   last rates       acc (double2), udot
+//   sorted scratch   *_s copies written by the permutation of a build
+//   neighbour list   nbr[k * cap + s] (uint32 slot), cnt[s] = (short, long):
+//                    short entries (r < h + skin at build) at k = 0.., long
+//                    entries (h + skin <= r < r_cut + skin) at k = lcap-1 down
 //   derived          rho (double2: rho, rho_lr), T, S (double4: Sxx, Sxy,
 //                    Syy, Sco = P/rho^2 and pco/rho_lr^2), Q (double2: q/rho^2)
-//   FP32 mirrors     rf (float4: x, y relative to the cell corner, m, cell
-//                    column|row), vf (float2), rho32 (float2), T32, S32, Q32
+//   FP32 mirrors     rf (float4: x, y relative to the corner of the build
+//                    cell, m, cell column|row), vf (float2), rho32, T32, S32, Q32
 // A box is a contiguous slot range [part_off, part_off + n) plus a key range
 // [key_base, key_base + ncx*ncy] of cells; free slot ranges own one key each
 // so a stable sort over the whole arena never moves a particle across boxes.
@@ -24,7 +30,8 @@ enum BoxState : int { BOX_FREE = 0, BOX_INIT = 1, BOX_RUN = 2, BOX_DONE = 3 };
 enum Health : int {
     HEALTH_NONFINITE = 1,   // NaN/Inf in density, rates or state
     HEALTH_EOS_POLE = 2,    // rho * b >= 1 (vdW pole, properties.py:41)
-    HEALTH_SKIN = 4,        // displacement since the last cell build exceeded the skin
+    HEALTH_SKIN = 4,        // displacement since the last list build exceeded skin/2
+    HEALTH_LISTCAP = 8,     // a particle had more neighbours than the list holds
 };
 
 // Static description of one box (host-owned, uploaded when it changes).
@@ -37,6 +44,7 @@ struct BoxDesc {
     double a, b, kB;               // vdW constants
     double eta, zeta, kcond;       // transport coefficients
     double dt, T0;                 // time step, initial temperature
+    double skin, rnl2, hsk2, hskin2;  // list skin, (r_cut+skin)^2, (h+skin)^2, (skin/2)^2
     int ncx, ncy, sx, sy;          // cell grid and stencil half widths (0/1)
     int key_base, part_off, n, terms;
     int precision, pad0, pad1, pad2;
@@ -50,6 +58,9 @@ struct BoxDyn {
     int64_t k;      // current step index in [0, nsteps]
     int64_t nsteps; // steps requested for this run
     double E0, E1;  // total energy at the first and last synchronised state
+    int64_t next_rebuild;   // step index of the next neighbour-list build
+    unsigned long long vmax2, amax2;  // max |v|^2, |a|^2 at the last build (double bits)
+    int64_t builds;         // list builds so far
 };
 
 // Segment of the slot space: a box or a free range, sorted by offset.
@@ -60,27 +71,25 @@ struct SegDesc {
 };
 
 struct ArenaView {
-    // canonical state
-    double2 *r, *v;
-    double *u, *m;
+    // state (slot order of the last build)
+    double2 *r, *v, *vp, *acc;
+    double *u, *m, *up, *udot;
     int *id;
-    // sorted copies
-    double2 *r_s, *v_s;
-    double *u_s, *m_s;
-    int *id_s, *pcell;
-    double2 *vp_s;
-    double *up_s;
-    double2 *acc_s;
-    double *udot_s;
-    // per-slot box index (-1 = free)
-    int *slot_box;
+    // sorted scratch written by the permutation of a build
+    double2 *r_s, *v_s, *vp_s, *acc_s;
+    double *u_s, *m_s, *up_s, *udot_s;
+    int *id_s;
+    // build bookkeeping
+    int *pcell;          // cell of the slot at the last build
+    double2 *r_ref;      // position at the last build (skin check)
+    int *slot_box;       // box of the slot (-1 = free)
+    unsigned *nbr;       // neighbour lists, nbr[k * cap + s]
+    int2 *cnt;           // (short, long) entries per slot
     // derived fields
     double2 *rho;
     double *T;
     double4 *S;
     double2 *Q;
-    double2 *acc;
-    double *udot;
     double4 *gradv;
     double2 *jq;
     double *e;
@@ -96,7 +105,6 @@ struct ArenaView {
     unsigned *keys;      // sorted keys (final buffer of the radix sort)
     unsigned *vals;      // source slot of each sorted slot
     unsigned *keys_in;   // unsorted keys written by the key pass
-    unsigned *vals_in;
     // boxes
     const BoxDesc *boxes;
     BoxDyn *dyn;
@@ -104,6 +112,8 @@ struct ArenaView {
     int nseg;
     int n;               // slots in use (high-water mark)
     int nkeys;           // total keys
+    int cap;             // slot capacity (stride of nbr)
+    int lcap;            // list entries per slot
     int pad;
 };
 

@@ -1,17 +1,22 @@
 // sph_kernels.hip -- gfx950 kernels of the smooth-particle hot path.
 //
-// Every pass is a per-particle gather: work-item s owns sorted slot s and
-// walks the 3x3 periodic cell stencil around the cell it was binned into,
-// so each pair is evaluated from both ends and no float atomics are needed.
-// Slots are sorted by (box, cell), so the 64 lanes of a wavefront cover a
-// few adjacent cells and their candidate reads hit the same cache lines.
+// Every pass is a per-particle gather: work-item s owns slot s and sums over
+// its own neighbour list, so each pair is evaluated from both ends and no
+// float atomics are needed.  Lists are built from a periodic cell list with
+// a skin and reused until the particles of a box could have moved more than
+// half the skin; slots are sorted by (box, cell) at each build, so the 64
+// lanes of a wavefront own a few adjacent cells and their neighbour reads hit
+// the same cache lines.
 //
 // Passes (reference functions they restate in brackets):
-//   P0 bin + stable radix sort + permute     [VerletList.build, neighbour_list.py:160-189]
-//   P1 density (h and h_l) + vdW EOS          [properties.py:63-120, f_properties.py:100-120]
-//   P2 velocity gradient + heat flux          [f_properties.py:111,130 (Fortran absent)]
-//   P3 pressure/cohesion/viscous/conduction   [forces.py:338-405, c_forces.pyx:96-237,
-//      rates + fused leapfrog kick/drift       spam_complete_force.py:158-165]
+//   build  bin + stable radix sort + permute + neighbour lists
+//                                      [VerletList.build, neighbour_list.py:160-189]
+//   P1     density (h and h_l) + vdW EOS [properties.py:63-120, f_properties.py:100-120]
+//   P2     velocity gradient + heat flux [f_properties.py:111,130 (Fortran absent)]
+//   P3     pressure/cohesion/viscous/conduction rates
+//                                      [forces.py:338-405, c_forces.pyx:96-237,
+//                                       spam_complete_force.py:158-165]
+//   P4     leapfrog kick/drift, predictor, periodic wrap, skin check
 #include <math.h>
 #include "sph_common.hpp"
 #include "sph_kernels.hpp"
@@ -19,6 +24,16 @@
 namespace sph {
 
 static constexpr int NT = 256;  // work-items per block (4 waves)
+// list entries fetched per round in the passes (measured best on MI355X:
+// 2 for FP64, whose wider registers cut occupancy, 4 for FP32)
+#ifndef SPH_UNROLL64
+#define SPH_UNROLL64 2
+#endif
+#ifndef SPH_UNROLL32
+#define SPH_UNROLL32 4
+#endif
+static constexpr int U64 = SPH_UNROLL64;
+static constexpr int U32 = SPH_UNROLL32;
 
 // gfx950 dispatches block i to XCD (i mod 8).  Remap so that each XCD gets a
 // contiguous run of blocks: neighbouring slots (neighbouring cells) then share
@@ -78,13 +93,41 @@ __device__ __forceinline__ bool state_in(int st, int mask) { return ((1 << st) &
 
 __device__ __forceinline__ void flag_health(BoxDyn *dyn, int b, int bits) { atomicOr(&dyn[b].health, bits); }
 
+// A box rebuilds its cell order and neighbour list in this launch when it is
+// initialising, when its running schedule says so, or when forced (the
+// stand-alone passes build every box whose state is in force_mask).
+__device__ __forceinline__ bool building(const BoxDyn &D, int force_mask)
+{
+    if (state_in(D.state, force_mask)) return true;
+    return D.state == BOX_INIT || (D.state == BOX_RUN && D.k >= D.next_rebuild);
+}
+
+// atomicMax on a non-negative double through its bit pattern (monotone for
+// non-negative values); reads first so that most lanes issue no atomic.
+__device__ __forceinline__ void max_bits(unsigned long long *addr, double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    if (b > *(volatile unsigned long long *)addr) atomicMax(addr, b);
+}
+
+// Slot position relative to the corner of its build cell, folded into the
+// box with the minimum image (drifts may carry it across the periodic edge).
+__device__ __forceinline__ float4 rel_pos(double2 r, int cell, double m, const BoxDesc &B)
+{
+    const int cx = cell % B.ncx, cy = cell / B.ncx;
+    double xl = r.x - cx * B.cwx, yl = r.y - cy * B.cwy;
+    xl = xl > B.hLx ? xl - B.Lx : (xl < -B.hLx ? xl + B.Lx : xl);
+    yl = yl > B.hLy ? yl - B.Ly : (yl < -B.hLy ? yl + B.Ly : yl);
+    return make_float4((float)xl, (float)yl, (float)m, __int_as_float(cx | (cy << 16)));
+}
+
 // ===========================================================================
-// P0  binning, permutation, cell starts
+// build: binning, permutation, cell starts, neighbour lists
 // ===========================================================================
 
-// key = box key base + cell (row major); boxes outside state_mask and free
-// ranges keep one constant key, so the stable sort leaves them in place.
-__global__ __launch_bounds__(NT) void k_keys(ArenaView A, int state_mask)
+// key = box key base + cell (row major) for building boxes; other boxes and
+// free ranges keep one constant key, so the stable sort leaves them in place.
+__global__ __launch_bounds__(NT) void k_keys(ArenaView A, int force_mask)
 {
     const int s = blockIdx.x * NT + threadIdx.x;
     if (s >= A.n) return;
@@ -98,7 +141,7 @@ __global__ __launch_bounds__(NT) void k_keys(ArenaView A, int state_mask)
     int key = sg.key, box = -1;
     if (sg.box >= 0) {
         box = sg.box;
-        if (state_in(A.dyn[box].state, state_mask)) {
+        if (building(A.dyn[box], force_mask)) {
             const BoxDesc &B = A.boxes[box];
             const double2 r = A.r[s];
             int cx = (int)(r.x * B.icwx), cy = (int)(r.y * B.icwy);
@@ -111,50 +154,24 @@ __global__ __launch_bounds__(NT) void k_keys(ArenaView A, int state_mask)
     A.keys_in[s] = (unsigned)key;
 }
 
-// Gather the state into sorted order; FP32 mirrors for the compensated form.
-__global__ __launch_bounds__(NT) void k_permute(ArenaView A, int fp32)
+// Gather every per-slot quantity into sorted scratch (identity for boxes
+// that do not build, whose keys are constant).
+__global__ __launch_bounds__(NT) void k_permute(ArenaView A)
 {
     const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
-    const int src = (int)A.vals[s];
-    const double2 r = A.r[src], v = A.v[src];
-    const double m = A.m[src], u = A.u[src];
-    A.r_s[s] = r;
-    A.v_s[s] = v;
-    A.u_s[s] = u;
-    A.m_s[s] = m;
-    A.id_s[s] = A.id[src];
     const int b = A.slot_box[s];
-    int cell = 0;
-    double2 vp = v;
-    double up = u;
-    if (b >= 0) {
-        const BoxDesc &B = A.boxes[b];
-        const BoxDyn &D = A.dyn[b];
-        const double2 a = A.acc[src];
-        const double ud = A.udot[src];
-        if (D.state == BOX_RUN) {
-            if (D.k > 0) {  // predictor to the integer time: v_{k-1/2} + dt/2 a_{k-1}
-                const double hdt = 0.5 * B.dt;
-                vp = make_double2(v.x + hdt * a.x, v.y + hdt * a.y);
-                up = u + hdt * ud;
-            }
-        } else {
-            A.acc_s[s] = a;
-            A.udot_s[s] = ud;
-        }
-        cell = (int)A.keys[s] - B.key_base;
-        if (cell < 0 || cell >= B.ncx * B.ncy) cell = 0;  // frozen boxes
-        if (fp32) {
-            const int cx = cell % B.ncx, cy = cell / B.ncx;
-            double xl = r.x - cx * B.cwx, yl = r.y - cy * B.cwy;
-            A.rf[s] = make_float4((float)xl, (float)yl, (float)m, __int_as_float(cx | (cy << 16)));
-            A.vf[s] = make_float2((float)vp.x, (float)vp.y);
-        }
-    }
-    A.vp_s[s] = vp;
-    A.up_s[s] = up;
-    A.pcell[s] = cell;
+    if (b < 0) return;
+    const int src = (int)A.vals[s];
+    A.r_s[s] = A.r[src];
+    A.v_s[s] = A.v[src];
+    A.vp_s[s] = A.vp[src];
+    A.acc_s[s] = A.acc[src];
+    A.u_s[s] = A.u[src];
+    A.m_s[s] = A.m[src];
+    A.up_s[s] = A.up[src];
+    A.udot_s[s] = A.udot[src];
+    A.id_s[s] = A.id[src];
 }
 
 // cell_start[k] = first sorted slot with key >= k, for k in [0, nkeys].
@@ -173,6 +190,69 @@ __global__ __launch_bounds__(NT) void k_cell_start(ArenaView A)
     const int k = (int)A.keys[s];
     const int kp = s == 0 ? -1 : (int)A.keys[s - 1];
     for (int c = kp + 1; c <= k; ++c) A.cell_start[c] = s;
+}
+
+// Neighbour list of a building slot: every other slot of the 3x3 cell
+// stencil with r^2 < (r_cut + skin)^2, short entries (r^2 < (h + skin)^2)
+// from the front, the rest from the back.  The sorted state is copied back
+// into the state arrays, the build position kept for the skin check, and the
+// box's largest speed and acceleration recorded for its next schedule.
+__global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask, int fp32)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    BoxDyn &D = A.dyn[b];
+    if (!building(D, force_mask)) return;
+    const BoxDesc &B = A.boxes[b];
+    const double2 ri = A.r_s[s];
+    const double2 vpi = A.vp_s[s], ai = A.acc_s[s];
+    const double mi = A.m_s[s];
+    A.r[s] = ri;
+    A.v[s] = A.v_s[s];
+    A.vp[s] = vpi;
+    A.acc[s] = ai;
+    A.u[s] = A.u_s[s];
+    A.m[s] = mi;
+    A.up[s] = A.up_s[s];
+    A.udot[s] = A.udot_s[s];
+    A.id[s] = A.id_s[s];
+    A.r_ref[s] = ri;
+    int cell = (int)A.keys[s] - B.key_base;
+    cell = (cell < 0 || cell >= B.ncx * B.ncy) ? 0 : cell;
+    A.pcell[s] = cell;
+    if (fp32) {
+        A.rf[s] = rel_pos(ri, cell, mi, B);
+        A.vf[s] = make_float2((float)vpi.x, (float)vpi.y);
+    }
+    const int cap = A.cap, lcap = A.lcap;
+    int ns = 0, nl = 0;
+    for_each_cell(B, A.cell_start, cell, [&](int t0, int t1, int, int) {
+        for (int t = t0; t < t1; ++t) {
+            const double2 rj = A.r_s[t];
+            double dx = rj.x - ri.x, dy = rj.y - ri.y;
+            min_image(dx, dy, B);
+            const double r2 = rsq_exact(dx, dy);
+            if (r2 < B.rnl2 && t != s) {
+                if (ns + nl < lcap) {
+                    if (r2 < B.hsk2) A.nbr[(size_t)ns * cap + s] = (unsigned)t;
+                    else A.nbr[(size_t)(lcap - 1 - nl) * cap + s] = (unsigned)t;
+                }
+                if (r2 < B.hsk2) ++ns;
+                else ++nl;
+            }
+        }
+    });
+    if (ns + nl > lcap) {
+        flag_health(A.dyn, b, HEALTH_LISTCAP);
+        const int keep = ns < lcap ? ns : lcap;
+        nl = lcap - keep;
+        ns = keep;
+    }
+    A.cnt[s] = make_int2(ns, nl);
+    max_bits(&D.vmax2, vpi.x * vpi.x + vpi.y * vpi.y);
+    max_bits(&D.amax2, ai.x * ai.x + ai.y * ai.y);
 }
 
 // ===========================================================================
@@ -195,35 +275,72 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
     const int st = A.dyn[b].state;
     if (!state_in(st, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
-    const double2 ri = A.r_s[s];
+    const double2 ri = A.r[s];
+    const int2 c = A.cnt[s];
+    const size_t cap = A.cap;
+    const unsigned *nb = A.nbr + s;
     double rho = B.qh, rhol = B.qhl;
-    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int, int) {
-        for (int t = t0; t < t1; ++t) {
-            const double2 rj = A.r_s[t];
-            double dx = rj.x - ri.x, dy = rj.y - ri.y;
+    const double cutl = B.cut2 < B.hl2 ? B.cut2 : B.hl2;
+    // U64 entries per round: all their loads are issued before any use so
+    // several neighbour fetches are in flight per wavefront
+    for (int k0 = 0; k0 < c.x; k0 += U64) {
+        int t[U64];
+        double2 rj[U64];
+        double mj[U64];
+#pragma unroll
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            rj[u] = A.r[t[u]];
+            mj[u] = A.m[t[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
             min_image(dx, dy, B);
             const double r2 = rsq_exact(dx, dy);
-            if (r2 < B.cut2 && t != s) {
-                const double mj = A.m_s[t];
+            if (k0 + u < c.x && r2 < B.cut2) {
                 const double rr = sqrt(r2);
                 if (r2 < B.hl2) {
                     const double q = rr * B.ihl, w = 1.0 - q;
-                    rhol += mj * (B.qhl * (1.0 + 3.0 * q) * (w * w * w));
+                    rhol += mj[u] * (B.qhl * (1.0 + 3.0 * q) * (w * w * w));
                 }
                 if (r2 < B.h2) {
                     const double q = rr * B.ih, w = 1.0 - q;
-                    rho += mj * (B.qh * (1.0 + 3.0 * q) * (w * w * w));
+                    rho += mj[u] * (B.qh * (1.0 + 3.0 * q) * (w * w * w));
                 }
             }
         }
-    });
-    A.m[s] = A.m_s[s];
-    A.id[s] = A.id_s[s];
-    double ui = A.up_s[s];
+    }
+    const int kl = A.lcap - c.y;
+    for (int k0 = kl; k0 < A.lcap; k0 += U64) {
+        int t[U64];
+        double2 rj[U64];
+        double mj[U64];
+#pragma unroll
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < A.lcap ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            rj[u] = A.r[t[u]];
+            mj[u] = A.m[t[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
+            min_image(dx, dy, B);
+            const double r2 = rsq_exact(dx, dy);
+            if (k0 + u < A.lcap && r2 < cutl) {
+                const double rr = sqrt(r2);
+                const double q = rr * B.ihl, w = 1.0 - q;
+                rhol += mj[u] * (B.qhl * (1.0 + 3.0 * q) * (w * w * w));
+            }
+        }
+    }
+    double ui = A.up[s];
     if (st == BOX_INIT) {  // u = kB T0 - a rho (properties.py:44-46,119)
         ui = B.kB * B.T0 - B.a * rho;
-        A.u_s[s] = ui;
-        A.up_s[s] = ui;
+        A.u[s] = ui;
+        A.up[s] = ui;
     }
     double T = (ui + B.a * rho) / B.kB;  // properties.py:48-49
     T = T < 0.0 ? 0.0 : T;               // spam_complete_force.py:152
@@ -251,25 +368,40 @@ __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int 
     if (!state_in(A.dyn[b].state, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
     if (!diag && !(B.terms & 12u)) return;
-    const double2 ri = A.r_s[s], vi = A.vp_s[s];
+    const double2 ri = A.r[s], vi = A.vp[s];
     const double Ti = A.T[s];
     const double cut = B.R2list < B.h2 ? B.R2list : B.h2;
+    const int ns = A.cnt[s].x;
+    const size_t cap = A.cap;
+    const unsigned *nb = A.nbr + s;
     double g00 = 0, g01 = 0, g10 = 0, g11 = 0, gt0 = 0, gt1 = 0;
-    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int, int) {
-        for (int t = t0; t < t1; ++t) {
-            const double2 rj = A.r_s[t];
-            double dx = rj.x - ri.x, dy = rj.y - ri.y;
+    for (int k0 = 0; k0 < ns; k0 += U64) {
+        int t[U64];
+        double2 rj[U64], vv[U64];
+        double mj[U64], rhoj[U64], Tj[U64];
+#pragma unroll
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            rj[u] = A.r[t[u]];
+            vv[u] = A.vp[t[u]];
+            mj[u] = A.m[t[u]];
+            rhoj[u] = A.rho[t[u]].x;
+            Tj[u] = A.T[t[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
             min_image(dx, dy, B);
             const double r2 = rsq_exact(dx, dy);
-            if (r2 < cut && t != s) {
+            if (k0 + u < ns && r2 < cut) {
                 const double rr = sqrt(r2);
                 const double q = rr * B.ih, w = 1.0 - q;
                 const double F = B.fh * w * w;
                 const double gx = -F * dx, gy = -F * dy;
-                const double vj = A.m_s[t] / A.rho[t].x;
-                const double2 vv = A.vp_s[t];
-                const double dvx = vv.x - vi.x, dvy = vv.y - vi.y;
-                const double dT = A.T[t] - Ti;
+                const double vj = mj[u] / rhoj[u];
+                const double dvx = vv[u].x - vi.x, dvy = vv[u].y - vi.y;
+                const double dT = Tj[u] - Ti;
                 g00 += vj * dvx * gx;
                 g01 += vj * dvx * gy;
                 g10 += vj * dvy * gx;
@@ -278,7 +410,7 @@ __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int 
                 gt1 += vj * dT * gy;
             }
         }
-    });
+    }
     const double rho = A.rho[s].x, r2i = rho * rho;
     double pxx = 0, pxy = 0, pyy = 0;
     if (B.terms & 4u) {
@@ -302,100 +434,111 @@ __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int 
     }
 }
 
-// P3: rates and the fused leapfrog.
+// P3: rates.
 //   f   = (S_i + S_j) . dwdx_h + (Sco_i + Sco_j) dwdx_hl
 //   a_i = sum_j m_j f;   udot_i = sum_j m_j (dv . f / 2 + (Q_i + Q_j) . dwdx_h)
-// mode 0: rates only (store acc/udot, state unchanged);  mode 1: step --
-// running boxes kick by dt/2 at k = 0 and k = nsteps, by dt otherwise, and
-// drift (with periodic wrap) while k < nsteps; other boxes copy through.
-__global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int mode)
+__global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
 {
     const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
-    const BoxDyn &D = A.dyn[b];
-    const double2 ri = A.r_s[s], vh = A.v_s[s], vi = A.vp_s[s];
-    const double uh = A.u_s[s];
-    if (mode == 1 && D.state != BOX_RUN) {
-        A.r[s] = ri;
-        A.v[s] = vh;
-        A.u[s] = uh;
-        A.acc[s] = A.acc_s[s];
-        A.udot[s] = A.udot_s[s];
-        return;
-    }
+    if (!state_in(A.dyn[b].state, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
     const bool cond = (B.terms & 8u) != 0;
+    const double2 ri = A.r[s], vi = A.vp[s];
     const double4 Si = A.S[s];
     const double2 Qi = cond ? A.Q[s] : make_double2(0.0, 0.0);
+    const int2 c = A.cnt[s];
+    const size_t cap = A.cap;
+    const unsigned *nb = A.nbr + s;
+    const double cutl = B.cut2 < B.hl2 ? B.cut2 : B.hl2;
     double ax = 0, ay = 0, ud = 0;
-    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int, int) {
-        for (int t = t0; t < t1; ++t) {
-            const double2 rj = A.r_s[t];
-            double dx = rj.x - ri.x, dy = rj.y - ri.y;
+    for (int k0 = 0; k0 < c.x; k0 += U64) {
+        int t[U64];
+        double2 rj[U64], vv[U64], Qj[U64];
+        double4 Sj[U64];
+        double mj[U64];
+#pragma unroll
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            rj[u] = A.r[t[u]];
+            vv[u] = A.vp[t[u]];
+            Sj[u] = A.S[t[u]];
+            mj[u] = A.m[t[u]];
+            Qj[u] = cond ? A.Q[t[u]] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
             min_image(dx, dy, B);
             const double r2 = rsq_exact(dx, dy);
-            if (r2 < B.cut2 && t != s) {
+            if (k0 + u < c.x && r2 < B.cut2) {
                 const double rr = sqrt(r2);
-                const double4 Sj = A.S[t];
-                double fx = 0.0, fy = 0.0, c = 0.0;
+                double fx = 0.0, fy = 0.0, cc = 0.0;
                 if (r2 < B.h2) {
                     const double q = rr * B.ih, w = 1.0 - q;
                     const double F = B.fh * w * w;
                     const double wx = F * dx, wy = F * dy;
-                    const double axx = Si.x + Sj.x, axy = Si.y + Sj.y, ayy = Si.z + Sj.z;
+                    const double axx = Si.x + Sj[u].x, axy = Si.y + Sj[u].y, ayy = Si.z + Sj[u].z;
                     fx = axx * wx + axy * wy;
                     fy = axy * wx + ayy * wy;
-                    if (cond) {
-                        const double2 Qj = A.Q[t];
-                        c = (Qi.x + Qj.x) * wx + (Qi.y + Qj.y) * wy;
-                    }
+                    if (cond) cc = (Qi.x + Qj[u].x) * wx + (Qi.y + Qj[u].y) * wy;
                 }
                 if (r2 < B.hl2) {
                     const double q = rr * B.ihl, w = 1.0 - q;
                     const double Fl = B.fhl * w * w;
-                    const double cs = Si.w + Sj.w;
+                    const double cs = Si.w + Sj[u].w;
                     fx = fx + cs * (Fl * dx);
                     fy = fy + cs * (Fl * dy);
                 }
-                const double mj = A.m_s[t];
-                const double2 vv = A.vp_s[t];
-                const double e = 0.5 * ((vv.x - vi.x) * fx + (vv.y - vi.y) * fy);
-                ax += mj * fx;
-                ay += mj * fy;
-                ud += mj * (e + c);
+                const double e = 0.5 * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                ax += mj[u] * fx;
+                ay += mj[u] * fy;
+                ud += mj[u] * (e + cc);
             }
         }
-    });
+    }
+    const int kl = A.lcap - c.y;
+    for (int k0 = kl; k0 < A.lcap; k0 += U64) {
+        int t[U64];
+        double2 rj[U64], vv[U64];
+        double Sw[U64], mj[U64];
+#pragma unroll
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < A.lcap ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            rj[u] = A.r[t[u]];
+            vv[u] = A.vp[t[u]];
+            Sw[u] = A.S[t[u]].w;
+            mj[u] = A.m[t[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
+            min_image(dx, dy, B);
+            const double r2 = rsq_exact(dx, dy);
+            if (k0 + u < A.lcap && r2 < cutl) {
+                const double rr = sqrt(r2);
+                const double q = rr * B.ihl, w = 1.0 - q;
+                const double Fl = B.fhl * w * w;
+                const double cs = Si.w + Sw[u];
+                const double fx = cs * (Fl * dx), fy = cs * (Fl * dy);
+                const double e = 0.5 * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                ax += mj[u] * fx;
+                ay += mj[u] * fy;
+                ud += mj[u] * e;
+            }
+        }
+    }
     if (!isfinite(ax) || !isfinite(ay) || !isfinite(ud)) flag_health(A.dyn, b, HEALTH_NONFINITE);
     A.acc[s] = make_double2(ax, ay);
     A.udot[s] = ud;
-    if (mode == 0) {
-        A.r[s] = ri;
-        A.v[s] = vh;
-        A.u[s] = uh;
-        return;
-    }
-    const int64_t k = D.k, K = D.nsteps;
-    const double kick = (k == 0 || k == K) ? 0.5 * B.dt : B.dt;
-    const double vx = vh.x + kick * ax, vy = vh.y + kick * ay;
-    const double un = uh + kick * ud;
-    const double mi = A.m_s[s];
-    if (k == 0) A.e[s] = mi * (0.5 * (vh.x * vh.x + vh.y * vh.y) + uh);
-    if (k == K) A.e[s] = mi * (0.5 * (vx * vx + vy * vy) + un);
-    double2 rn = ri;
-    if (k < K) {
-        rn.x = wrap_coord(ri.x + B.dt * vx, B.Lx);
-        rn.y = wrap_coord(ri.y + B.dt * vy, B.Ly);
-    }
-    A.r[s] = rn;
-    A.v[s] = make_double2(vx, vy);
-    A.u[s] = un;
 }
 
 // ===========================================================================
-// FP32-compensated form: pair arithmetic in FP32 on cell-relative
+// FP32-compensated form: pair arithmetic in FP32 on build-cell-relative
 // coordinates, Kahan-compensated sums, FP64 EOS / integration / energy.
 // ===========================================================================
 
@@ -415,8 +558,9 @@ struct Kahan {
 
 struct BoxF {
     float cwx, cwy, Lx, Ly, hLx, hLy;
-    float cut2, h2, hl2, cuts;
+    float cut2, cutl, h2, cuts;
     float ih, ihl, qh, qhl, fh, fhl;
+    int ncx, ncy, sx, sy;
 };
 
 __device__ __forceinline__ BoxF boxf(const BoxDesc &B)
@@ -424,25 +568,30 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B)
     BoxF f;
     f.cwx = (float)B.cwx; f.cwy = (float)B.cwy;
     f.Lx = (float)B.Lx; f.Ly = (float)B.Ly; f.hLx = (float)B.hLx; f.hLy = (float)B.hLy;
-    f.cut2 = (float)B.cut2; f.h2 = (float)B.h2; f.hl2 = (float)B.hl2;
+    f.cut2 = (float)B.cut2; f.cutl = (float)(B.cut2 < B.hl2 ? B.cut2 : B.hl2); f.h2 = (float)B.h2;
     f.cuts = (float)(B.R2list < B.h2 ? B.R2list : B.h2);
     f.ih = (float)B.ih; f.ihl = (float)B.ihl; f.qh = (float)B.qh; f.qhl = (float)B.qhl;
     f.fh = (float)B.fh; f.fhl = (float)B.fhl;
+    f.ncx = B.ncx; f.ncy = B.ncy; f.sx = B.sx; f.sy = B.sy;
     return f;
 }
 
-// Separation of candidate rj from ri (both cell-relative) in stencil cell
-// (ox, oy); boxes with a single cell along an axis use the minimum image.
-__device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, float ox, float oy,
-                                      const BoxDesc &B, const BoxF &F, float &dx, float &dy)
+// Separation of rj from ri, both relative to their build-cell corners: the
+// cell offset (with the periodic wrap of the cell index) carries the image;
+// boxes with a single cell along an axis use the minimum image instead.
+__device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const BoxF &F, float &dx, float &dy)
 {
-    dx = (rj.x - ri.x) + ox * F.cwx;
-    dy = (rj.y - ri.y) + oy * F.cwy;
-    if (B.sx == 0) {
+    const int ci = __float_as_int(ri.w), cj = __float_as_int(rj.w);
+    int ox = (cj & 0xffff) - (ci & 0xffff), oy = (cj >> 16) - (ci >> 16);
+    ox = ox > 1 ? ox - F.ncx : (ox < -1 ? ox + F.ncx : ox);
+    oy = oy > 1 ? oy - F.ncy : (oy < -1 ? oy + F.ncy : oy);
+    dx = (rj.x - ri.x) + (float)ox * F.cwx;
+    dy = (rj.y - ri.y) + (float)oy * F.cwy;
+    if (F.sx == 0) {
         dx = dx > F.hLx ? dx - F.Lx : dx;
         dx = dx < -F.hLx ? dx + F.Lx : dx;
     }
-    if (B.sy == 0) {
+    if (F.sy == 0) {
         dy = dy > F.hLy ? dy - F.Ly : dy;
         dy = dy < -F.hLy ? dy + F.Ly : dy;
     }
@@ -459,34 +608,60 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     const BoxDesc &B = A.boxes[b];
     const BoxF F = boxf(B);
     const float4 ri = A.rf[s];
+    const int2 c = A.cnt[s];
+    const size_t cap = A.cap;
+    const unsigned *nb = A.nbr + s;
     Kahan rho(F.qh), rhol(F.qhl);
-    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int ox, int oy) {
-        for (int t = t0; t < t1; ++t) {
-            const float4 rj = A.rf[t];
+    for (int k0 = 0; k0 < c.x; k0 += U32) {
+        int t[U32];
+        float4 rj[U32];
+#pragma unroll
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U32; ++u) rj[u] = A.rf[t[u]];
+#pragma unroll
+        for (int u = 0; u < U32; ++u) {
             float dx, dy;
-            sep32(ri, rj, (float)ox, (float)oy, B, F, dx, dy);
+            sep32(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
-            if (r2 < F.cut2 && t != s) {
+            if (k0 + u < c.x && r2 < F.cut2) {
                 const float rr = sqrtf(r2);
-                if (r2 < F.hl2) {
+                if (r2 < F.cutl) {
                     const float q = rr * F.ihl, w = 1.f - q;
-                    rhol.add(rj.z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
+                    rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
                 }
                 if (r2 < F.h2) {
                     const float q = rr * F.ih, w = 1.f - q;
-                    rho.add(rj.z * (F.qh * (1.f + 3.f * q) * (w * w * w)));
+                    rho.add(rj[u].z * (F.qh * (1.f + 3.f * q) * (w * w * w)));
                 }
             }
         }
-    });
-    A.m[s] = A.m_s[s];
-    A.id[s] = A.id_s[s];
+    }
+    for (int k0 = A.lcap - c.y; k0 < A.lcap; k0 += U32) {
+        int t[U32];
+        float4 rj[U32];
+#pragma unroll
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < A.lcap ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U32; ++u) rj[u] = A.rf[t[u]];
+#pragma unroll
+        for (int u = 0; u < U32; ++u) {
+            float dx, dy;
+            sep32(ri, rj[u], F, dx, dy);
+            const float r2 = dx * dx + dy * dy;
+            if (k0 + u < A.lcap && r2 < F.cutl) {
+                const float rr = sqrtf(r2);
+                const float q = rr * F.ihl, w = 1.f - q;
+                rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
+            }
+        }
+    }
     const double rd = rho.value(), rl = rhol.value();
-    double ui = A.up_s[s];
+    double ui = A.up[s];
     if (st == BOX_INIT) {
         ui = B.kB * B.T0 - B.a * rd;
-        A.u_s[s] = ui;
-        A.up_s[s] = ui;
+        A.u[s] = ui;
+        A.up[s] = ui;
     }
     double T = (ui + B.a * rd) / B.kB;
     T = T < 0.0 ? 0.0 : T;
@@ -519,22 +694,37 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
     const float4 ri = A.rf[s];
     const float2 vi = A.vf[s];
     const float Ti = A.T32[s];
+    const int ns = A.cnt[s].x;
+    const size_t cap = A.cap;
+    const unsigned *nb = A.nbr + s;
     Kahan g00, g01, g10, g11, gt0, gt1;
-    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int ox, int oy) {
-        for (int t = t0; t < t1; ++t) {
-            const float4 rj = A.rf[t];
+    for (int k0 = 0; k0 < ns; k0 += U32) {
+        int t[U32];
+        float4 rj[U32];
+        float2 vv[U32];
+        float rhoj[U32], Tj[U32];
+#pragma unroll
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U32; ++u) {
+            rj[u] = A.rf[t[u]];
+            vv[u] = A.vf[t[u]];
+            rhoj[u] = A.rho32[t[u]].x;
+            Tj[u] = A.T32[t[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U32; ++u) {
             float dx, dy;
-            sep32(ri, rj, (float)ox, (float)oy, B, F, dx, dy);
+            sep32(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
-            if (r2 < F.cuts && t != s) {
+            if (k0 + u < ns && r2 < F.cuts) {
                 const float rr = sqrtf(r2);
                 const float q = rr * F.ih, w = 1.f - q;
                 const float Fk = F.fh * w * w;
                 const float gx = -Fk * dx, gy = -Fk * dy;
-                const float vj = rj.z / A.rho32[t].x;
-                const float2 vv = A.vf[t];
-                const float dvx = vv.x - vi.x, dvy = vv.y - vi.y;
-                const float dT = A.T32[t] - Ti;
+                const float vj = rj[u].z / rhoj[u];
+                const float dvx = vv[u].x - vi.x, dvy = vv[u].y - vi.y;
+                const float dT = Tj[u] - Ti;
                 g00.add(vj * dvx * gx);
                 g01.add(vj * dvx * gy);
                 g10.add(vj * dvy * gx);
@@ -543,7 +733,7 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
                 gt1.add(vj * dT * gy);
             }
         }
-    });
+    }
     const double G00 = g00.value(), G01 = g01.value(), G10 = g10.value(), G11 = g11.value();
     const double rho = A.rho[s].x, r2i = rho * rho;
     double qx = 0, qy = 0;
@@ -570,23 +760,13 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
     }
 }
 
-__global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int mode)
+__global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
 {
     const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
-    const BoxDyn &D = A.dyn[b];
-    const double2 rid = A.r_s[s], vid = A.v_s[s];
-    const double ui = A.u_s[s];
-    if (mode == 1 && D.state != BOX_RUN) {
-        A.r[s] = rid;
-        A.v[s] = vid;
-        A.u[s] = ui;
-        A.acc[s] = A.acc_s[s];
-        A.udot[s] = A.udot_s[s];
-        return;
-    }
+    if (!state_in(A.dyn[b].state, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
     const BoxF F = boxf(B);
     const bool cond = (B.terms & 8u) != 0;
@@ -594,78 +774,150 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int mode)
     const float2 vi = A.vf[s];
     const float4 Si = A.S32[s];
     const float2 Qi = cond ? A.Q32[s] : make_float2(0.f, 0.f);
+    const int2 c = A.cnt[s];
+    const size_t cap = A.cap;
+    const unsigned *nb = A.nbr + s;
     Kahan ax, ay, ud;
-    for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int ox, int oy) {
-        for (int t = t0; t < t1; ++t) {
-            const float4 rj = A.rf[t];
+    for (int k0 = 0; k0 < c.x; k0 += U32) {
+        int t[U32];
+        float4 rj[U32], Sj[U32];
+        float2 vv[U32], Qj[U32];
+#pragma unroll
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U32; ++u) {
+            rj[u] = A.rf[t[u]];
+            Sj[u] = A.S32[t[u]];
+            vv[u] = A.vf[t[u]];
+            Qj[u] = cond ? A.Q32[t[u]] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < U32; ++u) {
             float dx, dy;
-            sep32(ri, rj, (float)ox, (float)oy, B, F, dx, dy);
+            sep32(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
-            if (r2 < F.cut2 && t != s) {
+            if (k0 + u < c.x && r2 < F.cut2) {
                 const float rr = sqrtf(r2);
-                const float4 Sj = A.S32[t];
-                float fx = 0.f, fy = 0.f, c = 0.f;
+                float fx = 0.f, fy = 0.f, cc = 0.f;
                 if (r2 < F.h2) {
                     const float q = rr * F.ih, w = 1.f - q;
                     const float Fk = F.fh * w * w;
                     const float wx = Fk * dx, wy = Fk * dy;
-                    const float axx = Si.x + Sj.x, axy = Si.y + Sj.y, ayy = Si.z + Sj.z;
+                    const float axx = Si.x + Sj[u].x, axy = Si.y + Sj[u].y, ayy = Si.z + Sj[u].z;
                     fx = axx * wx + axy * wy;
                     fy = axy * wx + ayy * wy;
-                    if (cond) {
-                        const float2 Qj = A.Q32[t];
-                        c = (Qi.x + Qj.x) * wx + (Qi.y + Qj.y) * wy;
-                    }
+                    if (cond) cc = (Qi.x + Qj[u].x) * wx + (Qi.y + Qj[u].y) * wy;
                 }
-                if (r2 < F.hl2) {
+                if (r2 < F.cutl) {
                     const float q = rr * F.ihl, w = 1.f - q;
                     const float Fl = F.fhl * w * w;
-                    const float cs = Si.w + Sj.w;
+                    const float cs = Si.w + Sj[u].w;
                     fx = fx + cs * (Fl * dx);
                     fy = fy + cs * (Fl * dy);
                 }
-                const float mj = rj.z;
-                const float2 vv = A.vf[t];
-                const float e = 0.5f * ((vv.x - vi.x) * fx + (vv.y - vi.y) * fy);
+                const float mj = rj[u].z;
+                const float e = 0.5f * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
                 ax.add(mj * fx);
                 ay.add(mj * fy);
-                ud.add(mj * (e + c));
+                ud.add(mj * (e + cc));
             }
         }
-    });
+    }
+    for (int k0 = A.lcap - c.y; k0 < A.lcap; k0 += U32) {
+        int t[U32];
+        float4 rj[U32];
+        float2 vv[U32];
+        float Sw[U32];
+#pragma unroll
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < A.lcap ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U32; ++u) {
+            rj[u] = A.rf[t[u]];
+            vv[u] = A.vf[t[u]];
+            Sw[u] = A.S32[t[u]].w;
+        }
+#pragma unroll
+        for (int u = 0; u < U32; ++u) {
+            float dx, dy;
+            sep32(ri, rj[u], F, dx, dy);
+            const float r2 = dx * dx + dy * dy;
+            if (k0 + u < A.lcap && r2 < F.cutl) {
+                const float rr = sqrtf(r2);
+                const float q = rr * F.ihl, w = 1.f - q;
+                const float Fl = F.fhl * w * w;
+                const float cs = Si.w + Sw[u];
+                const float fx = cs * (Fl * dx), fy = cs * (Fl * dy);
+                const float mj = rj[u].z;
+                const float e = 0.5f * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                ax.add(mj * fx);
+                ay.add(mj * fy);
+                ud.add(mj * e);
+            }
+        }
+    }
     const double axd = ax.value(), ayd = ay.value(), udd = ud.value();
     if (!isfinite(axd) || !isfinite(ayd) || !isfinite(udd)) flag_health(A.dyn, b, HEALTH_NONFINITE);
     A.acc[s] = make_double2(axd, ayd);
     A.udot[s] = udd;
-    if (mode == 0) {
-        A.r[s] = rid;
-        A.v[s] = vid;
-        A.u[s] = ui;
-        return;
-    }
+}
+
+// ===========================================================================
+// P4 leapfrog, shared by both forms (FP64 state)
+// ===========================================================================
+
+// Running boxes kick by dt/2 at k = 0 and k = nsteps and by dt otherwise,
+// and drift (with periodic wrap) while k < nsteps.  Between steps v and u are
+// half-step values; vp / up get the prediction to the next integer time.
+// The energy sum m (v^2/2 + u) of each slot is captured at the first and
+// last synchronised state.
+__global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    const BoxDyn &D = A.dyn[b];
+    if (D.state != BOX_RUN) return;
+    const BoxDesc &B = A.boxes[b];
     const int64_t k = D.k, K = D.nsteps;
     const double kick = (k == 0 || k == K) ? 0.5 * B.dt : B.dt;
-    const double vx = vid.x + kick * axd, vy = vid.y + kick * ayd;
-    const double un = ui + kick * udd;
-    const double mi = A.m_s[s];
-    if (k == 0) A.e[s] = mi * (0.5 * (vid.x * vid.x + vid.y * vid.y) + ui);
-    if (k == K) A.e[s] = mi * (0.5 * (vx * vx + vy * vy) + un);
-    double2 rn = rid;
-    if (k < K) {
-        rn.x = wrap_coord(rid.x + B.dt * vx, B.Lx);
-        rn.y = wrap_coord(rid.y + B.dt * vy, B.Ly);
-    }
-    A.r[s] = rn;
+    const double2 v = A.v[s], a = A.acc[s];
+    const double u = A.u[s], ud = A.udot[s], m = A.m[s];
+    if (k == 0) A.e[s] = m * (0.5 * (v.x * v.x + v.y * v.y) + u);
+    const double vx = v.x + kick * a.x, vy = v.y + kick * a.y;
+    const double un = u + kick * ud;
     A.v[s] = make_double2(vx, vy);
     A.u[s] = un;
+    if (k == K) {
+        A.e[s] = m * (0.5 * (vx * vx + vy * vy) + un);
+        A.vp[s] = make_double2(vx, vy);
+        A.up[s] = un;
+        if (fp32) A.vf[s] = make_float2((float)vx, (float)vy);
+        return;
+    }
+    const double2 r = A.r[s];
+    double2 rn;
+    rn.x = wrap_coord(r.x + B.dt * vx, B.Lx);
+    rn.y = wrap_coord(r.y + B.dt * vy, B.Ly);
+    A.r[s] = rn;
+    const double hdt = 0.5 * B.dt;  // predictor to the next integer time
+    const double2 vp = make_double2(vx + hdt * a.x, vy + hdt * a.y);
+    A.vp[s] = vp;
+    A.up[s] = un + hdt * ud;
+    const double2 r0 = A.r_ref[s];
+    double ddx = rn.x - r0.x, ddy = rn.y - r0.y;
+    min_image(ddx, ddy, B);
+    if (ddx * ddx + ddy * ddy > B.hskin2) flag_health(A.dyn, b, HEALTH_SKIN);
+    if (fp32) {
+        A.rf[s] = rel_pos(rn, A.pcell[s], m, B);
+        A.vf[s] = make_float2((float)vp.x, (float)vp.y);
+    }
 }
 
 // ===========================================================================
 // per-box bookkeeping
 // ===========================================================================
 
-// Total energy sum m (v^2/2 + u) of boxes at their first (k = 0) or last
-// (k = nsteps) synchronised state; one block per box, fixed summation order.
 __global__ __launch_bounds__(NT) void k_box_energy(ArenaView A, int nbox)
 {
     const int b = blockIdx.x;
@@ -688,12 +940,39 @@ __global__ __launch_bounds__(NT) void k_box_energy(ArenaView A, int nbox)
     }
 }
 
-// INIT -> RUN (k = 0); RUN: k++ until k == nsteps, then DONE.
-__global__ void k_advance(BoxDyn *dyn, int nbox)
+// Steps a box may reuse the list it just built: the largest c with
+//   c dt vmax + (c dt)^2 amax <= 0.4 skin    (twice the kinematic bound on
+// the acceleration term, 80 % of the half skin), plus the build step.
+__device__ __forceinline__ int64_t reuse_steps(const BoxDesc &B, const BoxDyn &D)
+{
+    const double vmax = sqrt(__longlong_as_double((long long)D.vmax2));
+    const double amax = sqrt(__longlong_as_double((long long)D.amax2));
+    const double C = 0.4 * B.skin, Bq = B.dt * vmax, Aq = B.dt * B.dt * amax;
+    double c;
+    if (Aq > 0.0) c = (-Bq + sqrt(Bq * Bq + 4.0 * Aq * C)) / (2.0 * Aq);
+    else if (Bq > 0.0) c = C / Bq;
+    else c = 1e9;
+    int64_t K = (int64_t)floor(c) + 1;
+    if (amax == 0.0 && K > 2) K = 2;  // no rates yet: re-measure soon
+    return K < 1 ? 1 : (K > 64 ? 64 : K);
+}
+
+// INIT -> RUN (k = 0); RUN: k++ until k == nsteps, then DONE.  Boxes that
+// built this step get their next build step from their own speed and
+// acceleration, so a box's schedule never depends on the other boxes.
+__global__ void k_advance(ArenaView A, int nbox)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbox) return;
-    BoxDyn &D = dyn[b];
+    BoxDyn &D = A.dyn[b];
+    const bool built = building(D, 0);
+    if (built) {
+        const int64_t K = reuse_steps(A.boxes[b], D);
+        D.next_rebuild = (D.state == BOX_INIT ? 0 : D.k) + K;
+        D.builds += 1;
+        D.vmax2 = 0ull;
+        D.amax2 = 0ull;
+    }
     if (D.state == BOX_INIT) {
         D.state = D.nsteps > 0 ? BOX_RUN : BOX_DONE;
         D.k = 0;
@@ -718,19 +997,25 @@ __global__ __launch_bounds__(NT) void k_upload(ArenaView A, int off, int n, int 
     const int k = A.id[s];
     switch (field) {
     case 0: A.r[s] = make_double2(src[2 * k], src[2 * k + 1]); break;
-    case 1: A.v[s] = make_double2(src[2 * k], src[2 * k + 1]); break;
+    case 1: {
+        const double2 v = make_double2(src[2 * k], src[2 * k + 1]);
+        A.v[s] = v;
+        A.vp[s] = v;
+    } break;
     case 2: A.m[s] = src[k]; break;
-    case 3: A.u[s] = src[k]; break;
+    case 3: A.u[s] = src[k]; A.up[s] = src[k]; break;
     default: break;
     }
 }
 
-__global__ __launch_bounds__(NT) void k_init_ids(ArenaView A, int off, int n)
+__global__ __launch_bounds__(NT) void k_init_slots(ArenaView A, int off, int n)
 {
     const int i = blockIdx.x * NT + threadIdx.x;
     if (i >= n) return;
-    A.id[off + i] = i;
-    A.id_s[off + i] = i;
+    const int s = off + i;
+    A.id[s] = i;
+    A.acc[s] = make_double2(0.0, 0.0);
+    A.udot[s] = 0.0;
 }
 
 __global__ __launch_bounds__(NT) void k_download(ArenaView A, int box, int field,
@@ -773,7 +1058,7 @@ __global__ __launch_bounds__(NT) void k_download(ArenaView A, int box, int field
 // (a) pair export, separations, kernel and minimum image on demand
 // ===========================================================================
 
-// inv[local id] = sorted slot
+// inv[local id] = slot
 __global__ __launch_bounds__(NT) void k_inverse(const int *__restrict__ ids, int off, int n,
                                                  int *__restrict__ inv)
 {
@@ -783,7 +1068,8 @@ __global__ __launch_bounds__(NT) void k_inverse(const int *__restrict__ ids, int
 }
 
 // Pairs (i, j), i < j, rsq < R2list: count per i (pass 0) or write the j's
-// of i into its segment, sorted ascending (pass 1).  i runs over local ids.
+// of i into its segment, sorted ascending (pass 1).  i runs over local ids;
+// the cells are those of the build that precedes this call.
 __global__ __launch_bounds__(NT) void k_pairs(ArenaView A, int box, const int *__restrict__ inv,
                                                unsigned *__restrict__ count,
                                                const unsigned *__restrict__ offs,
@@ -793,14 +1079,14 @@ __global__ __launch_bounds__(NT) void k_pairs(ArenaView A, int box, const int *_
     const int i = blockIdx.x * NT + threadIdx.x;
     if (i >= B.n) return;
     const int s = inv[i];
-    const double2 ri = A.r_s[s];
+    const double2 ri = A.r[s];
     unsigned c = 0;
     const unsigned o = pass ? offs[i] : 0u;
     for_each_cell(B, A.cell_start, A.pcell[s], [&](int t0, int t1, int, int) {
         for (int t = t0; t < t1; ++t) {
-            const int j = A.id_s[t];
+            const int j = A.id[t];
             if (j <= i) continue;
-            const double2 rj = A.r_s[t];
+            const double2 rj = A.r[t];
             double dx = rj.x - ri.x, dy = rj.y - ri.y;
             min_image(dx, dy, B);
             if (rsq_exact(dx, dy) < B.R2list) {
@@ -843,7 +1129,7 @@ __global__ __launch_bounds__(NT) void k_pairs_emit(int n, const unsigned *__rest
     }
 }
 
-// neighbour_list.py:63-83 on a given pair list (local ids, canonical state).
+// neighbour_list.py:63-83 on a given pair list (local ids, current state).
 __global__ __launch_bounds__(NT) void k_separations(ArenaView A, int box, const int *__restrict__ inv,
                                                      long long np, const long long *__restrict__ iap,
                                                      double *__restrict__ drij, double *__restrict__ rij,
@@ -913,11 +1199,10 @@ __global__ __launch_bounds__(NT) void k_set_u(ArenaView A, int box, const double
     const double t = T ? T[A.id[s]] : T0;
     const double u = B.kB * t - B.a * A.rho[s].x;
     A.u[s] = u;
-    A.u_s[s] = u;
-    A.up_s[s] = u;
+    A.up[s] = u;
 }
 
-// out3 = (sum m v^2/2, sum m u, total) over one box's canonical state.
+// out3 = (sum m v^2/2, sum m u, total) over one box's state.
 __global__ __launch_bounds__(NT) void k_energy_now(ArenaView A, int box, double *__restrict__ out3)
 {
     const BoxDesc &B = A.boxes[box];
@@ -953,17 +1238,21 @@ __global__ __launch_bounds__(NT) void k_energy_now(ArenaView A, int box, double 
 
 static inline dim3 grid_for(long long n) { return dim3((unsigned)((n + NT - 1) / NT > 0 ? (n + NT - 1) / NT : 1)); }
 
-void launch_keys(const ArenaView &A, int state_mask, hipStream_t st)
+void launch_keys(const ArenaView &A, int force_mask, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_keys, grid_for(A.n), dim3(NT), 0, st, A, state_mask);
+    hipLaunchKernelGGL(k_keys, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
 }
-void launch_permute(const ArenaView &A, int fp32, hipStream_t st)
+void launch_permute(const ArenaView &A, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_permute, grid_for(A.n), dim3(NT), 0, st, A, fp32);
+    hipLaunchKernelGGL(k_permute, grid_for(A.n), dim3(NT), 0, st, A);
 }
 void launch_cell_start(const ArenaView &A, hipStream_t st)
 {
     hipLaunchKernelGGL(k_cell_start, grid_for((long long)A.n + 1), dim3(NT), 0, st, A);
+}
+void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_build_lists, grid_for(A.n), dim3(NT), 0, st, A, force_mask, fp32);
 }
 void launch_density(const ArenaView &A, int fp32, int state_mask, hipStream_t st)
 {
@@ -975,26 +1264,30 @@ void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, hipStre
     if (fp32) hipLaunchKernelGGL(k_grad32, grid_for(A.n), dim3(NT), 0, st, A, state_mask, diag);
     else hipLaunchKernelGGL(k_grad64, grid_for(A.n), dim3(NT), 0, st, A, state_mask, diag);
 }
-void launch_rates(const ArenaView &A, int fp32, int mode, hipStream_t st)
+void launch_rates(const ArenaView &A, int fp32, int state_mask, hipStream_t st)
 {
-    if (fp32) hipLaunchKernelGGL(k_rates32, grid_for(A.n), dim3(NT), 0, st, A, mode);
-    else hipLaunchKernelGGL(k_rates64, grid_for(A.n), dim3(NT), 0, st, A, mode);
+    if (fp32) hipLaunchKernelGGL(k_rates32, grid_for(A.n), dim3(NT), 0, st, A, state_mask);
+    else hipLaunchKernelGGL(k_rates64, grid_for(A.n), dim3(NT), 0, st, A, state_mask);
+}
+void launch_integrate(const ArenaView &A, int fp32, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_integrate, grid_for(A.n), dim3(NT), 0, st, A, fp32);
 }
 void launch_box_energy(const ArenaView &A, int nbox, hipStream_t st)
 {
     if (nbox > 0) hipLaunchKernelGGL(k_box_energy, dim3(nbox), dim3(NT), 0, st, A, nbox);
 }
-void launch_advance(BoxDyn *dyn, int nbox, hipStream_t st)
+void launch_advance(const ArenaView &A, int nbox, hipStream_t st)
 {
-    if (nbox > 0) hipLaunchKernelGGL(k_advance, grid_for(nbox), dim3(NT), 0, st, dyn, nbox);
+    if (nbox > 0) hipLaunchKernelGGL(k_advance, grid_for(nbox), dim3(NT), 0, st, A, nbox);
 }
 void launch_upload(const ArenaView &A, int off, int n, int field, const double *src, hipStream_t st)
 {
     hipLaunchKernelGGL(k_upload, grid_for(n), dim3(NT), 0, st, A, off, n, field, src);
 }
-void launch_init_ids(const ArenaView &A, int off, int n, hipStream_t st)
+void launch_init_slots(const ArenaView &A, int off, int n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_init_ids, grid_for(n), dim3(NT), 0, st, A, off, n);
+    hipLaunchKernelGGL(k_init_slots, grid_for(n), dim3(NT), 0, st, A, off, n);
 }
 void launch_download(const ArenaView &A, int box, int n, int field, double *dst, hipStream_t st)
 {

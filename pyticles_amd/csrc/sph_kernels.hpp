@@ -11,16 +11,18 @@ constexpr int MASK_RUN = (1 << BOX_RUN);
 constexpr int MASK_INIT = (1 << BOX_INIT);
 constexpr int MASK_ALL = (1 << BOX_INIT) | (1 << BOX_RUN) | (1 << BOX_DONE);
 
-void launch_keys(const ArenaView &A, int state_mask, hipStream_t st);
-void launch_permute(const ArenaView &A, int fp32, hipStream_t st);
+void launch_keys(const ArenaView &A, int force_mask, hipStream_t st);
+void launch_permute(const ArenaView &A, hipStream_t st);
 void launch_cell_start(const ArenaView &A, hipStream_t st);
+void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_t st);
 void launch_density(const ArenaView &A, int fp32, int state_mask, hipStream_t st);
 void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, hipStream_t st);
-void launch_rates(const ArenaView &A, int fp32, int mode, hipStream_t st);
+void launch_rates(const ArenaView &A, int fp32, int state_mask, hipStream_t st);
+void launch_integrate(const ArenaView &A, int fp32, hipStream_t st);
 void launch_box_energy(const ArenaView &A, int nbox, hipStream_t st);
-void launch_advance(BoxDyn *dyn, int nbox, hipStream_t st);
+void launch_advance(const ArenaView &A, int nbox, hipStream_t st);
 void launch_upload(const ArenaView &A, int off, int n, int field, const double *src, hipStream_t st);
-void launch_init_ids(const ArenaView &A, int off, int n, hipStream_t st);
+void launch_init_slots(const ArenaView &A, int off, int n, hipStream_t st);
 void launch_download(const ArenaView &A, int box, int n, int field, double *dst, hipStream_t st);
 void launch_inverse(const int *ids, int off, int n, int *inv, hipStream_t st);
 void launch_pairs(const ArenaView &A, int box, int n, const int *inv, unsigned *count,

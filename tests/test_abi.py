@@ -33,8 +33,8 @@ def test_version_and_error_strings_without_gpu():
 
 
 def test_params_struct_layout():
-    # 13 doubles + unsigned + int, no padding surprises
-    assert ctypes.sizeof(_lib.Params) == 13 * 8 + 8
+    # 14 doubles + unsigned + int, no padding surprises
+    assert ctypes.sizeof(_lib.Params) == 14 * 8 + 8
 
 
 def test_library_is_gfx950_code_object():
