@@ -55,7 +55,7 @@ def workload(name, precision):
 
 def run_gpu(s, steps, warmup, timing=True):
     from pyticles_amd import _lib
-    from pyticles_amd.box import from_system
+    from pyticles_amd.device import from_system
     b = from_system(s)
     if warmup:
         b.step(warmup)
@@ -93,7 +93,7 @@ def error_vs_oracle(s, sub=None):
     """Normwise error of one rate evaluation of the GPU path at the initial
     state against the FP64 oracle."""
     from oracle import oracle as O
-    from pyticles_amd.box import from_system
+    from pyticles_amd.device import from_system
     u0 = O.init_u(s.r, s.m, s.T, s.prm)
     ref = O.derivatives(s.r, s.v, s.m, u0, s.prm, terms=s.prm["terms"])
     with from_system(s) as b:

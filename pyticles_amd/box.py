@@ -1,147 +1,51 @@
-"""Device-resident box: the NumPy-facing handle over sph_box.
+"""Simulation boxes with the reference's surface (box.py).
 
-A Box owns one periodic 2-D system in HBM.  Every method is a C-ABI call
-into libpyticles_hip.so; host arrays are copied in and out only when asked.
+    Box(p, xmax, ymax, zmax)   box.py:16-26
+    PeriodicBox                box.py:28-47
+    MirrorBox                  box.py:49-74
+
+The device leapfrog wraps positions periodically with
+r - L floor(r / L) on the GPU; PeriodicBox.apply is kept for host-side
+callers and translates an escaped coordinate by the box side (the
+reference sets it to 0 or xmax, box.py:36-47, which is not a translation).
+MirrorBox reflects at the walls as the reference does; the device path is
+periodic only, so a system with a MirrorBox cannot be stepped on the GPU.
 """
-import ctypes
-
 import numpy as np
-
-from . import _lib
-from .configs import to_struct
-
-
-def _f64(a, shape=None):
-    a = np.ascontiguousarray(a, dtype=np.float64)
-    if shape is not None and a.shape != shape:
-        raise ValueError("expected shape %s, got %s" % (shape, a.shape))
-    return a
 
 
 class Box:
-    FIELDS = ("r", "v", "u", "rho", "rho_lr", "T", "p", "pco", "acc", "udot", "gradv", "jq", "P")
+    periodic = False
 
-    def __init__(self, n, prm, device=0):
-        self.n = int(n)
-        self.prm = dict(prm)
-        self.ctx = _lib.context(device)
-        self._h = _lib._P()
-        _lib.check(_lib.lib.sph_box_create(self.ctx, self.n, ctypes.byref(to_struct(self.prm)),
-                                           ctypes.byref(self._h)))
+    def __init__(self, p="none", xmax=64, ymax=48, zmax=100):
+        self.p = p
+        self.xmax = xmax
+        self.ymax = ymax
+        self.zmax = zmax
 
-    # -- lifetime ---------------------------------------------------------
-    def close(self):
-        if self._h:
-            _lib.lib.sph_box_destroy(self._h)
-            self._h = _lib._P()
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
-
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        self.close()
-
-    # -- state ------------------------------------------------------------
-    def set_params(self, **kw):
-        self.prm.update(kw)
-        _lib.check(_lib.lib.sph_box_set_params(self._h, ctypes.byref(to_struct(self.prm))))
-
-    def upload(self, r=None, v=None, m=None, u=None):
-        n = self.n
-        r = None if r is None else _f64(r, (n, 2))
-        v = None if v is None else _f64(v, (n, 2))
-        m = None if m is None else _f64(m, (n,))
-        u = None if u is None else _f64(u, (n,))
-        _lib.check(_lib.lib.sph_box_upload(self._h, _lib.dptr(r), _lib.dptr(v), _lib.dptr(m), _lib.dptr(u)))
-
-    def download(self, *names):
-        """Arrays in the caller's particle order, e.g. box.download('r', 'rho')."""
-        shapes = {"r": (2,), "v": (2,), "acc": (2,), "jq": (2,), "gradv": (2, 2), "P": (2, 2)}
-        out = {}
-        ptrs = []
-        for f in self.FIELDS:
-            if f in names:
-                a = np.empty((self.n,) + shapes.get(f, ()))
-                out[f] = a
-                ptrs.append(_lib.dptr(a))
-            else:
-                ptrs.append(None)
-        for nm in names:
-            if nm not in self.FIELDS:
-                raise KeyError(nm)
-        _lib.check(_lib.lib.sph_box_download(self._h, *ptrs))
-        return out[names[0]] if len(names) == 1 else out
-
-    # -- passes -----------------------------------------------------------
-    def density(self):
-        _lib.check(_lib.lib.sph_density(self._h))
-
-    def gradients(self):
-        _lib.check(_lib.lib.sph_gradients(self._h))
-
-    def rates(self, terms=_lib.ALL_TERMS):
-        _lib.check(_lib.lib.sph_rates(self._h, int(terms)))
-
-    def init_thermal(self, T):
-        if np.isscalar(T):
-            _lib.check(_lib.lib.sph_init_thermal(self._h, None, float(T)))
-        else:
-            T = _f64(T, (self.n,))
-            _lib.check(_lib.lib.sph_init_thermal(self._h, _lib.dptr(T), 0.0))
-
-    def step(self, nsteps):
-        _lib.check(_lib.lib.sph_step(self._h, int(nsteps)))
-
-    def energy(self):
-        ke, ue, et = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        _lib.check(_lib.lib.sph_energy(self._h, ctypes.byref(ke), ctypes.byref(ue), ctypes.byref(et)))
-        return ke.value, ue.value, et.value
-
-    def set_graph(self, on):
-        _lib.check(_lib.lib.sph_box_set_graph(self._h, 1 if on else 0))
-
-    def timing(self, reset=False):
-        out = np.zeros(7)
-        _lib.check(_lib.lib.sph_box_timing(self._h, _lib.dptr(out), 7, 1 if reset else 0))
-        return dict(zip(("build_ms", "density_ms", "gradients_ms", "rates_ms", "pairs_ms", "steps",
-                         "integrate_ms"), out))
-
-    # -- (a) neighbour list ----------------------------------------------
-    def pairs(self, cap=None):
-        """Every unordered pair (i<j) with rsq < rc^2 + tol^2, lexicographic."""
-        if cap is None:
-            cap = max(64, 16 * self.n)
-        while True:
-            iap = np.empty((cap, 2), dtype=np.int64)
-            nip = ctypes.c_int64()
-            rc = _lib.lib.sph_nlist_build(self._h, cap, _lib.i64ptr(iap), ctypes.byref(nip))
-            if rc == _lib.SPH_ERR_CAP:
-                cap = int(nip.value)
-                continue
-            _lib.check(rc)
-            return iap[: nip.value].copy()
-
-    def separations(self, iap):
-        iap = np.ascontiguousarray(iap, dtype=np.int64)
-        nip = iap.shape[0]
-        drij = np.empty((nip, 2))
-        dv = np.empty((nip, 2))
-        rij = np.empty(nip)
-        rsq = np.empty(nip)
-        _lib.check(_lib.lib.sph_separations(self._h, nip, _lib.i64ptr(iap), _lib.dptr(drij), _lib.dptr(rij),
-                                            _lib.dptr(rsq), _lib.dptr(dv)))
-        return drij, rij, rsq, dv
+    def apply(self, p=None):
+        print("Do nothing")
 
 
-def from_system(sysm, device=0):
-    """Box holding a configs.System with u = kB T - a rho set on device."""
-    b = Box(sysm.n, sysm.prm, device)
-    b.upload(r=sysm.r, v=sysm.v, m=sysm.m, u=np.zeros(sysm.n))
-    b.init_thermal(sysm.T)
-    return b
+class PeriodicBox(Box):
+    periodic = True
+
+    def apply(self, p):
+        n = p.n
+        for c, L in ((0, self.xmax), (1, self.ymax)):
+            x = p.r[:n, c]
+            x[:] = np.mod(x, L)
+            x[x >= L] -= L
+
+
+class MirrorBox(Box):
+    def apply(self, p):
+        n = p.n
+        for c, L in ((0, self.xmax), (1, self.ymax)):
+            x = p.r[:n, c]
+            v = p.v[:n, c]
+            hi = x > L
+            lo = x < 0
+            x[hi] = L
+            x[lo] = 0
+            v[hi | lo] = -v[hi | lo]

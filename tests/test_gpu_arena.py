@@ -1,0 +1,108 @@
+"""Batched and served forms: boxes of uneven size share one arena; each
+box's result must be bitwise the one it gets alone (same per-box list
+schedule, same summation order), including boxes admitted while others run
+and boxes placed in reused slots."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pyticles_amd import _lib, configs
+from pyticles_amd.arena import Arena
+from pyticles_amd.device import from_system
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(sysm, precision):
+    s = configs.System(sysm.r, sysm.v, sysm.m, sysm.T, dict(sysm.prm, precision=precision), sysm.steps, sysm.seed)
+    with from_system(s) as b:
+        e_start = b.energy()[2]
+        b.step(sysm.steps)
+        out = b.download("r", "v", "u", "rho")
+        out["E0"], out["E1"] = e_start, b.energy()[2]
+    return out
+
+
+def small_boxes():
+    return [configs.lognormal_box(seed, 256, 2048, 0.4, 1.0, steps=st) for seed, st in
+            ((101, 30), (102, 17), (103, 41))]
+
+
+def run_to_completion(ar, ids, limit=500):
+    for _ in range(limit):
+        st, k, hl = ar.poll_all()
+        assert not np.any(hl[ids]), "health flags %s" % hl
+        if all(st[i] == _lib.BOX_DONE for i in ids):
+            return
+        ar.step(1)
+    raise AssertionError("boxes did not finish")
+
+
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32C])
+def test_batched_equals_alone(precision):
+    boxes = small_boxes()
+    ref = [alone(s, precision) for s in boxes]
+    with Arena(sum(s.n for s in boxes) + 1000, precision=precision) as ar:
+        ids = [ar.add(s) for s in boxes]
+        run_to_completion(ar, ids)
+        for i, s, rf in zip(ids, boxes, ref):
+            got = ar.result(i)
+            for k in ("r", "v", "u", "rho"):
+                assert np.array_equal(got[k], rf[k]), (precision, i, k)
+            assert got["E0"] == pytest.approx(rf["E0"], rel=1e-12)
+            assert got["E1"] == pytest.approx(rf["E1"], rel=1e-12)
+
+
+def test_served_admission_mid_run_and_slot_reuse():
+    prec = _lib.FP32C
+    a, b, c = small_boxes()
+    ra, rb, rc = (alone(s, prec) for s in (a, b, c))
+    with Arena(a.n + b.n + 64, precision=prec) as ar:
+        ia = ar.add(a)
+        ar.step(7)
+        ib = ar.add(b)  # joins while a runs
+        run_to_completion(ar, [ia, ib])
+        for i, rf in ((ia, ra), (ib, rb)):
+            got = ar.result(i)
+            assert np.array_equal(got["r"], rf["r"]) and np.array_equal(got["u"], rf["u"])
+        ar.remove(ia)
+        big = configs.lognormal_box(105, a.n + 200, a.n + 201, 0.5, 0.6, steps=5)
+        with pytest.raises(_lib.SphError) as ei:
+            ar.add(big)  # no free run of slots is long enough
+        assert ei.value.code == _lib.SPH_ERR_NOMEM
+        small = configs.lognormal_box(104, 256, 300, 0.5, 0.6, steps=12)
+        rs = alone(small, prec)
+        isml = ar.add(small)  # reuses a's slots
+        run_to_completion(ar, [isml])
+        got = ar.result(isml)
+        assert np.array_equal(got["r"], rs["r"]) and np.array_equal(got["v"], rs["v"])
+
+
+def test_arena_fp64_matches_oracle():
+    s = configs.lognormal_box(7, 400, 500, 0.7, 0.8, steps=10, precision=_lib.FP64)
+    with Arena(s.n + 10, precision=_lib.FP64) as ar:
+        i = ar.add(s)
+        run_to_completion(ar, [i])
+        got = ar.result(i)
+    u0 = O.init_u(s.r, s.m, s.T, s.prm)
+    r1, v1, u1 = O.run(s.r, s.v, s.m, u0, s.prm, 10)
+    L = s.prm["Lx"]
+    dr = got["r"] - r1
+    dr = np.where(dr > L / 2, dr - L, np.where(dr < -L / 2, dr + L, dr))
+    assert np.max(np.abs(dr)) < 1e-10
+    assert np.max(np.abs(got["v"] - v1)) < 1e-10 * np.max(np.abs(v1))
+
+
+def test_fp32c_trajectory_error_vs_fp64():
+    s = configs.lognormal_box(9, 1500, 1600, 0.6, 0.7, steps=50, precision=_lib.FP64)
+    r64 = alone(s, _lib.FP64)
+    r32 = alone(s, _lib.FP32C)
+    L = s.prm["Lx"]
+    dr = r32["r"] - r64["r"]
+    dr = np.where(dr > L / 2, dr - L, np.where(dr < -L / 2, dr + L, dr))
+    assert np.max(np.abs(dr)) < 1e-5
+    assert np.max(np.abs(r32["rho"] - r64["rho"])) / np.max(r64["rho"]) < 1e-5
+    # conserved total energy: drift of the compensated form stays at the FP64 level
+    d64 = abs(r64["E1"] - r64["E0"]) / abs(r64["E0"])
+    d32 = abs(r32["E1"] - r32["E0"]) / abs(r32["E0"])
+    assert d32 < 10 * d64 + 1e-6

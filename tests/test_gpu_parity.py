@@ -9,7 +9,7 @@ import pytest
 
 from oracle import oracle as O
 from pyticles_amd import _lib, configs
-from pyticles_amd.box import Box, from_system
+from pyticles_amd.device import DeviceBox as Box, from_system
 
 pytestmark = pytest.mark.gpu
 
