@@ -1,14 +1,26 @@
 """Benchmark of the smooth-particle hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
-                    [--precision fp64|fp32c] [--no-cpu] [--no-extra]
+                    [--precision fp64|fp32c] [--legs c2x,c3,c4,c5] [--no-cpu]
+                    [--no-extra] [--tune PATH] [--calib-copy]
 
-Default workload: one periodic box 512 x 512, N = 262,144 jittered lattice,
-T = 0.9, full physics (pressure, cohesion, viscosity eta = 1, conduction
-k = 5), FP64, leapfrog dt = 0.005.  A step is one leapfrog step of the whole
-hot path (cell build + sort, density + EOS, gradients, rates, kick/drift);
-inputs are resident in HBM when the timed region starts.  Prints one JSON
-line (rank 0, N = 1 only).
+Headline: one periodic box 512 x 512, N = 262,144 jittered lattice, T = 0.9,
+full physics (pressure, cohesion, viscosity eta = 1, conduction k = 5),
+FP64, leapfrog dt = 0.005.  A step is one leapfrog step of the whole hot
+path (cell build + sort when due, density + EOS, gradients, rates,
+kick/drift); inputs are resident in HBM when the timed region starts.
+
+Further legs (each adds an object to the same JSON line):
+    c2x  the same box in the FP32-compensated form, errors of both forms
+         against the FP64 oracle
+    c3   quench, N = 1,048,576, FP32-compensated vs FP64 on the device:
+         per-field error against the FP64 oracle and total-energy drift
+    c4   512 boxes of N log-uniform in [256, 16384], 500 steps each,
+         batched FP32-compensated arena vs the plain FP64 form one box per
+         launch (the >= 5x comparison)
+    c5   served Poisson job stream: throughput, p50 / p99 job latency
+--tune PATH times every launch schedule for each workload class and writes
+the table to PATH (the autotuner's cache format).  Prints one JSON line.
 """
 import argparse
 import json
@@ -53,10 +65,20 @@ def workload(name, precision):
     return s, desc
 
 
-def run_gpu(s, steps, warmup, timing=True):
+def schedule(s, cls):
+    """Apply the autotuned (or default) schedule of a workload class to a
+    configs.System: sets its skin and returns (unroll, schedule)."""
+    from pyticles_amd import autotune
+    sc = autotune.schedule_for(s.prm["precision"], cls)
+    s.prm["skin"] = sc["skin"]
+    return sc["unroll"], sc
+
+
+def run_gpu(s, steps, warmup, timing=True, unroll=0):
     from pyticles_amd import _lib
     from pyticles_amd.device import from_system
     b = from_system(s)
+    b.set_schedule(unroll)
     if warmup:
         b.step(warmup)
     _lib.check(_lib.lib.sph_ctx_set_timing(b.ctx, 1 if timing else 0))
@@ -89,21 +111,186 @@ def cpu_baseline(s, target_s=12.0):
                        "%.1f s); single thread, pair list rebuilt every evaluation" % (nsteps, s.n, nsteps + 1, el))
 
 
-def error_vs_oracle(s, sub=None):
-    """Normwise error of one rate evaluation of the GPU path at the initial
-    state against the FP64 oracle."""
+def error_vs_oracle(s, r=None, v=None, u=None, unroll=0):
+    """Normwise error max|x - x_ref| / max|x_ref| of one rate evaluation of
+    the device path against the FP64 oracle, at the initial state of s or at
+    the state (r, v, u)."""
     from oracle import oracle as O
     from pyticles_amd.device import from_system
-    u0 = O.init_u(s.r, s.m, s.T, s.prm)
-    ref = O.derivatives(s.r, s.v, s.m, u0, s.prm, terms=s.prm["terms"])
+    r = s.r if r is None else r
+    v = s.v if v is None else v
+    u = O.init_u(s.r, s.m, s.T, s.prm) if u is None else u
+    ref = O.derivatives(r, v, s.m, u, s.prm, terms=s.prm["terms"])
     with from_system(s) as b:
-        b.upload(u=u0)
+        b.set_schedule(unroll)
+        b.upload(r=r, v=v, u=u)
         b.rates(s.prm["terms"])
         got = b.download("rho", "rho_lr", "acc", "udot")
-    out = {}
-    for k in ("rho", "rho_lr", "acc", "udot"):
-        out[k] = float(np.max(np.abs(got[k] - ref[k])) / np.max(np.abs(ref[k])))
+    return {k: float(np.max(np.abs(got[k] - ref[k])) / np.max(np.abs(ref[k])))
+            for k in ("rho", "rho_lr", "acc", "udot")}
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+# ---- legs ------------------------------------------------------------------
+
+def leg_c2x(value64, steps, warmup):
+    from pyticles_amd import _lib
+    s32, _ = workload("c2", "fp32c")
+    un, sc = schedule(s32, "box")
+    b32, el32, _, (e0, e1) = run_gpu(s32, steps, warmup, timing=False, unroll=un)
+    b32.close()
+    v32 = s32.n * steps / el32
+    out = {"value": v32, "unit": "particle-steps/s", "ms_per_step": 1e3 * el32 / steps,
+           "speedup_vs_fp64": v32 / value64, "energy_drift_rel": abs(e1 - e0) / abs(e0),
+           "schedule": sc}
+    out["rel_err_vs_fp64_oracle"] = error_vs_oracle(s32, unroll=un)
+    s64, _ = workload("c2", "fp64")
+    un64, _ = schedule(s64, "box")
+    out["fp64_rel_err_vs_oracle"] = error_vs_oracle(s64, unroll=un64)
+    assert s32.prm["precision"] == _lib.FP32C
     return out
+
+
+def leg_c3(steps):
+    """Quench at N = 1,048,576: both device forms run the same steps; errors
+    of one rate evaluation against the FP64 oracle at the initial state and
+    at the FP64 run's final (phase-separating) state."""
+    from pyticles_amd import _lib, configs
+    out = {"workload": "C3: periodic 1024x1024, N=1,048,576 jittered lattice at T=1.2, velocities rescaled to "
+                       "T=0.4 and u reset at step 0, eta=1, k=5, dt=0.005, %d steps" % steps}
+    res = {}
+    for name, prec in (("fp32c", _lib.FP32C), ("fp64", _lib.FP64)):
+        s = configs.c3(precision=prec)
+        un, sc = schedule(s, "box")
+        log("c3 %s %d steps" % (name, steps))
+        b, el, _, (e0, e1) = run_gpu(s, steps, 0, timing=False, unroll=un)
+        fin = b.download("r", "v", "u", "rho")
+        b.close()
+        res[name] = (s, un, fin)
+        out[name] = {"value": s.n * steps / el, "unit": "particle-steps/s", "ms_per_step": 1e3 * el / steps,
+                     "energy_drift_rel": abs(e1 - e0) / abs(e0), "E0": e0, "E1": e1, "schedule": sc,
+                     "rho_range_final": [float(fin["rho"].min()), float(fin["rho"].max())]}
+    out["fp32c"]["speedup_vs_fp64"] = out["fp32c"]["value"] / out["fp64"]["value"]
+    s32, un32, _ = res["fp32c"]
+    s64, un64, fin64 = res["fp64"]
+    log("c3 errors vs oracle")
+    out["fp32c"]["rel_err_vs_fp64_oracle_initial"] = error_vs_oracle(s32, unroll=un32)
+    out["fp32c"]["rel_err_vs_fp64_oracle_final_state"] = error_vs_oracle(
+        s32, r=fin64["r"], v=fin64["v"], u=fin64["u"], unroll=un32)
+    out["fp64"]["rel_err_vs_fp64_oracle_final_state"] = error_vs_oracle(
+        s64, r=fin64["r"], v=fin64["v"], u=fin64["u"], unroll=un64)
+    return out
+
+
+def run_arena(boxes, precision, chunk=25):
+    """All boxes in one arena until every box is done; (seconds, results)."""
+    from pyticles_amd import autotune
+    from pyticles_amd.arena import Arena
+    sc = autotune.schedule_for(precision, "batched")
+    total = sum(b.n for b in boxes)
+    ar = Arena(total, precision=precision)
+    ar.set_schedule(sc["unroll"])
+    ids = [ar.add(autotune._Resched(b, autotune.with_schedule(b, sc, precision))) for b in boxes]
+    ar.poll_all()
+    t = time.perf_counter()
+    while True:
+        ar.step(chunk)
+        st, _, hl = ar.poll_all()
+        if np.all((st[ids] == 3) | (hl[ids] != 0)):
+            break
+    el = time.perf_counter() - t
+    res = [ar.result(i) for i in ids]
+    _, _, hl = ar.poll_all()
+    ar.close()
+    return el, res, int(np.count_nonzero(hl[ids])), sc
+
+
+def leg_c4(nbox=512, plain_every=4):
+    """Batched FP32C (and FP64) arena over all boxes vs the plain FP64 form,
+    one box per launch sequence, on every plain_every-th box."""
+    from pyticles_amd import _lib, configs
+    from pyticles_amd.device import from_system
+    boxes = [configs.c4_box(b) for b in range(nbox)]
+    steps = boxes[0].steps
+    ps = sum(b.n for b in boxes) * steps
+    out = {"workload": "C4: %d periodic boxes, N_b log-uniform in [256, 16384] (total %d), density 1, T_b in "
+                       "[0.3, 1.0], eta=1, k=5, %d steps each" % (nbox, sum(b.n for b in boxes), steps)}
+    for name, prec in (("batched_fp32c", _lib.FP32C), ("batched_fp64", _lib.FP64)):
+        bx = [configs.c4_box(b, precision=prec) for b in range(nbox)] if prec != _lib.FP32C else boxes
+        log("c4 %s" % name)
+        el, res, failed, sc = run_arena(bx, prec)
+        drift = np.array([abs(r["E1"] - r["E0"]) / abs(r["E0"]) for r in res])
+        out[name] = {"value": ps / el, "unit": "particle-steps/s", "seconds": el, "failed_boxes": failed,
+                     "energy_drift_rel_median": float(np.median(drift)), "energy_drift_rel_max": float(drift.max()),
+                     "schedule": sc}
+    sample = list(range(0, nbox, plain_every))
+    el = 0.0
+    pss = 0
+    log("c4 plain fp64, %d boxes one at a time" % len(sample))
+    for i in sample:
+        sb = configs.c4_box(i, precision=_lib.FP64)
+        b = from_system(sb)
+        b.step(1)
+        t = time.perf_counter()
+        b.step(steps)
+        el += time.perf_counter() - t
+        pss += sb.n * steps
+        b.close()
+    out["plain_fp64_one_box_per_launch"] = {
+        "value": pss / el, "unit": "particle-steps/s", "seconds": el,
+        "sample": "boxes %s (every %dth of %d), %d steps each, timed per box excluding set-up" % (
+            "0,%d,...,%d" % (plain_every, sample[-1]), plain_every, nbox, steps)}
+    out["batched_fp32c_over_plain_fp64"] = out["batched_fp32c"]["value"] / out["plain_fp64_one_box_per_launch"]["value"]
+    out["batched_fp64_over_plain_fp64"] = out["batched_fp64"]["value"] / out["plain_fp64_one_box_per_launch"]["value"]
+    return out
+
+
+def leg_c5(njobs, warm, rate, max_resident, budget):
+    from pyticles_amd import _lib, autotune, configs
+    from pyticles_amd.serve import Server, latency_summary
+    sc = autotune.schedule_for(_lib.FP32C, "batched")
+    out = {"workload": "C5: Poisson job stream, N log-uniform in [1024, 65536], steps in [100, 2000], T in "
+                       "[0.3, 1.2], eta=1, k=5, FP32C", "budget_bytes": budget, "max_resident": max_resident,
+           "schedule": sc}
+
+    def serve(nj, wj, lam, tag):
+        jobs = [configs.c5_job(j) for j in range(nj)]
+        arr = configs.c5_arrivals(nj, rate=lam)
+        offered = sum(j.n * j.steps for j in jobs) / arr[-1]
+        log("c5 %s: %d jobs at %.1f/s (offered %.3g particle-steps/s)" % (tag, nj, lam, offered))
+        srv = Server(budget, precision=_lib.FP32C, max_resident=max_resident, schedule=sc, chunk=4)
+        recs = srv.run(jobs, arr, log=log)
+        srv.close()
+        sm = latency_summary(recs, warmup=wj)
+        sm.update(rate_jobs_per_s=lam, jobs_total=nj, warmup_jobs=wj, offered_particle_steps_per_s=offered,
+                  arena_capacity=srv.capacity)
+        sm["saturated"] = bool(sm["particle_steps_per_s"] < 0.9 * offered)
+        return sm
+
+    out["stream_200"] = serve(njobs, warm, rate, "stream")
+    cap = out["stream_200"]["particle_steps_per_s"]
+    mean_work = np.mean([j.n * j.steps for j in (configs.c5_job(k) for k in range(200))])
+    lam2 = 0.6 * cap / mean_work
+    n2 = max(100, min(600, int(lam2 * 15)))
+    out["stream_below_capacity"] = serve(n2, n2 // 10, lam2, "below capacity")
+    return out
+
+
+def tune_all(path):
+    from pyticles_amd import _lib, autotune, configs
+    res = {}
+    for prec in (_lib.FP64, _lib.FP32C):
+        s = configs.c2()
+        s.prm["precision"] = prec
+        best, table = autotune.tune(s, prec, "box", path=path, log=log)
+        res[autotune.key(prec, "box")] = best
+        boxes = [configs.c4_box(b, precision=prec) for b in range(0, 512, 8)]
+        best, table = autotune.tune(boxes, prec, "batched", path=path, log=log)
+        res[autotune.key(prec, "batched")] = best
+    return res
 
 
 def load_traffic(key):
@@ -123,8 +310,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"])
+    ap.add_argument("--legs", default="c2x,c3,c4,c5", help="comma list of extra legs (c2x, c3, c4, c5) or ''")
+    ap.add_argument("--c3-steps", type=int, default=2000)
+    ap.add_argument("--c5-jobs", type=int, default=2200)
+    ap.add_argument("--c5-warmup", type=int, default=200)
+    ap.add_argument("--c5-rate", type=float, default=200.0)
+    ap.add_argument("--c5-resident", type=int, default=2_000_000)
+    ap.add_argument("--c5-budget", type=float, default=200e9)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-extra", action="store_true", help="skip the FP32C comparison and error checks")
+    ap.add_argument("--no-extra", action="store_true", help="skip every extra leg")
+    ap.add_argument("--tune", metavar="PATH", help="run the autotuner for every workload class, write PATH, exit")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -146,12 +341,16 @@ def main():
               "across GPUs, which no run here exercises)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.tune:
+        print(json.dumps({"tuned": tune_all(args.tune), "path": args.tune}), flush=True)
+        return
+
     s, desc = workload(args.workload, args.precision)
+    unroll, sched = schedule(s, "box")
     full = bool(s.prm["terms"] & 12)
     bps = B_PS[(args.precision, full)]
-    print("[bench] %s, %s, %d steps after %d warmup" % (desc, args.precision, args.steps, args.warmup),
-          file=sys.stderr, flush=True)
-    b, el, tm, (E0, E1) = run_gpu(s, args.steps, args.warmup)
+    log("%s, %s, %d steps after %d warmup, schedule %s" % (desc, args.precision, args.steps, args.warmup, sched))
+    b, el, tm, (E0, E1) = run_gpu(s, args.steps, args.warmup, unroll=unroll)
     nip = int(b.pairs().shape[0])
     b.close()
     value = s.n * args.steps / el
@@ -180,7 +379,8 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+kahan/f64",
         "data": "synthetic (NumPy PCG64 seeded jittered lattice, Gaussian velocities)",
         "config": {"workload": desc, "N": s.n, "box": [s.prm["Lx"], s.prm["Ly"]], "precision": args.precision,
-                   "timed": "%d leapfrog steps (%d rate evaluations incl. the closing half kick)" % (args.steps, evals)},
+                   "timed": "%d leapfrog steps (%d rate evaluations incl. the closing half kick)" % (args.steps, evals),
+                   "schedule": sched},
         "roofline": roof,
         "pair_interactions_per_s": nip * args.steps / el,
         "nip": nip,
@@ -188,19 +388,19 @@ def main():
         "step_roofline_frac": round(value * step_bytes / (HBM_PEAK_GBS * 1e9), 5),
         "energy_drift_rel": abs(E1 - E0) / abs(E0),
     }
-    if not args.no_extra and args.workload == "c2" and args.precision == "fp64":
-        s32, _ = workload("c2", "fp32c")
-        b32, el32, tm32, (e0, e1) = run_gpu(s32, args.steps, args.warmup, timing=False)
-        b32.close()
-        v32 = s32.n * args.steps / el32
-        line["fp32c"] = {"value": v32, "ms_per_step": 1e3 * el32 / args.steps, "speedup_vs_fp64": v32 / value,
-                         "energy_drift_rel": abs(e1 - e0) / abs(e0)}
-        try:
-            line["fp32c"]["rel_err_vs_fp64_oracle"] = error_vs_oracle(s32)
-            line["fp64_rel_err_vs_oracle"] = error_vs_oracle(workload("c2", "fp64")[0])
-        except Exception as exc:  # noqa: BLE001
-            line["fp32c"]["rel_err_vs_fp64_oracle"] = "failed: %s" % exc
+    legs = [] if args.no_extra else [x for x in args.legs.split(",") if x]
+    is_head = args.workload == "c2" and args.precision == "fp64"
+    if "c2x" in legs and is_head:
+        log("c2 fp32c")
+        line["c2_fp32c"] = leg_c2x(value, args.steps, args.warmup)
+    if "c3" in legs:
+        line["c3_precision"] = leg_c3(args.c3_steps)
+    if "c4" in legs:
+        line["c4_batched"] = leg_c4()
+    if "c5" in legs:
+        line["c5_served"] = leg_c5(args.c5_jobs, args.c5_warmup, args.c5_rate, args.c5_resident, args.c5_budget)
     if not args.no_cpu:
+        log("cpu baseline")
         s_cpu, _ = workload(args.workload, "fp64")
         line["cpu_baseline"] = cpu_baseline(s_cpu)
         line["cpu_baseline"]["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]

@@ -169,7 +169,8 @@ int sph_arena_step(sph_arena *ar, int64_t nsteps);
 /* Per-box progress: steps done (or -1 while initialising) and done flag. */
 int sph_arena_poll(sph_arena *ar, int box_id, int64_t *steps_done, int *done);
 /* States (0 free, 1 initialising, 2 running, 3 done), step index and health
- * bits of box ids 0..nmax-1 with one device read. */
+ * bits of box ids 0..nmax-1 with one device read (ids never used or removed
+ * read as free). */
 int sph_arena_poll_all(sph_arena *ar, int nmax, int *state, int64_t *k, int *health);
 /* Copy a box's state out (same layout as sph_box_download; any NULL). */
 int sph_arena_result(sph_arena *ar, int box_id, double *r, double *v, double *u,
@@ -182,6 +183,15 @@ int sph_arena_stats(sph_arena *ar, int64_t *used, int64_t *capacity,
 /* Replay the step as a captured HIP graph (default on) or launch directly. */
 int sph_arena_set_graph(sph_arena *ar, int on);
 int sph_box_set_graph(sph_box *box, int on);
+/* Launch schedule of the pair passes: list entries whose loads are in
+ * flight per lane (1, 2 or 4; 0 = the precision's default).  Chosen by the
+ * autotuner (pyticles_amd/autotune.py); any value gives the same results up
+ * to summation order.  No counterpart in the reference. */
+int sph_arena_set_schedule(sph_arena *ar, int unroll);
+int sph_box_set_schedule(sph_box *box, int unroll);
+/* Device bytes an arena of particle_capacity slots allocates (the serving
+ * loop's HBM-budget admission).  No counterpart in the reference. */
+int sph_arena_footprint(int64_t particle_capacity, int precision, int64_t *bytes);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,30 @@ from . import _lib
 from .configs import to_struct
 
 
+def footprint(capacity, precision=_lib.FP32C):
+    """Device bytes of an arena with `capacity` particle slots."""
+    b = ctypes.c_int64()
+    _lib.check(_lib.lib.sph_arena_footprint(int(capacity), int(precision), ctypes.byref(b)))
+    return int(b.value)
+
+
+def capacity_for(budget_bytes, precision=_lib.FP32C):
+    """Largest slot capacity whose arena fits in budget_bytes of HBM."""
+    lo, hi = 0, 1
+    while footprint(hi, precision) <= budget_bytes:
+        lo, hi = hi, hi * 2
+        if hi > 2 ** 31 - 1:
+            hi = 2 ** 31 - 1
+            break
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        if footprint(mid, precision) <= budget_bytes:
+            lo = mid
+        else:
+            hi = mid
+    return lo
+
+
 class Arena:
     def __init__(self, capacity, precision=_lib.FP32C, device=0, graph=True):
         self.capacity = int(capacity)
@@ -49,6 +73,10 @@ class Arena:
 
     def set_graph(self, on):
         _lib.check(_lib.lib.sph_arena_set_graph(self._h, 1 if on else 0))
+
+    def set_schedule(self, unroll):
+        """List entries in flight per lane in the pair passes (1, 2, 4; 0 = default)."""
+        _lib.check(_lib.lib.sph_arena_set_schedule(self._h, int(unroll)))
 
     def add(self, sysm, nsteps=None, T0=None):
         """Admit a configs.System; returns its box id.  Raises SphError with

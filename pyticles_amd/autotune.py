@@ -1,0 +1,168 @@
+"""Launch-schedule autotuner for the pair passes.
+
+Two knobs set how the device walks a box's neighbours, and neither changes
+which pairs interact (only the summation order, so results agree to
+rounding):
+
+    unroll  list entries whose loads are in flight per lane in the density,
+            gradient and rate passes (1, 2 or 4): more hides more gather
+            latency, fewer keeps more waves resident
+    skin    neighbour-list skin: lists hold pairs within r_cut + skin and are
+            reused while no particle can have crossed half of it; a larger
+            skin rebuilds less often and walks longer lists
+
+The best pair depends on precision (FP64 registers are twice as wide), on box
+size (many small boxes share launches with a sort over all of them) and on
+density.  tune() times every candidate on a sample of the caller's workload
+through the same entry points the workload uses and keeps the fastest;
+results are cached per (precision, workload class) in schedules.json beside
+this file so later runs start from the measured choice.
+
+No counterpart in the reference (its loops have no launch schedule).
+"""
+import itertools
+import json
+import math
+import os
+import time
+
+from . import _lib
+
+UNROLLS = (1, 2, 4)
+SKINS = (0.3, 0.5, 0.75)
+CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "schedules.json")
+
+DEFAULT = {_lib.FP64: dict(unroll=2, skin=0.5), _lib.FP32C: dict(unroll=4, skin=0.5)}
+
+
+def prec_name(precision):
+    return "fp32c" if int(precision) == _lib.FP32C else "fp64"
+
+
+def key(precision, cls):
+    """Cache key of a workload class: 'box' (one large box per launch) or
+    'batched' (many boxes sharing an arena's launches)."""
+    if cls not in ("box", "batched"):
+        raise ValueError("workload class must be 'box' or 'batched', got %r" % (cls,))
+    return "%s:%s" % (prec_name(precision), cls)
+
+
+def load_cache(path=CACHE):
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        data = json.load(f)
+    if not isinstance(data, dict):
+        raise ValueError("%s: expected a JSON object" % path)
+    return data
+
+
+def save_cache(data, path=CACHE):
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+        f.write("\n")
+    os.replace(tmp, path)
+
+
+def valid(s):
+    return (isinstance(s, dict) and s.get("unroll") in UNROLLS and isinstance(s.get("skin"), (int, float))
+            and 0.0 < float(s["skin"]) <= 2.0)
+
+
+def schedule_for(precision, cls, path=CACHE):
+    """The cached schedule of a workload class, else the precision's default.
+    Returns dict(unroll, skin, source)."""
+    entry = load_cache(path).get(key(precision, cls))
+    if entry and valid(entry.get("best")):
+        b = entry["best"]
+        return dict(unroll=int(b["unroll"]), skin=float(b["skin"]), source="tuned")
+    d = DEFAULT[int(precision)]
+    return dict(d, source="default")
+
+
+def candidates():
+    return [dict(unroll=u, skin=s) for u, s in itertools.product(UNROLLS, SKINS)]
+
+
+def with_schedule(sysm, sched, precision):
+    """A copy of a configs.System's parameters carrying the schedule's skin."""
+    return dict(sysm.prm, skin=float(sched["skin"]), precision=int(precision))
+
+
+def _time_box(sysm, sched, precision, steps, warm):
+    from .device import from_system
+    b = from_system(_Resched(sysm, with_schedule(sysm, sched, precision)))
+    try:
+        b.set_schedule(sched["unroll"])
+        b.step(warm)
+        t = time.perf_counter()
+        b.step(steps)
+        dt = time.perf_counter() - t
+    finally:
+        b.close()
+    return sysm.n * steps / dt
+
+
+def _time_batched(systems, sched, precision, steps, warm):
+    from .arena import Arena
+    total = sum(s.n for s in systems)
+    ar = Arena(total, precision=precision)
+    try:
+        ar.set_schedule(sched["unroll"])
+        for s in systems:
+            ar.add(_Resched(s, with_schedule(s, sched, precision)), nsteps=steps + warm + 10)
+        ar.step(warm)
+        ar.poll_all()  # synchronises
+        t = time.perf_counter()
+        ar.step(steps)
+        ar.poll_all()
+        dt = time.perf_counter() - t
+    finally:
+        ar.close()
+    return total * steps / dt
+
+
+class _Resched:
+    """A configs.System view with replaced parameters."""
+
+    def __init__(self, s, prm):
+        self.r, self.v, self.m, self.T, self.steps, self.n = s.r, s.v, s.m, s.T, s.steps, s.n
+        self.prm = prm
+
+
+def tune(systems, precision, cls=None, steps=None, warm=5, save=True, path=CACHE, log=None):
+    """Time every candidate schedule on `systems` (one large box, or a list
+    of boxes run batched) and return (best, table).  With save=True the
+    result is stored under key(precision, cls) in the cache."""
+    if not isinstance(systems, (list, tuple)):
+        systems = [systems]
+    if cls is None:
+        cls = "box" if len(systems) == 1 else "batched"
+    if steps is None:
+        steps = 40 if cls == "box" else 20
+    table = []
+    for c in candidates():
+        if cls == "box":
+            rate = _time_box(systems[0], c, precision, steps, warm)
+        else:
+            rate = _time_batched(systems, c, precision, steps, warm)
+        table.append(dict(c, rate=rate))
+        if log:
+            log("tune %s unroll=%d skin=%.2f  %.3e particle-steps/s" % (key(precision, cls), c["unroll"],
+                                                                        c["skin"], rate))
+    best = max(table, key=lambda e: e["rate"])
+    best = dict(unroll=best["unroll"], skin=best["skin"])
+    if save:
+        data = load_cache(path)
+        data[key(precision, cls)] = dict(
+            best=best, table=table, particles=int(sum(s.n for s in systems)), boxes=len(systems),
+            steps=int(steps))
+        save_cache(data, path)
+    return best, table
+
+
+def rate_spread(table):
+    """max / min rate over a tuning table (how much the choice matters)."""
+    r = [e["rate"] for e in table]
+    return max(r) / min(r) if r and min(r) > 0 else math.nan

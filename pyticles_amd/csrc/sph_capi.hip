@@ -89,6 +89,27 @@ namespace {
 
 enum Timer { T_SORT = 0, T_DENS, T_GRAD, T_RATES, T_PAIRS, T_INT, T_N };
 
+// Neighbour-list entries per slot in an arena: room for the densest state a
+// served box reaches (liquid at rho ~ 1.8 within r_cut + skin = 4.5 holds
+// ~115 neighbours).
+static constexpr int ARENA_LIST_ENTRIES = 192;
+
+// Device bytes Arena::alloc takes for cap slots (kept in step with alloc).
+static size_t arena_bytes(size_t cap, int prec, int lcap)
+{
+    const size_t d2 = sizeof(double2), d1 = sizeof(double), i1 = sizeof(int);
+    size_t b = 0;
+    b += cap * (4 * d2 + 4 * d1 + i1);          // r v vp acc, u m up udot, id
+    b += cap * (4 * d2 + 4 * d1 + i1);          // sorted scratch of the same
+    b += cap * (2 * i1 + d2 + sizeof(int2));    // pcell slot_box, r_ref, cnt
+    b += cap * (size_t)lcap * sizeof(unsigned); // nbr
+    b += cap * (3 * d2 + 2 * d1 + 2 * sizeof(double4));  // rho Q jq, T e, S gradv
+    if (prec == SPH_FP32C) b += cap * (2 * sizeof(float4) + 3 * sizeof(float2) + sizeof(float));
+    b += cap * 5 * sizeof(unsigned);             // sort keys / values
+    b += radix_sort_temp_bytes((int)cap);
+    return b;
+}
+
 struct Arena {
     sph_ctx *ctx = nullptr;
     int precision = SPH_FP64;
@@ -155,6 +176,7 @@ struct Arena {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int use_graph = 1;
     int always_build = 0;  // arenas: build stage every step (per-box schedule on device)
+    int unroll = 0;        // list entries in flight per lane in the pair passes (0 = default)
 
     bool fp32() const { return precision == SPH_FP32C; }
 
@@ -398,13 +420,13 @@ struct Arena {
             mark(T_SORT, false);
         }
         mark(T_DENS, true);
-        launch_density(A, f, MASK_STEP, st);
+        launch_density(A, f, MASK_STEP, unroll, st);
         mark(T_DENS, false);
         mark(T_GRAD, true);
-        launch_grad(A, f, MASK_RUN, 0, st);
+        launch_grad(A, f, MASK_RUN, 0, unroll, st);
         mark(T_GRAD, false);
         mark(T_RATES, true);
-        launch_rates(A, f, MASK_RUN, st);
+        launch_rates(A, f, MASK_RUN, unroll, st);
         mark(T_RATES, false);
         mark(T_INT, true);
         launch_integrate(A, f, st);
@@ -930,16 +952,16 @@ static int eval_box(sph_box *box, int grad, int rates)
     A.enqueue_build(MASK_ALL);
     A.mark(T_SORT, false);
     A.mark(T_DENS, true);
-    launch_density(V, f, MASK_ALL, st);
+    launch_density(V, f, MASK_ALL, A.unroll, st);
     A.mark(T_DENS, false);
     if (grad) {
         A.mark(T_GRAD, true);
-        launch_grad(V, f, MASK_ALL, 1, st);
+        launch_grad(V, f, MASK_ALL, 1, A.unroll, st);
         A.mark(T_GRAD, false);
     }
     if (rates) {
         A.mark(T_RATES, true);
-        launch_rates(V, f, MASK_ALL, st);
+        launch_rates(V, f, MASK_ALL, A.unroll, st);
         A.mark(T_RATES, false);
     }
     HIPCHK(hipGetLastError());
@@ -1060,7 +1082,7 @@ int sph_arena_create(sph_ctx *ctx, int64_t particle_capacity, int precision, sph
     a->ar.ctx = ctx;
     // boxes of number density up to 2.5 (the vdW pole is at 1/b = 2) with
     // the default list radius 4.5
-    int rc = a->ar.alloc((int)particle_capacity, precision, 192);
+    int rc = a->ar.alloc((int)particle_capacity, precision, ARENA_LIST_ENTRIES);
     a->ar.always_build = 1;
     if (!rc) rc = a->ar.relayout();
     if (rc) {
@@ -1137,12 +1159,13 @@ int sph_arena_poll_all(sph_arena *ar, int nmax, int *state, int64_t *k, int *hea
     Arena &A = ar->ar;
     if (int rc = A.read_dyn()) return rc;
     const int nb = std::min(nmax, (int)A.boxes.size());
-    for (int b = 0; b < nb; ++b) {
-        if (state) state[b] = A.box_live[b] ? A.h_dyn[b].state : BOX_FREE;
-        if (k) k[b] = A.h_dyn[b].k;
-        if (health) health[b] = A.h_dyn[b].health;
+    for (int b = 0; b < nmax; ++b) {
+        const bool live = b < nb && A.box_live[b];
+        if (state) state[b] = live ? A.h_dyn[b].state : BOX_FREE;
+        if (k) k[b] = live ? A.h_dyn[b].k : 0;
+        if (health) health[b] = live ? A.h_dyn[b].health : 0;
     }
-    return (int)A.boxes.size() > nmax ? fail(SPH_ERR_CAP, "more boxes than nmax") : SPH_OK;
+    return SPH_OK;
 }
 
 int sph_arena_result(sph_arena *ar, int box_id, double *r, double *v, double *u, double *rho, double *e0,
@@ -1208,6 +1231,35 @@ int sph_box_set_graph(sph_box *box, int on)
     if (!box) return fail(SPH_ERR_ARG, "null box");
     box->ar.use_graph = on ? 1 : 0;
     box->ar.invalidate_graph();
+    return SPH_OK;
+}
+
+static int set_schedule(Arena &A, int unroll)
+{
+    if (unroll != 0 && unroll != 1 && unroll != 2 && unroll != 4)
+        return fail(SPH_ERR_ARG, "unroll must be 0 (default), 1, 2 or 4");
+    A.unroll = unroll;
+    A.invalidate_graph();
+    return SPH_OK;
+}
+
+int sph_arena_set_schedule(sph_arena *ar, int unroll)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    return set_schedule(ar->ar, unroll);
+}
+
+int sph_box_set_schedule(sph_box *box, int unroll)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    return set_schedule(box->ar, unroll);
+}
+
+int sph_arena_footprint(int64_t particle_capacity, int precision, int64_t *bytes)
+{
+    if (!bytes || particle_capacity < 0 || (precision != SPH_FP64 && precision != SPH_FP32C))
+        return fail(SPH_ERR_ARG, "invalid argument");
+    *bytes = (int64_t)arena_bytes((size_t)particle_capacity, precision, ARENA_LIST_ENTRIES);
     return SPH_OK;
 }
 

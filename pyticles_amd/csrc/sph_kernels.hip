@@ -24,16 +24,10 @@
 namespace sph {
 
 static constexpr int NT = 256;  // work-items per block (4 waves)
-// list entries fetched per round in the passes (measured best on MI355X:
-// 2 for FP64, whose wider registers cut occupancy, 4 for FP32)
-#ifndef SPH_UNROLL64
-#define SPH_UNROLL64 2
-#endif
-#ifndef SPH_UNROLL32
-#define SPH_UNROLL32 4
-#endif
-static constexpr int U64 = SPH_UNROLL64;
-static constexpr int U32 = SPH_UNROLL32;
+// The pair passes are templates on the number of list entries whose loads
+// are in flight per lane (U64 / U32 = 1, 2 or 4); the schedule picks one at
+// launch time.  Defaults: 2 for FP64, whose wider registers cut occupancy,
+// 4 for FP32.
 
 // gfx950 dispatches block i to XCD (i mod 8).  Remap so that each XCD gets a
 // contiguous run of blocks: neighbouring slots (neighbouring cells) then share
@@ -266,6 +260,7 @@ __device__ __forceinline__ double vdw_p(double rho, double T, const BoxDesc &B)
 
 // P1: rho_i = W(0,h) + sum_j m_j W(r_ij, h)  (self term not times m_i,
 // properties.py:77-78), rho_lr likewise with h_l; then T, p, pco.
+template <int U64>
 __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
 {
     const int s = xcd_block() * NT + threadIdx.x;
@@ -359,6 +354,7 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
 // P2: grad v and grad T with grad_i W_ij = -dwdx_ij (dwdx from r_j - r_i);
 // viscous stress Pi = -eta (G + G^T - tr G I) - zeta tr G I (2-D) and heat
 // flux q = -k grad T.  diag != 0 also stores grad v and q for download.
+template <int U64>
 __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int diag)
 {
     const int s = xcd_block() * NT + threadIdx.x;
@@ -437,6 +433,7 @@ __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int 
 // P3: rates.
 //   f   = (S_i + S_j) . dwdx_h + (Sco_i + Sco_j) dwdx_hl
 //   a_i = sum_j m_j f;   udot_i = sum_j m_j (dv . f / 2 + (Q_i + Q_j) . dwdx_h)
+template <int U64>
 __global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
 {
     const int s = xcd_block() * NT + threadIdx.x;
@@ -597,6 +594,7 @@ __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const 
     }
 }
 
+template <int U32>
 __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
 {
     const int s = xcd_block() * NT + threadIdx.x;
@@ -681,6 +679,7 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     if (!isfinite(rd) || !isfinite(rl) || !isfinite(ui)) flag_health(A.dyn, b, HEALTH_NONFINITE);
 }
 
+template <int U32>
 __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int diag)
 {
     const int s = xcd_block() * NT + threadIdx.x;
@@ -760,6 +759,7 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
     }
 }
 
+template <int U32>
 __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
 {
     const int s = xcd_block() * NT + threadIdx.x;
@@ -1254,20 +1254,32 @@ void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_
 {
     hipLaunchKernelGGL(k_build_lists, grid_for(A.n), dim3(NT), 0, st, A, force_mask, fp32);
 }
-void launch_density(const ArenaView &A, int fp32, int state_mask, hipStream_t st)
+// unroll: entries in flight per lane (1, 2 or 4; anything else = default)
+#define SPH_DISPATCH(K64, K32, ...)                                                              \
+    do {                                                                                         \
+        const dim3 g = grid_for(A.n), b(NT);                                                     \
+        if (fp32) {                                                                              \
+            if (unroll == 1) hipLaunchKernelGGL(K32<1>, g, b, 0, st, __VA_ARGS__);               \
+            else if (unroll == 2) hipLaunchKernelGGL(K32<2>, g, b, 0, st, __VA_ARGS__);          \
+            else hipLaunchKernelGGL(K32<4>, g, b, 0, st, __VA_ARGS__);                           \
+        } else {                                                                                 \
+            if (unroll == 1) hipLaunchKernelGGL(K64<1>, g, b, 0, st, __VA_ARGS__);               \
+            else if (unroll == 4) hipLaunchKernelGGL(K64<4>, g, b, 0, st, __VA_ARGS__);          \
+            else hipLaunchKernelGGL(K64<2>, g, b, 0, st, __VA_ARGS__);                           \
+        }                                                                                        \
+    } while (0)
+
+void launch_density(const ArenaView &A, int fp32, int state_mask, int unroll, hipStream_t st)
 {
-    if (fp32) hipLaunchKernelGGL(k_density32, grid_for(A.n), dim3(NT), 0, st, A, state_mask);
-    else hipLaunchKernelGGL(k_density64, grid_for(A.n), dim3(NT), 0, st, A, state_mask);
+    SPH_DISPATCH(k_density64, k_density32, A, state_mask);
 }
-void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, hipStream_t st)
+void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, int unroll, hipStream_t st)
 {
-    if (fp32) hipLaunchKernelGGL(k_grad32, grid_for(A.n), dim3(NT), 0, st, A, state_mask, diag);
-    else hipLaunchKernelGGL(k_grad64, grid_for(A.n), dim3(NT), 0, st, A, state_mask, diag);
+    SPH_DISPATCH(k_grad64, k_grad32, A, state_mask, diag);
 }
-void launch_rates(const ArenaView &A, int fp32, int state_mask, hipStream_t st)
+void launch_rates(const ArenaView &A, int fp32, int state_mask, int unroll, hipStream_t st)
 {
-    if (fp32) hipLaunchKernelGGL(k_rates32, grid_for(A.n), dim3(NT), 0, st, A, state_mask);
-    else hipLaunchKernelGGL(k_rates64, grid_for(A.n), dim3(NT), 0, st, A, state_mask);
+    SPH_DISPATCH(k_rates64, k_rates32, A, state_mask);
 }
 void launch_integrate(const ArenaView &A, int fp32, hipStream_t st)
 {

@@ -15,9 +15,9 @@ void launch_keys(const ArenaView &A, int force_mask, hipStream_t st);
 void launch_permute(const ArenaView &A, hipStream_t st);
 void launch_cell_start(const ArenaView &A, hipStream_t st);
 void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_t st);
-void launch_density(const ArenaView &A, int fp32, int state_mask, hipStream_t st);
-void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, hipStream_t st);
-void launch_rates(const ArenaView &A, int fp32, int state_mask, hipStream_t st);
+void launch_density(const ArenaView &A, int fp32, int state_mask, int unroll, hipStream_t st);
+void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, int unroll, hipStream_t st);
+void launch_rates(const ArenaView &A, int fp32, int state_mask, int unroll, hipStream_t st);
 void launch_integrate(const ArenaView &A, int fp32, hipStream_t st);
 void launch_box_energy(const ArenaView &A, int nbox, hipStream_t st);
 void launch_advance(const ArenaView &A, int nbox, hipStream_t st);

@@ -106,6 +106,10 @@ class DeviceBox:
     def set_graph(self, on):
         _lib.check(_lib.lib.sph_box_set_graph(self._h, 1 if on else 0))
 
+    def set_schedule(self, unroll):
+        """List entries in flight per lane in the pair passes (1, 2, 4; 0 = default)."""
+        _lib.check(_lib.lib.sph_box_set_schedule(self._h, int(unroll)))
+
     def timing(self, reset=False):
         out = np.zeros(7)
         _lib.check(_lib.lib.sph_box_timing(self._h, _lib.dptr(out), 7, 1 if reset else 0))

@@ -1,0 +1,94 @@
+"""Serving loop, launch schedules and the autotuner on the device.
+
+A served stream admits jobs while others run (they share every launch);
+each job's result must be bitwise the one it gets alone.  Launch schedules
+change how neighbours are walked, never which pairs interact."""
+import numpy as np
+import pytest
+
+from pyticles_amd import _lib, autotune, configs
+from pyticles_amd.device import from_system
+from pyticles_amd.serve import Server, latency_summary
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(sysm, precision, unroll=0, skin=None):
+    prm = dict(sysm.prm, precision=precision)
+    if skin is not None:
+        prm["skin"] = skin
+    s = configs.System(sysm.r, sysm.v, sysm.m, sysm.T, prm, sysm.steps, sysm.seed)
+    with from_system(s) as b:
+        b.set_schedule(unroll)
+        b.step(sysm.steps)
+        return b.download("r", "v", "u", "rho")
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def test_served_stream_overlaps_and_equals_alone():
+    prec = _lib.FP32C
+    jobs = [configs.lognormal_box(300 + j, 256, 1500, 0.3, 1.2, slo=8, shi=40) for j in range(10)]
+    arr = np.linspace(0.0, 0.03, len(jobs))
+    sched = dict(unroll=2, skin=0.75)
+    srv = Server(2e9, precision=prec, max_resident=4000, max_job=1500, chunk=3, keep_results=True,
+                 schedule=sched)
+    try:
+        recs = srv.run(jobs, arr)
+    finally:
+        srv.close()
+    assert srv.max_running >= 2, "jobs never overlapped on the device"
+    assert all(r.finished is not None and r.health == 0 for r in recs)
+    for rec in recs:
+        job = jobs[rec.job_id]
+        ref = alone(job, prec, unroll=sched["unroll"], skin=sched["skin"])
+        for k in ("r", "v", "u", "rho"):
+            assert np.array_equal(rec.result[k], ref[k]), (rec.job_id, k)
+    s = latency_summary(recs)
+    assert s["jobs"] == len(jobs) and s["p99_s"] >= s["p50_s"] > 0
+
+
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32C])
+def test_unroll_schedules_agree(precision):
+    # Same pairs in the same order; the compiler may fuse multiply-adds
+    # differently per unroll, so the FP32 form agrees to FP32 rounding of the
+    # rates propagated over 25 steps (measured ~1e-9 relative), the FP64
+    # form to FP64 rounding.
+    tol = 1e-13 if precision == _lib.FP64 else 1e-7
+    s = configs.square_system(24, 0.9, 7, 25)
+    base = alone(s, precision, unroll=0)
+    for u in (1, 2, 4):
+        got = alone(s, precision, unroll=u)
+        for k in ("r", "v", "u", "rho"):
+            assert rel(got[k], base[k]) <= tol, (u, k)
+
+
+def test_skin_changes_lists_not_physics():
+    s = configs.square_system(24, 0.9, 8, 30)
+    a = alone(s, _lib.FP64, skin=0.3)
+    b = alone(s, _lib.FP64, skin=0.75)
+    for k in ("r", "v", "u", "rho"):
+        assert rel(a[k], b[k]) <= 1e-11, k
+
+
+def test_bad_schedule_rejected():
+    s = configs.square_system(8, 0.9, 9, 1)
+    with from_system(s) as b:
+        with pytest.raises(_lib.SphError) as ei:
+            b.set_schedule(3)
+        assert ei.value.code == _lib.SPH_ERR_ARG
+
+
+def test_autotune_small_workload(tmp_path):
+    p = str(tmp_path / "sched.json")
+    s = configs.square_system(32, 0.9, 10, 10)
+    best, table = autotune.tune(s, _lib.FP64, "box", steps=3, warm=1, path=p)
+    assert autotune.valid(best) and len(table) == len(autotune.candidates())
+    assert all(e["rate"] > 0 for e in table)
+    assert autotune.schedule_for(_lib.FP64, "box", path=p)["source"] == "tuned"
+    boxes = [configs.lognormal_box(400 + j, 256, 600, 0.4, 0.9, steps=20) for j in range(4)]
+    best, table = autotune.tune(boxes, _lib.FP32C, "batched", steps=3, warm=1, path=p)
+    assert autotune.valid(best)
+    assert set(autotune.load_cache(p)) == {"fp64:box", "fp32c:batched"}
