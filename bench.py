@@ -112,9 +112,9 @@ def cpu_baseline(s, target_s=12.0):
 
 
 def error_vs_oracle(s, r=None, v=None, u=None, unroll=0):
-    """Normwise error max|x - x_ref| / max|x_ref| of one rate evaluation of
-    the device path against the FP64 oracle, at the initial state of s or at
-    the state (r, v, u)."""
+    """Normwise errors of one rate evaluation of the device path against the
+    FP64 oracle, at the initial state of s or at the state (r, v, u):
+    max|x - x_ref| / max|x_ref| per field, and the RMS ratio as <field>_rms."""
     from oracle import oracle as O
     from pyticles_amd.device import from_system
     r = s.r if r is None else r
@@ -126,8 +126,12 @@ def error_vs_oracle(s, r=None, v=None, u=None, unroll=0):
         b.upload(r=r, v=v, u=u)
         b.rates(s.prm["terms"])
         got = b.download("rho", "rho_lr", "acc", "udot")
-    return {k: float(np.max(np.abs(got[k] - ref[k])) / np.max(np.abs(ref[k])))
-            for k in ("rho", "rho_lr", "acc", "udot")}
+    out = {}
+    for k in ("rho", "rho_lr", "acc", "udot"):
+        d = got[k] - ref[k]
+        out[k] = float(np.max(np.abs(d)) / np.max(np.abs(ref[k])))
+        out[k + "_rms"] = float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(ref[k] * ref[k])))
+    return out
 
 
 def log(msg):

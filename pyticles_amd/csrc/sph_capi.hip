@@ -544,6 +544,8 @@ int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
     if (ncy < 3) ncy = 1;
     if ((long long)ncx * ncy > (1ll << 26))
         return fail(SPH_ERR_ARG, "box has too many cells for the cell list (> 2^26)");
+    if (p->precision == SPH_FP32C && (ncx > 65535 || ncy > 65535))
+        return fail(SPH_ERR_ARG, "FP32C boxes hold at most 65535 cells per axis");
     B.ncx = ncx; B.ncy = ncy; B.sx = ncx >= 3 ? 1 : 0; B.sy = ncy >= 3 ? 1 : 0;
     B.cwx = p->Lx / ncx; B.cwy = p->Ly / ncy; B.icwx = ncx / p->Lx; B.icwy = ncy / p->Ly;
     B.h = p->h; B.hl = p->hl; B.ih = 1.0 / p->h; B.ihl = 1.0 / p->hl;

@@ -104,15 +104,27 @@ __device__ __forceinline__ void max_bits(unsigned long long *addr, double x)
     if (b > *(volatile unsigned long long *)addr) atomicMax(addr, b);
 }
 
-// Slot position relative to the corner of its build cell, folded into the
-// box with the minimum image (drifts may carry it across the periodic edge).
-__device__ __forceinline__ float4 rel_pos(double2 r, int cell, double m, const BoxDesc &B)
+// FP32 mirror of a position: fixed point relative to the corner of the
+// particle's build cell, 2^29 units per cell width, plus its mass and the
+// cell's (x, y) index.  Separations taken from two such records are exact
+// integers plus whole cells, so the FP32 separation is the exact one
+// rounded once (FP32 coordinates would round each position first and leave
+// several ulps of the coordinate in every separation).  Offsets are clamped
+// at 1.5 cell widths, far beyond the half skin a particle may move between
+// builds before the skin check trips.
+static constexpr double FIX_ONE = 536870912.0;  // 2^29
+static constexpr double FIX_HALF = 268435456.0;
+
+__device__ __forceinline__ float4 fix_pos(double2 r, int cell, double m, const BoxDesc &B)
 {
     const int cx = cell % B.ncx, cy = cell / B.ncx;
     double xl = r.x - cx * B.cwx, yl = r.y - cy * B.cwy;
     xl = xl > B.hLx ? xl - B.Lx : (xl < -B.hLx ? xl + B.Lx : xl);
     yl = yl > B.hLy ? yl - B.Ly : (yl < -B.hLy ? yl + B.Ly : yl);
-    return make_float4((float)xl, (float)yl, (float)m, __int_as_float(cx | (cy << 16)));
+    const double fx = fmin(fmax(xl * B.icwx, -1.5), 1.5) * FIX_ONE;
+    const double fy = fmin(fmax(yl * B.icwy, -1.5), 1.5) * FIX_ONE;
+    return make_float4(__int_as_float((int)rint(fx)), __int_as_float((int)rint(fy)), (float)m,
+                       __int_as_float(cx | (cy << 16)));
 }
 
 // ===========================================================================
@@ -217,7 +229,7 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
     cell = (cell < 0 || cell >= B.ncx * B.ncy) ? 0 : cell;
     A.pcell[s] = cell;
     if (fp32) {
-        A.rf[s] = rel_pos(ri, cell, mi, B);
+        A.rf[s] = fix_pos(ri, cell, mi, B);
         A.vf[s] = make_float2((float)vpi.x, (float)vpi.y);
     }
     const int cap = A.cap, lcap = A.lcap;
@@ -554,7 +566,7 @@ struct Kahan {
 };
 
 struct BoxF {
-    float cwx, cwy, Lx, Ly, hLx, hLy;
+    double sclx, scly;  // length of one fixed-point unit
     float cut2, cutl, h2, cuts;
     float ih, ihl, qh, qhl, fh, fhl;
     int ncx, ncy, sx, sy;
@@ -563,8 +575,7 @@ struct BoxF {
 __device__ __forceinline__ BoxF boxf(const BoxDesc &B)
 {
     BoxF f;
-    f.cwx = (float)B.cwx; f.cwy = (float)B.cwy;
-    f.Lx = (float)B.Lx; f.Ly = (float)B.Ly; f.hLx = (float)B.hLx; f.hLy = (float)B.hLy;
+    f.sclx = B.cwx / FIX_ONE; f.scly = B.cwy / FIX_ONE;
     f.cut2 = (float)B.cut2; f.cutl = (float)(B.cut2 < B.hl2 ? B.cut2 : B.hl2); f.h2 = (float)B.h2;
     f.cuts = (float)(B.R2list < B.h2 ? B.R2list : B.h2);
     f.ih = (float)B.ih; f.ihl = (float)B.ihl; f.qh = (float)B.qh; f.qhl = (float)B.qhl;
@@ -573,25 +584,28 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B)
     return f;
 }
 
-// Separation of rj from ri, both relative to their build-cell corners: the
-// cell offset (with the periodic wrap of the cell index) carries the image;
-// boxes with a single cell along an axis use the minimum image instead.
+// Separation of rj from ri (fixed-point records): the integer difference
+// plus the cell offset (with the periodic wrap of the cell index) is exact in
+// FP64, and so is the minimum image of an axis with a single cell (period
+// 2^29 units); one rounding to FP32 at the end.
 __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const BoxF &F, float &dx, float &dy)
 {
     const int ci = __float_as_int(ri.w), cj = __float_as_int(rj.w);
     int ox = (cj & 0xffff) - (ci & 0xffff), oy = (cj >> 16) - (ci >> 16);
     ox = ox > 1 ? ox - F.ncx : (ox < -1 ? ox + F.ncx : ox);
     oy = oy > 1 ? oy - F.ncy : (oy < -1 ? oy + F.ncy : oy);
-    dx = (rj.x - ri.x) + (float)ox * F.cwx;
-    dy = (rj.y - ri.y) + (float)oy * F.cwy;
+    double ddx = (double)(__float_as_int(rj.x) - __float_as_int(ri.x)) + (double)ox * FIX_ONE;
+    double ddy = (double)(__float_as_int(rj.y) - __float_as_int(ri.y)) + (double)oy * FIX_ONE;
     if (F.sx == 0) {
-        dx = dx > F.hLx ? dx - F.Lx : dx;
-        dx = dx < -F.hLx ? dx + F.Lx : dx;
+        ddx = ddx > FIX_HALF ? ddx - FIX_ONE : ddx;
+        ddx = ddx < -FIX_HALF ? ddx + FIX_ONE : ddx;
     }
     if (F.sy == 0) {
-        dy = dy > F.hLy ? dy - F.Ly : dy;
-        dy = dy < -F.hLy ? dy + F.Ly : dy;
+        ddy = ddy > FIX_HALF ? ddy - FIX_ONE : ddy;
+        ddy = ddy < -FIX_HALF ? ddy + FIX_ONE : ddy;
     }
+    dx = (float)(ddx * F.sclx);
+    dy = (float)(ddy * F.scly);
 }
 
 template <int U32>
@@ -909,7 +923,7 @@ __global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
     min_image(ddx, ddy, B);
     if (ddx * ddx + ddy * ddy > B.hskin2) flag_health(A.dyn, b, HEALTH_SKIN);
     if (fp32) {
-        A.rf[s] = rel_pos(rn, A.pcell[s], m, B);
+        A.rf[s] = fix_pos(rn, A.pcell[s], m, B);
         A.vf[s] = make_float2((float)vp.x, (float)vp.y);
     }
 }
