@@ -319,13 +319,14 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
             }
         }
     }
-    const int kl = A.lcap - c.y;
-    for (int k0 = kl; k0 < A.lcap; k0 += U64) {
+    // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
+    // read the same row at step k (coalesced), whatever their counts
+    for (int k0 = 0; k0 < c.y; k0 += U64) {
         int t[U64];
         double2 rj[U64];
         double mj[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < A.lcap ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
             min_image(dx, dy, B);
             const double r2 = rsq_exact(dx, dy);
-            if (k0 + u < A.lcap && r2 < cutl) {
+            if (k0 + u < c.y && r2 < cutl) {
                 const double rr = sqrt(r2);
                 const double q = rr * B.ihl, w = 1.0 - q;
                 rhol += mj[u] * (B.qhl * (1.0 + 3.0 * q) * (w * w * w));
@@ -509,13 +510,14 @@ __global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
             }
         }
     }
-    const int kl = A.lcap - c.y;
-    for (int k0 = kl; k0 < A.lcap; k0 += U64) {
+    // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
+    // read the same row at step k (coalesced), whatever their counts
+    for (int k0 = 0; k0 < c.y; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64];
         double Sw[U64], mj[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < A.lcap ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -528,7 +530,7 @@ __global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
             min_image(dx, dy, B);
             const double r2 = rsq_exact(dx, dy);
-            if (k0 + u < A.lcap && r2 < cutl) {
+            if (k0 + u < c.y && r2 < cutl) {
                 const double rr = sqrt(r2);
                 const double q = rr * B.ihl, w = 1.0 - q;
                 const double Fl = B.fhl * w * w;
@@ -649,11 +651,13 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
             }
         }
     }
-    for (int k0 = A.lcap - c.y; k0 < A.lcap; k0 += U32) {
+    // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
+    // read the same row at step k (coalesced), whatever their counts
+    for (int k0 = 0; k0 < c.y; k0 += U32) {
         int t[U32];
         float4 rj[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < A.lcap ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
 #pragma unroll
         for (int u = 0; u < U32; ++u) rj[u] = A.rf[t[u]];
 #pragma unroll
@@ -661,7 +665,7 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
             float dx, dy;
             sep32(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
-            if (k0 + u < A.lcap && r2 < F.cutl) {
+            if (k0 + u < c.y && r2 < F.cutl) {
                 const float rr = sqrtf(r2);
                 const float q = rr * F.ihl, w = 1.f - q;
                 rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
@@ -837,13 +841,15 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
             }
         }
     }
-    for (int k0 = A.lcap - c.y; k0 < A.lcap; k0 += U32) {
+    // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
+    // read the same row at step k (coalesced), whatever their counts
+    for (int k0 = 0; k0 < c.y; k0 += U32) {
         int t[U32];
         float4 rj[U32];
         float2 vv[U32];
         float Sw[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < A.lcap ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -855,7 +861,7 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
             float dx, dy;
             sep32(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
-            if (k0 + u < A.lcap && r2 < F.cutl) {
+            if (k0 + u < c.y && r2 < F.cutl) {
                 const float rr = sqrtf(r2);
                 const float q = rr * F.ihl, w = 1.f - q;
                 const float Fl = F.fhl * w * w;
