@@ -127,6 +127,9 @@ class VerletList(NeighbourList):
         self.rebuild_list = False
         p = self.particle
         self.r_old[:, :] = p.r[:, :]
+        if p.n == 0:
+            self.nip = 0
+            return
         dev = self._device()
         iap = dev.pairs()
         self._store_pairs(iap)

@@ -175,6 +175,9 @@ class SmoothParticleSystem(ParticleSystem):
         n = self.n
         t1 = time.time()
         self.dt = dt
+        if n == 0:
+            self.steps += int(nsteps)
+            return
         prm = self._step_params(dt)
         if not getattr(self.box, "periodic", False):
             raise NotImplementedError("GPU stepping needs a periodic box")

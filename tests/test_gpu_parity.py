@@ -232,7 +232,67 @@ def test_fp32_compensated_error_vs_fp64():
         got = b.download("rho", "rho_lr", "acc", "udot")
     assert rel_norm(got["rho"], ref["rho"]) < 1e-6
     assert rel_norm(got["rho_lr"], ref["rho_lr"]) < 1e-6
-    assert rel_norm(got["acc"], ref["acc"]) < 1e-5
+    assert rel_norm(got["acc"], ref["acc"]) < 1e-6
+    assert rel_norm(got["udot"], ref["udot"]) < 1e-6
     print("fp32c errors rho %.2e rho_lr %.2e acc %.2e udot %.2e" % (
         rel_norm(got["rho"], ref["rho"]), rel_norm(got["rho_lr"], ref["rho_lr"]),
         rel_norm(got["acc"], ref["acc"]), rel_norm(got["udot"], ref["udot"])))
+
+
+# ------------------------------------------------------------------ box shapes and edge sizes
+def _rect_system(nx, ny, Lx, Ly, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    r = np.stack([(i.ravel() + 0.5) * Lx / nx, (j.ravel() + 0.5) * Ly / ny], axis=1)
+    r += rng.uniform(-0.1, 0.1, size=r.shape)
+    r[:, 0] = np.mod(r[:, 0], Lx)
+    r[:, 1] = np.mod(r[:, 1], Ly)
+    n = nx * ny
+    v = configs.velocities(n, 0.9, rng)
+    m = rng.uniform(0.9, 1.1, size=n)
+    u = rng.uniform(-1.5, -0.8, size=n)
+    return np.ascontiguousarray(r), v, m, u
+
+
+@pytest.mark.parametrize("nx,ny,Lx,Ly", [(40, 13, 40.0, 13.0), (64, 8, 64.0, 8.0), (6, 50, 6.0, 50.0),
+                                         (30, 30, 30.0, 30.5)])
+def test_rectangular_boxes(nx, ny, Lx, Ly):
+    # elongated boxes, and boxes with a single cell along one axis (minimum
+    # image instead of cell offsets, in the FP64 build and the FP32 mirrors)
+    r, v, m, u = _rect_system(nx, ny, Lx, Ly, seed=nx * ny)
+    prm = configs.params(Lx, Ly=Ly)
+    ref_pairs = O.pairs(r, Lx, Ly, 17.0)
+    ref = O.derivatives(r, v, m, u, prm, terms=15)
+    with _box(r, v, m, prm, u=u) as b:
+        assert np.array_equal(b.pairs(), ref_pairs)
+        b.rates(15)
+        got = b.download("rho", "acc", "udot")
+    assert rel_norm(got["rho"], ref["rho"]) < 1e-13
+    assert rel_norm(got["acc"], ref["acc"]) < TOL64 and rel_norm(got["udot"], ref["udot"]) < TOL64
+    with _box(r, v, m, dict(prm, precision=_lib.FP32C), u=u) as b:
+        b.rates(15)
+        got = b.download("rho", "acc")
+    assert rel_norm(got["rho"], ref["rho"]) < 1e-6 and rel_norm(got["acc"], ref["acc"]) < 2e-6
+
+
+def test_empty_and_single_particle():
+    with pytest.raises(_lib.SphError) as ei:
+        Box(0, configs.params(10.0))
+    assert ei.value.code == _lib.SPH_ERR_ARG
+    from pyticles_amd import neighbour_list, particles, properties
+    p = particles.SmoothParticleSystem(0, d=2, maxn=4, xmax=10, ymax=10)
+    nl = neighbour_list.VerletList(p, cutoff=4.0, tolerance=1.0)
+    nl.build()
+    nl.separations()
+    properties.spam_properties(p, nl)
+    assert nl.nip == 0
+    # one particle: only the self term, no force
+    r = np.array([[3.0, 4.0]])
+    prm = configs.params(10.0)
+    with _box(r, np.array([[0.1, -0.2]]), np.array([2.0]), prm, u=np.array([-1.0])) as b:
+        assert b.pairs().shape == (0, 2)
+        b.rates(15)
+        got = b.download("rho", "rho_lr", "acc", "udot")
+    assert got["rho"][0] == pytest.approx(5.0 / (math.pi * 4.0), rel=1e-15)
+    assert got["rho_lr"][0] == pytest.approx(5.0 / (math.pi * 16.0), rel=1e-15)
+    assert np.all(got["acc"] == 0.0) and got["udot"][0] == 0.0
