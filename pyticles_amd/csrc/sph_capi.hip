@@ -104,7 +104,8 @@ static size_t arena_bytes(size_t cap, int prec, int lcap)
     b += cap * (2 * i1 + d2 + sizeof(int2));    // pcell slot_box, r_ref, cnt
     b += cap * (size_t)lcap * sizeof(unsigned); // nbr
     b += cap * (3 * d2 + 2 * d1 + 2 * sizeof(double4));  // rho Q jq, T e, S gradv
-    if (prec == SPH_FP32C) b += cap * (2 * sizeof(float4) + 3 * sizeof(float2) + sizeof(float));
+    b += cap * (2 * d2 + d1);                             // vt, aco, uco
+    if (prec == SPH_FP32C) b += cap * (3 * sizeof(float4) + 2 * sizeof(float2));  // rf S32 vq32, vf vt32
     b += cap * 5 * sizeof(unsigned);             // sort keys / values
     b += radix_sort_temp_bytes((int)cap);
     return b;
@@ -132,8 +133,10 @@ struct Arena {
     double *T = nullptr, *e = nullptr;
     double4 *S = nullptr, *gradv = nullptr;
     float4 *rf = nullptr, *S32 = nullptr;
-    float2 *vf = nullptr, *rho32 = nullptr, *Q32 = nullptr;
-    float *T32 = nullptr;
+    float2 *vf = nullptr, *vt32 = nullptr;
+    float4 *vq32 = nullptr;
+    double2 *vt = nullptr, *aco = nullptr;
+    double *uco = nullptr;
     unsigned *keys_in = nullptr, *k_out = nullptr, *v_out = nullptr, *k_alt = nullptr, *v_alt = nullptr;
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
@@ -189,7 +192,8 @@ struct Arena {
         A.u_s = u_s; A.m_s = m_s; A.up_s = up_s; A.udot_s = udot_s; A.id_s = id_s;
         A.pcell = pcell; A.r_ref = r_ref; A.slot_box = slot_box; A.nbr = nbr; A.cnt = cnt;
         A.rho = rho; A.T = T; A.S = S; A.Q = Q; A.gradv = gradv; A.jq = jq; A.e = e;
-        A.rf = rf; A.vf = vf; A.rho32 = rho32; A.T32 = T32; A.S32 = S32; A.Q32 = Q32;
+        A.rf = rf; A.vf = vf; A.S32 = S32; A.vq32 = vq32; A.vt32 = vt32;
+        A.vt = vt; A.aco = aco; A.uco = uco;
         A.cell_start = cell_start;
         A.keys = k_out; A.vals = v_out; A.keys_in = keys_in;
         A.boxes = d_boxes; A.dyn = d_dyn; A.segs = d_segs;
@@ -218,9 +222,10 @@ struct Arena {
         rc |= dalloc(&rho, cap); rc |= dalloc(&Q, cap); rc |= dalloc(&jq, cap);
         rc |= dalloc(&T, cap); rc |= dalloc(&e, cap);
         rc |= dalloc(&S, cap); rc |= dalloc(&gradv, cap);
+        rc |= dalloc(&vt, cap); rc |= dalloc(&aco, cap); rc |= dalloc(&uco, cap);
         if (prec == SPH_FP32C) {
             rc |= dalloc(&rf, cap); rc |= dalloc(&S32, cap); rc |= dalloc(&vf, cap);
-            rc |= dalloc(&rho32, cap); rc |= dalloc(&Q32, cap); rc |= dalloc(&T32, cap);
+            rc |= dalloc(&vq32, cap); rc |= dalloc(&vt32, cap);
         }
         rc |= dalloc(&keys_in, cap); rc |= dalloc(&k_out, cap); rc |= dalloc(&v_out, cap);
         rc |= dalloc(&k_alt, cap); rc |= dalloc(&v_alt, cap);
@@ -235,7 +240,8 @@ struct Arena {
             hipMemset(rho, 0, z2) || hipMemset(T, 0, z1) || hipMemset(S, 0, sizeof(double4) * cap) ||
             hipMemset(Q, 0, z2) || hipMemset(acc, 0, z2) || hipMemset(udot, 0, z1) ||
             hipMemset(gradv, 0, sizeof(double4) * cap) || hipMemset(jq, 0, z2) || hipMemset(e, 0, z1) ||
-            hipMemset(r_ref, 0, z2) || hipMemset(cnt, 0, sizeof(int2) * cap))
+            hipMemset(r_ref, 0, z2) || hipMemset(cnt, 0, sizeof(int2) * cap) || hipMemset(vt, 0, z2) ||
+            hipMemset(aco, 0, z2) || hipMemset(uco, 0, z1))
             return fail(SPH_ERR_HIP, "hipMemset failed");
         // the context stream does not synchronise with the null stream
         HIPCHK(hipDeviceSynchronize());
@@ -251,7 +257,7 @@ struct Arena {
         dfree(r_s); dfree(v_s); dfree(vp_s); dfree(acc_s); dfree(u_s); dfree(m_s); dfree(up_s); dfree(udot_s);
         dfree(id_s); dfree(pcell); dfree(slot_box); dfree(r_ref); dfree(nbr); dfree(cnt);
         dfree(rho); dfree(Q); dfree(jq); dfree(T); dfree(e); dfree(S); dfree(gradv);
-        dfree(rf); dfree(S32); dfree(vf); dfree(rho32); dfree(Q32); dfree(T32);
+        dfree(rf); dfree(S32); dfree(vf); dfree(vq32); dfree(vt32); dfree(vt); dfree(aco); dfree(uco);
         dfree(keys_in); dfree(k_out); dfree(v_out); dfree(k_alt); dfree(v_alt);
         if (sort_tmp) (void)hipFree(sort_tmp);
         sort_tmp = nullptr;

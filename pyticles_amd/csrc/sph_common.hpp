@@ -13,9 +13,11 @@
 //                    short entries (r < h + skin at build) at k = 0.., long
 //                    entries (h + skin <= r < r_cut + skin) at k = lcap-1 down
 //   derived          rho (double2: rho, rho_lr), T, S (double4: Sxx, Sxy,
-//                    Syy, Sco = P/rho^2 and pco/rho_lr^2), Q (double2: q/rho^2)
-//   FP32 mirrors     rf (float4: x, y relative to the corner of the build
-//                    cell, m, cell column|row), vf (float2), rho32, T32, S32, Q32
+//                    Syy = (p I + Pi)/rho^2, and m), Q (double2: q/rho^2),
+//                    vt (double2: m/rho, T), aco / uco (cohesive rates)
+//   FP32 mirrors     rf (int x, y fixed point relative to the build cell
+//                    corner, float m, cell column|row), vf (float2 v),
+//                    S32 (float4), vq32 (float4: v, q/rho^2), vt32 (m/rho, T)
 // A box is a contiguous slot range [part_off, part_off + n) plus a key range
 // [key_base, key_base + ncx*ncy] of cells; free slot ranges own one key each
 // so a stable sort over the whole arena never moves a particle across boxes.
@@ -90,16 +92,18 @@ struct ArenaView {
     double *T;
     double4 *S;
     double2 *Q;
+    double2 *vt;         // (m / rho, T): neighbour record of the gradient pass
+    double2 *aco;        // cohesive acceleration from the density pass
+    double *uco;         // cohesive du/dt from the density pass
     double4 *gradv;
     double2 *jq;
     double *e;
     // FP32 mirrors
     float4 *rf;
     float2 *vf;
-    float2 *rho32;
-    float *T32;
     float4 *S32;
-    float2 *Q32;
+    float4 *vq32;        // (v, q / rho^2): neighbour record of the rate pass
+    float2 *vt32;        // (m / rho, T)
     // cells and sort
     int *cell_start;
     unsigned *keys;      // sorted keys (final buffer of the radix sort)

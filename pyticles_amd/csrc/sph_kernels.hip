@@ -272,6 +272,14 @@ __device__ __forceinline__ double vdw_p(double rho, double T, const BoxDesc &B)
 
 // P1: rho_i = W(0,h) + sum_j m_j W(r_ij, h)  (self term not times m_i,
 // properties.py:77-78), rho_lr likewise with h_l; then T, p, pco.
+//
+// The cohesive part of the rates rides along.  Its pair factor is
+// pco_i/rho_lr,i^2 + pco_j/rho_lr,j^2 with pco = -a rho_lr^2
+// (f_properties.py:119), i.e. the constant -2a, so it needs only the
+// positions, masses and velocities this pass walks anyway:
+//   a_co,i  = -2a sum_j m_j dwdx_hl,ij
+//   du_co,i = -a  sum_j m_j (v_j - v_i) . dwdx_hl,ij      (dv . f / 2)
+// and the rate pass only walks the short entries (r < h + skin).
 template <int U64>
 __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
 {
@@ -282,17 +290,18 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
     const int st = A.dyn[b].state;
     if (!state_in(st, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
-    const double2 ri = A.r[s];
+    const bool coh = (B.terms & 2u) != 0;
+    const double2 ri = A.r[s], vi = A.vp[s];
     const int2 c = A.cnt[s];
     const size_t cap = A.cap;
     const unsigned *nb = A.nbr + s;
-    double rho = B.qh, rhol = B.qhl;
+    double rho = B.qh, rhol = B.qhl, gx = 0.0, gy = 0.0, ev = 0.0;
     const double cutl = B.cut2 < B.hl2 ? B.cut2 : B.hl2;
     // U64 entries per round: all their loads are issued before any use so
     // several neighbour fetches are in flight per wavefront
     for (int k0 = 0; k0 < c.x; k0 += U64) {
         int t[U64];
-        double2 rj[U64];
+        double2 rj[U64], vv[U64];
         double mj[U64];
 #pragma unroll
         for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
@@ -300,6 +309,7 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
             mj[u] = A.m[t[u]];
+            vv[u] = coh ? A.vp[t[u]] : vi;
         }
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
@@ -311,6 +321,11 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
                 if (r2 < B.hl2) {
                     const double q = rr * B.ihl, w = 1.0 - q;
                     rhol += mj[u] * (B.qhl * (1.0 + 3.0 * q) * (w * w * w));
+                    const double Fl = B.fhl * w * w;
+                    const double fx = Fl * dx, fy = Fl * dy;
+                    gx += mj[u] * fx;
+                    gy += mj[u] * fy;
+                    ev += mj[u] * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
                 }
                 if (r2 < B.h2) {
                     const double q = rr * B.ih, w = 1.0 - q;
@@ -323,7 +338,7 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
     // read the same row at step k (coalesced), whatever their counts
     for (int k0 = 0; k0 < c.y; k0 += U64) {
         int t[U64];
-        double2 rj[U64];
+        double2 rj[U64], vv[U64];
         double mj[U64];
 #pragma unroll
         for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
@@ -331,6 +346,7 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
             mj[u] = A.m[t[u]];
+            vv[u] = coh ? A.vp[t[u]] : vi;
         }
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
@@ -341,9 +357,15 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
                 const double rr = sqrt(r2);
                 const double q = rr * B.ihl, w = 1.0 - q;
                 rhol += mj[u] * (B.qhl * (1.0 + 3.0 * q) * (w * w * w));
+                const double Fl = B.fhl * w * w;
+                const double fx = Fl * dx, fy = Fl * dy;
+                gx += mj[u] * fx;
+                gy += mj[u] * fy;
+                ev += mj[u] * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
             }
         }
     }
+    const double mi = A.m[s];
     double ui = A.up[s];
     if (st == BOX_INIT) {  // u = kB T0 - a rho (properties.py:44-46,119)
         ui = B.kB * B.T0 - B.a * rho;
@@ -353,13 +375,14 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
     double T = (ui + B.a * rho) / B.kB;  // properties.py:48-49
     T = T < 0.0 ? 0.0 : T;               // spam_complete_force.py:152
     const double p = vdw_p(rho, T, B);
-    const double pco = -B.a * rhol * rhol;
     A.rho[s] = make_double2(rho, rhol);
     A.T[s] = T;
-    const double ps = (B.terms & 1u) ? (p + 0.0) / (rho * rho) : 0.0;
-    const double cs = (B.terms & 2u) ? pco / (rhol * rhol) : 0.0;
-    A.S[s] = make_double4(ps, 0.0, ps, cs);
+    const double ps = (B.terms & 1u) ? p / (rho * rho) : 0.0;
+    A.S[s] = make_double4(ps, 0.0, ps, mi);  // .w carries m for the rate pass
     A.Q[s] = make_double2(0.0, 0.0);
+    A.vt[s] = make_double2(mi / rho, T);
+    A.aco[s] = coh ? make_double2(-2.0 * B.a * gx, -2.0 * B.a * gy) : make_double2(0.0, 0.0);
+    A.uco[s] = coh ? -B.a * ev : 0.0;
     if (!(rho * B.b < 1.0)) flag_health(A.dyn, b, HEALTH_EOS_POLE);
     if (!isfinite(rho) || !isfinite(rhol) || !isfinite(ui)) flag_health(A.dyn, b, HEALTH_NONFINITE);
 }
@@ -367,6 +390,7 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
 // P2: grad v and grad T with grad_i W_ij = -dwdx_ij (dwdx from r_j - r_i);
 // viscous stress Pi = -eta (G + G^T - tr G I) - zeta tr G I (2-D) and heat
 // flux q = -k grad T.  diag != 0 also stores grad v and q for download.
+// Neighbour record: position, velocity, (m/rho, T) from P1.
 template <int U64>
 __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int diag)
 {
@@ -386,17 +410,14 @@ __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int 
     double g00 = 0, g01 = 0, g10 = 0, g11 = 0, gt0 = 0, gt1 = 0;
     for (int k0 = 0; k0 < ns; k0 += U64) {
         int t[U64];
-        double2 rj[U64], vv[U64];
-        double mj[U64], rhoj[U64], Tj[U64];
+        double2 rj[U64], vv[U64], vt[U64];
 #pragma unroll
         for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
             vv[u] = A.vp[t[u]];
-            mj[u] = A.m[t[u]];
-            rhoj[u] = A.rho[t[u]].x;
-            Tj[u] = A.T[t[u]];
+            vt[u] = A.vt[t[u]];
         }
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
@@ -408,9 +429,9 @@ __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int 
                 const double q = rr * B.ih, w = 1.0 - q;
                 const double F = B.fh * w * w;
                 const double gx = -F * dx, gy = -F * dy;
-                const double vj = mj[u] / rhoj[u];
+                const double vj = vt[u].x;
                 const double dvx = vv[u].x - vi.x, dvy = vv[u].y - vi.y;
-                const double dT = Tj[u] - Ti;
+                const double dT = vt[u].y - Ti;
                 g00 += vj * dvx * gx;
                 g01 += vj * dvx * gy;
                 g10 += vj * dvy * gx;
@@ -443,9 +464,10 @@ __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int 
     }
 }
 
-// P3: rates.
-//   f   = (S_i + S_j) . dwdx_h + (Sco_i + Sco_j) dwdx_hl
-//   a_i = sum_j m_j f;   udot_i = sum_j m_j (dv . f / 2 + (Q_i + Q_j) . dwdx_h)
+// P3: rates over the short entries, plus the cohesive part from P1.
+//   f   = (S_i + S_j) . dwdx_h           (S = (p I + Pi) / rho^2)
+//   a_i = sum_j m_j f + a_co,i;   udot_i = sum_j m_j (dv . f / 2 + (Q_i + Q_j) . dwdx_h) + du_co,i
+// Neighbour record: position, velocity, (Sxx, Sxy, Syy, m), Q.
 template <int U64>
 __global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
 {
@@ -456,27 +478,26 @@ __global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
     if (!state_in(A.dyn[b].state, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
     const bool cond = (B.terms & 8u) != 0;
+    const bool shortr = (B.terms & 13u) != 0;
     const double2 ri = A.r[s], vi = A.vp[s];
     const double4 Si = A.S[s];
     const double2 Qi = cond ? A.Q[s] : make_double2(0.0, 0.0);
-    const int2 c = A.cnt[s];
+    const int ns = shortr ? A.cnt[s].x : 0;
     const size_t cap = A.cap;
     const unsigned *nb = A.nbr + s;
-    const double cutl = B.cut2 < B.hl2 ? B.cut2 : B.hl2;
+    const double cut = B.cut2 < B.h2 ? B.cut2 : B.h2;
     double ax = 0, ay = 0, ud = 0;
-    for (int k0 = 0; k0 < c.x; k0 += U64) {
+    for (int k0 = 0; k0 < ns; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64], Qj[U64];
         double4 Sj[U64];
-        double mj[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
             vv[u] = A.vp[t[u]];
             Sj[u] = A.S[t[u]];
-            mj[u] = A.m[t[u]];
             Qj[u] = cond ? A.Q[t[u]] : make_double2(0.0, 0.0);
         }
 #pragma unroll
@@ -484,65 +505,27 @@ __global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
             min_image(dx, dy, B);
             const double r2 = rsq_exact(dx, dy);
-            if (k0 + u < c.x && r2 < B.cut2) {
+            if (k0 + u < ns && r2 < cut) {
                 const double rr = sqrt(r2);
-                double fx = 0.0, fy = 0.0, cc = 0.0;
-                if (r2 < B.h2) {
-                    const double q = rr * B.ih, w = 1.0 - q;
-                    const double F = B.fh * w * w;
-                    const double wx = F * dx, wy = F * dy;
-                    const double axx = Si.x + Sj[u].x, axy = Si.y + Sj[u].y, ayy = Si.z + Sj[u].z;
-                    fx = axx * wx + axy * wy;
-                    fy = axy * wx + ayy * wy;
-                    if (cond) cc = (Qi.x + Qj[u].x) * wx + (Qi.y + Qj[u].y) * wy;
-                }
-                if (r2 < B.hl2) {
-                    const double q = rr * B.ihl, w = 1.0 - q;
-                    const double Fl = B.fhl * w * w;
-                    const double cs = Si.w + Sj[u].w;
-                    fx = fx + cs * (Fl * dx);
-                    fy = fy + cs * (Fl * dy);
-                }
+                const double q = rr * B.ih, w = 1.0 - q;
+                const double F = B.fh * w * w;
+                const double wx = F * dx, wy = F * dy;
+                const double axx = Si.x + Sj[u].x, axy = Si.y + Sj[u].y, ayy = Si.z + Sj[u].z;
+                const double fx = axx * wx + axy * wy;
+                const double fy = axy * wx + ayy * wy;
+                const double cc = cond ? (Qi.x + Qj[u].x) * wx + (Qi.y + Qj[u].y) * wy : 0.0;
                 const double e = 0.5 * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
-                ax += mj[u] * fx;
-                ay += mj[u] * fy;
-                ud += mj[u] * (e + cc);
+                const double mj = Sj[u].w;
+                ax += mj * fx;
+                ay += mj * fy;
+                ud += mj * (e + cc);
             }
         }
     }
-    // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
-    // read the same row at step k (coalesced), whatever their counts
-    for (int k0 = 0; k0 < c.y; k0 += U64) {
-        int t[U64];
-        double2 rj[U64], vv[U64];
-        double Sw[U64], mj[U64];
-#pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
-#pragma unroll
-        for (int u = 0; u < U64; ++u) {
-            rj[u] = A.r[t[u]];
-            vv[u] = A.vp[t[u]];
-            Sw[u] = A.S[t[u]].w;
-            mj[u] = A.m[t[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < U64; ++u) {
-            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            min_image(dx, dy, B);
-            const double r2 = rsq_exact(dx, dy);
-            if (k0 + u < c.y && r2 < cutl) {
-                const double rr = sqrt(r2);
-                const double q = rr * B.ihl, w = 1.0 - q;
-                const double Fl = B.fhl * w * w;
-                const double cs = Si.w + Sw[u];
-                const double fx = cs * (Fl * dx), fy = cs * (Fl * dy);
-                const double e = 0.5 * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
-                ax += mj[u] * fx;
-                ay += mj[u] * fy;
-                ud += mj[u] * e;
-            }
-        }
-    }
+    const double2 ac = A.aco[s];
+    ax += ac.x;
+    ay += ac.y;
+    ud += A.uco[s];
     if (!isfinite(ax) || !isfinite(ay) || !isfinite(ud)) flag_health(A.dyn, b, HEALTH_NONFINITE);
     A.acc[s] = make_double2(ax, ay);
     A.udot[s] = ud;
@@ -610,6 +593,8 @@ __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const 
     dy = (float)(ddy * F.scly);
 }
 
+// P1 in FP32: rf carries position and mass, vf the predicted velocity; the
+// cohesive rate part (see k_density64) is accumulated with the densities.
 template <int U32>
 __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
 {
@@ -621,18 +606,24 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     if (!state_in(st, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
     const BoxF F = boxf(B);
+    const bool coh = (B.terms & 2u) != 0;
     const float4 ri = A.rf[s];
+    const float2 vi = A.vf[s];
     const int2 c = A.cnt[s];
     const size_t cap = A.cap;
     const unsigned *nb = A.nbr + s;
-    Kahan rho(F.qh), rhol(F.qhl);
+    Kahan rho(F.qh), rhol(F.qhl), gx, gy, ev;
     for (int k0 = 0; k0 < c.x; k0 += U32) {
         int t[U32];
         float4 rj[U32];
+        float2 vv[U32];
 #pragma unroll
         for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
 #pragma unroll
-        for (int u = 0; u < U32; ++u) rj[u] = A.rf[t[u]];
+        for (int u = 0; u < U32; ++u) {
+            rj[u] = A.rf[t[u]];
+            vv[u] = coh ? A.vf[t[u]] : vi;
+        }
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
@@ -643,6 +634,13 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
                 if (r2 < F.cutl) {
                     const float q = rr * F.ihl, w = 1.f - q;
                     rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
+                    if (coh) {
+                        const float Fl = F.fhl * w * w;
+                        const float fx = Fl * dx, fy = Fl * dy;
+                        gx.add(rj[u].z * fx);
+                        gy.add(rj[u].z * fy);
+                        ev.add(rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy));
+                    }
                 }
                 if (r2 < F.h2) {
                     const float q = rr * F.ih, w = 1.f - q;
@@ -656,10 +654,14 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     for (int k0 = 0; k0 < c.y; k0 += U32) {
         int t[U32];
         float4 rj[U32];
+        float2 vv[U32];
 #pragma unroll
         for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
 #pragma unroll
-        for (int u = 0; u < U32; ++u) rj[u] = A.rf[t[u]];
+        for (int u = 0; u < U32; ++u) {
+            rj[u] = A.rf[t[u]];
+            vv[u] = coh ? A.vf[t[u]] : vi;
+        }
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
@@ -669,10 +671,18 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
                 const float rr = sqrtf(r2);
                 const float q = rr * F.ihl, w = 1.f - q;
                 rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
+                if (coh) {
+                    const float Fl = F.fhl * w * w;
+                    const float fx = Fl * dx, fy = Fl * dy;
+                    gx.add(rj[u].z * fx);
+                    gy.add(rj[u].z * fy);
+                    ev.add(rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy));
+                }
             }
         }
     }
     const double rd = rho.value(), rl = rhol.value();
+    const double mi = A.m[s];
     double ui = A.up[s];
     if (st == BOX_INIT) {
         ui = B.kB * B.T0 - B.a * rd;
@@ -682,21 +692,22 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     double T = (ui + B.a * rd) / B.kB;
     T = T < 0.0 ? 0.0 : T;
     const double p = vdw_p(rd, T, B);
-    const double pco = -B.a * rl * rl;
     A.rho[s] = make_double2(rd, rl);
     A.T[s] = T;
-    A.rho32[s] = make_float2((float)rd, (float)rl);
-    A.T32[s] = (float)T;
     const double ps = (B.terms & 1u) ? p / (rd * rd) : 0.0;
-    const double cs = (B.terms & 2u) ? pco / (rl * rl) : 0.0;
-    A.S[s] = make_double4(ps, 0.0, ps, cs);
-    A.S32[s] = make_float4((float)ps, 0.f, (float)ps, (float)cs);
+    A.S[s] = make_double4(ps, 0.0, ps, mi);
+    A.S32[s] = make_float4((float)ps, 0.f, (float)ps, 0.f);
     A.Q[s] = make_double2(0.0, 0.0);
-    A.Q32[s] = make_float2(0.f, 0.f);
+    A.vq32[s] = make_float4(vi.x, vi.y, 0.f, 0.f);
+    A.vt32[s] = make_float2((float)(mi / rd), (float)T);
+    A.aco[s] = coh ? make_double2(-2.0 * B.a * gx.value(), -2.0 * B.a * gy.value()) : make_double2(0.0, 0.0);
+    A.uco[s] = coh ? -B.a * ev.value() : 0.0;
     if (!(rd * B.b < 1.0)) flag_health(A.dyn, b, HEALTH_EOS_POLE);
     if (!isfinite(rd) || !isfinite(rl) || !isfinite(ui)) flag_health(A.dyn, b, HEALTH_NONFINITE);
 }
 
+// P2 in FP32; neighbour record: rf, vf, (m/rho, T).  Writes the rate
+// pass's (v, q/rho^2) record.
 template <int U32>
 __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int diag)
 {
@@ -710,7 +721,7 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
     const BoxF F = boxf(B);
     const float4 ri = A.rf[s];
     const float2 vi = A.vf[s];
-    const float Ti = A.T32[s];
+    const float Ti = A.vt32[s].y;
     const int ns = A.cnt[s].x;
     const size_t cap = A.cap;
     const unsigned *nb = A.nbr + s;
@@ -718,16 +729,14 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32];
-        float2 vv[U32];
-        float rhoj[U32], Tj[U32];
+        float2 vv[U32], vt[U32];
 #pragma unroll
         for (int u = 0; u < U32; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
             vv[u] = A.vf[t[u]];
-            rhoj[u] = A.rho32[t[u]].x;
-            Tj[u] = A.T32[t[u]];
+            vt[u] = A.vt32[t[u]];
         }
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
@@ -739,9 +748,9 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
                 const float q = rr * F.ih, w = 1.f - q;
                 const float Fk = F.fh * w * w;
                 const float gx = -Fk * dx, gy = -Fk * dy;
-                const float vj = rj[u].z / rhoj[u];
+                const float vj = vt[u].x;
                 const float dvx = vv[u].x - vi.x, dvy = vv[u].y - vi.y;
-                const float dT = Tj[u] - Ti;
+                const float dT = vt[u].y - Ti;
                 g00.add(vj * dvx * gx);
                 g01.add(vj * dvx * gy);
                 g10.add(vj * dvy * gx);
@@ -763,20 +772,22 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
         const double4 S0 = A.S[s];
         const double4 Sn = make_double4((pr + pxx) / r2i, pxy / r2i, (pr + pyy) / r2i, S0.w);
         A.S[s] = Sn;
-        A.S32[s] = make_float4((float)Sn.x, (float)Sn.y, (float)Sn.z, (float)Sn.w);
+        A.S32[s] = make_float4((float)Sn.x, (float)Sn.y, (float)Sn.z, 0.f);
     }
     if (B.terms & 8u) {
         qx = -B.kcond * gt0.value();
         qy = -B.kcond * gt1.value();
         A.Q[s] = make_double2(qx / r2i, qy / r2i);
-        A.Q32[s] = make_float2((float)(qx / r2i), (float)(qy / r2i));
     }
+    A.vq32[s] = make_float4(vi.x, vi.y, (float)(qx / r2i), (float)(qy / r2i));
     if (diag) {
         A.gradv[s] = make_double4(G00, G01, G10, G11);
         A.jq[s] = make_double2(qx, qy);
     }
 }
 
+// P3 in FP32 over the short entries; neighbour record: rf (m in .z),
+// (v, q/rho^2), S.  Adds the cohesive part from P1.
 template <int U32>
 __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
 {
@@ -788,94 +799,49 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
     const BoxDesc &B = A.boxes[b];
     const BoxF F = boxf(B);
     const bool cond = (B.terms & 8u) != 0;
+    const bool shortr = (B.terms & 13u) != 0;
     const float4 ri = A.rf[s];
-    const float2 vi = A.vf[s];
+    const float4 vqi = A.vq32[s];
     const float4 Si = A.S32[s];
-    const float2 Qi = cond ? A.Q32[s] : make_float2(0.f, 0.f);
-    const int2 c = A.cnt[s];
+    const int ns = shortr ? A.cnt[s].x : 0;
     const size_t cap = A.cap;
     const unsigned *nb = A.nbr + s;
     Kahan ax, ay, ud;
-    for (int k0 = 0; k0 < c.x; k0 += U32) {
+    for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
-        float4 rj[U32], Sj[U32];
-        float2 vv[U32], Qj[U32];
+        float4 rj[U32], Sj[U32], vq[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
             Sj[u] = A.S32[t[u]];
-            vv[u] = A.vf[t[u]];
-            Qj[u] = cond ? A.Q32[t[u]] : make_float2(0.f, 0.f);
+            vq[u] = A.vq32[t[u]];
         }
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
             sep32(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
-            if (k0 + u < c.x && r2 < F.cut2) {
+            if (k0 + u < ns && r2 < F.cuts) {
                 const float rr = sqrtf(r2);
-                float fx = 0.f, fy = 0.f, cc = 0.f;
-                if (r2 < F.h2) {
-                    const float q = rr * F.ih, w = 1.f - q;
-                    const float Fk = F.fh * w * w;
-                    const float wx = Fk * dx, wy = Fk * dy;
-                    const float axx = Si.x + Sj[u].x, axy = Si.y + Sj[u].y, ayy = Si.z + Sj[u].z;
-                    fx = axx * wx + axy * wy;
-                    fy = axy * wx + ayy * wy;
-                    if (cond) cc = (Qi.x + Qj[u].x) * wx + (Qi.y + Qj[u].y) * wy;
-                }
-                if (r2 < F.cutl) {
-                    const float q = rr * F.ihl, w = 1.f - q;
-                    const float Fl = F.fhl * w * w;
-                    const float cs = Si.w + Sj[u].w;
-                    fx = fx + cs * (Fl * dx);
-                    fy = fy + cs * (Fl * dy);
-                }
+                const float q = rr * F.ih, w = 1.f - q;
+                const float Fk = F.fh * w * w;
+                const float wx = Fk * dx, wy = Fk * dy;
+                const float axx = Si.x + Sj[u].x, axy = Si.y + Sj[u].y, ayy = Si.z + Sj[u].z;
+                const float fx = axx * wx + axy * wy;
+                const float fy = axy * wx + ayy * wy;
+                const float cc = cond ? (vqi.z + vq[u].z) * wx + (vqi.w + vq[u].w) * wy : 0.f;
                 const float mj = rj[u].z;
-                const float e = 0.5f * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                const float e = 0.5f * ((vq[u].x - vqi.x) * fx + (vq[u].y - vqi.y) * fy);
                 ax.add(mj * fx);
                 ay.add(mj * fy);
                 ud.add(mj * (e + cc));
             }
         }
     }
-    // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
-    // read the same row at step k (coalesced), whatever their counts
-    for (int k0 = 0; k0 < c.y; k0 += U32) {
-        int t[U32];
-        float4 rj[U32];
-        float2 vv[U32];
-        float Sw[U32];
-#pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
-#pragma unroll
-        for (int u = 0; u < U32; ++u) {
-            rj[u] = A.rf[t[u]];
-            vv[u] = A.vf[t[u]];
-            Sw[u] = A.S32[t[u]].w;
-        }
-#pragma unroll
-        for (int u = 0; u < U32; ++u) {
-            float dx, dy;
-            sep32(ri, rj[u], F, dx, dy);
-            const float r2 = dx * dx + dy * dy;
-            if (k0 + u < c.y && r2 < F.cutl) {
-                const float rr = sqrtf(r2);
-                const float q = rr * F.ihl, w = 1.f - q;
-                const float Fl = F.fhl * w * w;
-                const float cs = Si.w + Sw[u];
-                const float fx = cs * (Fl * dx), fy = cs * (Fl * dy);
-                const float mj = rj[u].z;
-                const float e = 0.5f * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
-                ax.add(mj * fx);
-                ay.add(mj * fy);
-                ud.add(mj * e);
-            }
-        }
-    }
-    const double axd = ax.value(), ayd = ay.value(), udd = ud.value();
+    const double2 ac = A.aco[s];
+    const double axd = ax.value() + ac.x, ayd = ay.value() + ac.y, udd = ud.value() + A.uco[s];
     if (!isfinite(axd) || !isfinite(ayd) || !isfinite(udd)) flag_health(A.dyn, b, HEALTH_NONFINITE);
     A.acc[s] = make_double2(axd, ayd);
     A.udot[s] = udd;
