@@ -648,6 +648,14 @@ int Arena::upload(int b, int field, const double *host, int ncomp)
     HIPCHK(hipMemcpyAsync(d_stage, host, bytes, hipMemcpyHostToDevice, ctx->stream));
     launch_upload(view(), boxes[b].part_off, n, field, d_stage, ctx->stream);
     HIPCHK(hipGetLastError());
+    if (field == 2) {
+        // equal masses (every configuration of the reference's drivers):
+        // the density pass then skips the per-neighbour mass fetch
+        bool uni = n > 0 && host[0] > 0.0 && std::isfinite(host[0]);
+        for (int i = 1; uni && i < n; ++i) uni = host[i] == host[0];
+        boxes[b].munif = uni ? host[0] : 0.0;
+        HIPCHK(hipMemcpyAsync(d_boxes + b, &boxes[b], sizeof(BoxDesc), hipMemcpyHostToDevice, ctx->stream));
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SPH_OK;
 }
@@ -777,6 +785,8 @@ int sph_box_set_params(sph_box *box, const sph_params *prm)
     if (prm->precision != box->ar.precision) return fail(SPH_ERR_ARG, "precision cannot change");
     BoxDesc B;
     if (int rc = fill_desc(prm, box->ar.boxes[box->b].n, B)) return rc;
+    B.munif = box->ar.boxes[box->b].munif;  // state, not parameters
+    B.T0 = box->ar.boxes[box->b].T0;
     box->ar.boxes[box->b] = B;
     box->prm = *prm;
     return box->ar.relayout();

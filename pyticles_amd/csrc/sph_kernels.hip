@@ -291,6 +291,7 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
     if (!state_in(st, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
     const bool coh = (B.terms & 2u) != 0;
+    const double mu = B.munif;
     const double2 ri = A.r[s], vi = A.vp[s];
     const int2 c = A.cnt[s];
     const size_t cap = A.cap;
@@ -308,7 +309,7 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
-            mj[u] = A.m[t[u]];
+            mj[u] = mu > 0.0 ? mu : A.m[t[u]];
             vv[u] = coh ? A.vp[t[u]] : vi;
         }
 #pragma unroll
@@ -345,7 +346,7 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
-            mj[u] = A.m[t[u]];
+            mj[u] = mu > 0.0 ? mu : A.m[t[u]];
             vv[u] = coh ? A.vp[t[u]] : vi;
         }
 #pragma unroll
