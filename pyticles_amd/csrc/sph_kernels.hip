@@ -43,7 +43,8 @@ __device__ __forceinline__ int xcd_block()
 __device__ __forceinline__ int wrapi(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
 // Minimum image exactly as neighbour_list.py:110-123 (all '>' tests, then '<').
-__device__ __forceinline__ void min_image(double &dx, double &dy, const BoxDesc &B)
+template <class Box>
+__device__ __forceinline__ void min_image(double &dx, double &dy, const Box &B)
 {
     dx = dx > B.hLx ? dx - B.Lx : dx;
     dy = dy > B.hLy ? dy - B.Ly : dy;
@@ -261,6 +262,74 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
     max_bits(&D.amax2, ai.x * ai.x + ai.y * ai.y);
 }
 
+// ---------------------------------------------------------------------------
+// Box constants of the pair passes.  A wave whose active lanes all work on
+// one box (every wave but those straddling a box boundary) makes them
+// wave-uniform, so they sit in scalar registers for the whole list walk and
+// the loop issues no loads but the neighbour records.
+// ---------------------------------------------------------------------------
+struct BoxK {
+    double Lx, Ly, hLx, hLy;
+    double cut2, h2, hl2, cutl, cuth;  // cutl = min(cut2, hl2), cuth = min(cut2, h2)
+    double ih, ihl, qh, qhl, fh, fhl, munif;
+    int terms;
+};
+
+__device__ __forceinline__ double wave_uniform(double x)
+{
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ float wave_uniform(float x)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Runs f(bu) once for each distinct box bu among the active lanes, on the
+// lanes of that box, with bu wave-uniform (a waterfall over the boxes of the
+// wave: one round for all waves but those straddling a box boundary).
+template <class F>
+__device__ __forceinline__ void per_box(int b, F &&f)
+{
+    bool done = false;
+    for (;;) {
+        if (!done) {
+            const int bu = __builtin_amdgcn_readfirstlane(b);
+            if (b == bu) {
+                f(bu);
+                done = true;
+            }
+        }
+        if (__ballot(!done) == 0ull) break;
+    }
+}
+
+__device__ __forceinline__ BoxK box_k(const BoxDesc &B, bool uniform)
+{
+    BoxK k;
+    k.Lx = B.Lx; k.Ly = B.Ly; k.hLx = B.hLx; k.hLy = B.hLy;
+    k.cut2 = B.cut2; k.h2 = B.h2; k.hl2 = B.hl2;
+    k.cutl = B.cut2 < B.hl2 ? B.cut2 : B.hl2;
+    k.cuth = B.cut2 < B.h2 ? B.cut2 : B.h2;
+    k.ih = B.ih; k.ihl = B.ihl; k.qh = B.qh; k.qhl = B.qhl; k.fh = B.fh; k.fhl = B.fhl;
+    k.munif = B.munif;
+    k.terms = B.terms;
+    if (uniform) {
+        k.Lx = wave_uniform(k.Lx); k.Ly = wave_uniform(k.Ly); k.hLx = wave_uniform(k.hLx); k.hLy = wave_uniform(k.hLy);
+        k.cut2 = wave_uniform(k.cut2); k.h2 = wave_uniform(k.h2); k.hl2 = wave_uniform(k.hl2);
+        k.cutl = wave_uniform(k.cutl); k.cuth = wave_uniform(k.cuth);
+        k.ih = wave_uniform(k.ih); k.ihl = wave_uniform(k.ihl); k.qh = wave_uniform(k.qh); k.qhl = wave_uniform(k.qhl);
+        k.fh = wave_uniform(k.fh); k.fhl = wave_uniform(k.fhl); k.munif = wave_uniform(k.munif);
+        k.terms = wave_uniform(k.terms);
+    }
+    return k;
+}
+
 // ===========================================================================
 // FP64 form
 // ===========================================================================
@@ -268,6 +337,100 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
 __device__ __forceinline__ double vdw_p(double rho, double T, const BoxDesc &B)
 {
     return (rho * B.kB * T) / (1.0 - rho * B.b);  // properties.py:41
+}
+
+struct Dens64 {
+    double rho, rhol, gx, gy, ev;
+};
+
+// P1 list walk.  COH: accumulate the cohesive terms (needs velocities);
+// UNIM: all masses equal (no mass fetch).
+template <int U64, bool COH, bool UNIM>
+__device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK &K, int s)
+{
+    const double2 ri = A.r[s], vi = A.vp[s];
+    const int2 c = A.cnt[s];
+    const size_t cap = A.cap;
+    const unsigned *nb = A.nbr + s;
+    Dens64 d;
+    d.rho = K.qh;
+    d.rhol = K.qhl;
+    d.gx = d.gy = d.ev = 0.0;
+    // U64 entries per round: all their loads are issued before any use so
+    // several neighbour fetches are in flight per wavefront
+    for (int k0 = 0; k0 < c.x; k0 += U64) {
+        int t[U64];
+        double2 rj[U64], vv[U64];
+        double mj[U64];
+#pragma unroll
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            rj[u] = A.r[t[u]];
+            if (!UNIM) mj[u] = A.m[t[u]];
+            if (COH) vv[u] = A.vp[t[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
+            min_image(dx, dy, K);
+            const double r2 = rsq_exact(dx, dy);
+            if (k0 + u < c.x && r2 < K.cut2) {
+                const double m = UNIM ? K.munif : mj[u];
+                const double rr = sqrt(r2);
+                if (r2 < K.hl2) {
+                    const double q = rr * K.ihl, w = 1.0 - q;
+                    d.rhol += m * (K.qhl * (1.0 + 3.0 * q) * (w * w * w));
+                    if (COH) {
+                        const double Fl = K.fhl * w * w;
+                        const double fx = Fl * dx, fy = Fl * dy;
+                        d.gx += m * fx;
+                        d.gy += m * fy;
+                        d.ev += m * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                    }
+                }
+                if (r2 < K.h2) {
+                    const double q = rr * K.ih, w = 1.0 - q;
+                    d.rho += m * (K.qh * (1.0 + 3.0 * q) * (w * w * w));
+                }
+            }
+        }
+    }
+    // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
+    // read the same row at step k (coalesced), whatever their counts
+    for (int k0 = 0; k0 < c.y; k0 += U64) {
+        int t[U64];
+        double2 rj[U64], vv[U64];
+        double mj[U64];
+#pragma unroll
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            rj[u] = A.r[t[u]];
+            if (!UNIM) mj[u] = A.m[t[u]];
+            if (COH) vv[u] = A.vp[t[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
+            min_image(dx, dy, K);
+            const double r2 = rsq_exact(dx, dy);
+            if (k0 + u < c.y && r2 < K.cutl) {
+                const double m = UNIM ? K.munif : mj[u];
+                const double rr = sqrt(r2);
+                const double q = rr * K.ihl, w = 1.0 - q;
+                d.rhol += m * (K.qhl * (1.0 + 3.0 * q) * (w * w * w));
+                if (COH) {
+                    const double Fl = K.fhl * w * w;
+                    const double fx = Fl * dx, fy = Fl * dy;
+                    d.gx += m * fx;
+                    d.gy += m * fy;
+                    d.ev += m * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                }
+            }
+        }
+    }
+    return d;
 }
 
 // P1: rho_i = W(0,h) + sum_j m_j W(r_ij, h)  (self term not times m_i,
@@ -289,83 +452,16 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
     if (b < 0) return;
     const int st = A.dyn[b].state;
     if (!state_in(st, state_mask)) return;
+    Dens64 d;
+    per_box(b, [&](int bu) {
+        const BoxK K = box_k(A.boxes[bu], true);
+        const bool coh = (K.terms & 2) != 0, unim = K.munif > 0.0;
+        if (coh) d = unim ? density64_walk<U64, true, true>(A, K, s) : density64_walk<U64, true, false>(A, K, s);
+        else d = unim ? density64_walk<U64, false, true>(A, K, s) : density64_walk<U64, false, false>(A, K, s);
+    });
     const BoxDesc &B = A.boxes[b];
     const bool coh = (B.terms & 2u) != 0;
-    const double mu = B.munif;
-    const double2 ri = A.r[s], vi = A.vp[s];
-    const int2 c = A.cnt[s];
-    const size_t cap = A.cap;
-    const unsigned *nb = A.nbr + s;
-    double rho = B.qh, rhol = B.qhl, gx = 0.0, gy = 0.0, ev = 0.0;
-    const double cutl = B.cut2 < B.hl2 ? B.cut2 : B.hl2;
-    // U64 entries per round: all their loads are issued before any use so
-    // several neighbour fetches are in flight per wavefront
-    for (int k0 = 0; k0 < c.x; k0 += U64) {
-        int t[U64];
-        double2 rj[U64], vv[U64];
-        double mj[U64];
-#pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
-#pragma unroll
-        for (int u = 0; u < U64; ++u) {
-            rj[u] = A.r[t[u]];
-            mj[u] = mu > 0.0 ? mu : A.m[t[u]];
-            vv[u] = coh ? A.vp[t[u]] : vi;
-        }
-#pragma unroll
-        for (int u = 0; u < U64; ++u) {
-            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            min_image(dx, dy, B);
-            const double r2 = rsq_exact(dx, dy);
-            if (k0 + u < c.x && r2 < B.cut2) {
-                const double rr = sqrt(r2);
-                if (r2 < B.hl2) {
-                    const double q = rr * B.ihl, w = 1.0 - q;
-                    rhol += mj[u] * (B.qhl * (1.0 + 3.0 * q) * (w * w * w));
-                    const double Fl = B.fhl * w * w;
-                    const double fx = Fl * dx, fy = Fl * dy;
-                    gx += mj[u] * fx;
-                    gy += mj[u] * fy;
-                    ev += mj[u] * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
-                }
-                if (r2 < B.h2) {
-                    const double q = rr * B.ih, w = 1.0 - q;
-                    rho += mj[u] * (B.qh * (1.0 + 3.0 * q) * (w * w * w));
-                }
-            }
-        }
-    }
-    // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
-    // read the same row at step k (coalesced), whatever their counts
-    for (int k0 = 0; k0 < c.y; k0 += U64) {
-        int t[U64];
-        double2 rj[U64], vv[U64];
-        double mj[U64];
-#pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
-#pragma unroll
-        for (int u = 0; u < U64; ++u) {
-            rj[u] = A.r[t[u]];
-            mj[u] = mu > 0.0 ? mu : A.m[t[u]];
-            vv[u] = coh ? A.vp[t[u]] : vi;
-        }
-#pragma unroll
-        for (int u = 0; u < U64; ++u) {
-            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            min_image(dx, dy, B);
-            const double r2 = rsq_exact(dx, dy);
-            if (k0 + u < c.y && r2 < cutl) {
-                const double rr = sqrt(r2);
-                const double q = rr * B.ihl, w = 1.0 - q;
-                rhol += mj[u] * (B.qhl * (1.0 + 3.0 * q) * (w * w * w));
-                const double Fl = B.fhl * w * w;
-                const double fx = Fl * dx, fy = Fl * dy;
-                gx += mj[u] * fx;
-                gy += mj[u] * fy;
-                ev += mj[u] * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
-            }
-        }
-    }
+    const double rho = d.rho, rhol = d.rhol;
     const double mi = A.m[s];
     double ui = A.up[s];
     if (st == BOX_INIT) {  // u = kB T0 - a rho (properties.py:44-46,119)
@@ -382,10 +478,60 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
     A.S[s] = make_double4(ps, 0.0, ps, mi);  // .w carries m for the rate pass
     A.Q[s] = make_double2(0.0, 0.0);
     A.vt[s] = make_double2(mi / rho, T);
-    A.aco[s] = coh ? make_double2(-2.0 * B.a * gx, -2.0 * B.a * gy) : make_double2(0.0, 0.0);
-    A.uco[s] = coh ? -B.a * ev : 0.0;
+    A.aco[s] = coh ? make_double2(-2.0 * B.a * d.gx, -2.0 * B.a * d.gy) : make_double2(0.0, 0.0);
+    A.uco[s] = coh ? -B.a * d.ev : 0.0;
     if (!(rho * B.b < 1.0)) flag_health(A.dyn, b, HEALTH_EOS_POLE);
     if (!isfinite(rho) || !isfinite(rhol) || !isfinite(ui)) flag_health(A.dyn, b, HEALTH_NONFINITE);
+}
+
+struct Grad64 {
+    double g00, g01, g10, g11, gt0, gt1;
+};
+
+template <int U64>
+__device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K, int s)
+{
+    const double2 ri = A.r[s], vi = A.vp[s];
+    const double Ti = A.T[s];
+    const int ns = A.cnt[s].x;
+    const size_t cap = A.cap;
+    const unsigned *nb = A.nbr + s;
+    Grad64 g;
+    g.g00 = g.g01 = g.g10 = g.g11 = g.gt0 = g.gt1 = 0.0;
+    for (int k0 = 0; k0 < ns; k0 += U64) {
+        int t[U64];
+        double2 rj[U64], vv[U64], vt[U64];
+#pragma unroll
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            rj[u] = A.r[t[u]];
+            vv[u] = A.vp[t[u]];
+            vt[u] = A.vt[t[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
+            min_image(dx, dy, K);
+            const double r2 = rsq_exact(dx, dy);
+            if (k0 + u < ns && r2 < K.cuth) {
+                const double rr = sqrt(r2);
+                const double q = rr * K.ih, w = 1.0 - q;
+                const double F = K.fh * w * w;
+                const double gx = -F * dx, gy = -F * dy;
+                const double vj = vt[u].x;
+                const double dvx = vv[u].x - vi.x, dvy = vv[u].y - vi.y;
+                const double dT = vt[u].y - Ti;
+                g.g00 += vj * dvx * gx;
+                g.g01 += vj * dvx * gy;
+                g.g10 += vj * dvy * gx;
+                g.g11 += vj * dvy * gy;
+                g.gt0 += vj * dT * gx;
+                g.gt1 += vj * dT * gy;
+            }
+        }
+    }
+    return g;
 }
 
 // P2: grad v and grad T with grad_i W_ij = -dwdx_ij (dwdx from r_j - r_i);
@@ -402,67 +548,82 @@ __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int 
     if (!state_in(A.dyn[b].state, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
     if (!diag && !(B.terms & 12u)) return;
-    const double2 ri = A.r[s], vi = A.vp[s];
+    Grad64 g;
+    per_box(b, [&](int bu) { g = grad64_walk<U64>(A, box_k(A.boxes[bu], true), s); });
     const double Ti = A.T[s];
-    const double cut = B.R2list < B.h2 ? B.R2list : B.h2;
-    const int ns = A.cnt[s].x;
-    const size_t cap = A.cap;
-    const unsigned *nb = A.nbr + s;
-    double g00 = 0, g01 = 0, g10 = 0, g11 = 0, gt0 = 0, gt1 = 0;
-    for (int k0 = 0; k0 < ns; k0 += U64) {
-        int t[U64];
-        double2 rj[U64], vv[U64], vt[U64];
-#pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
-#pragma unroll
-        for (int u = 0; u < U64; ++u) {
-            rj[u] = A.r[t[u]];
-            vv[u] = A.vp[t[u]];
-            vt[u] = A.vt[t[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < U64; ++u) {
-            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            min_image(dx, dy, B);
-            const double r2 = rsq_exact(dx, dy);
-            if (k0 + u < ns && r2 < cut) {
-                const double rr = sqrt(r2);
-                const double q = rr * B.ih, w = 1.0 - q;
-                const double F = B.fh * w * w;
-                const double gx = -F * dx, gy = -F * dy;
-                const double vj = vt[u].x;
-                const double dvx = vv[u].x - vi.x, dvy = vv[u].y - vi.y;
-                const double dT = vt[u].y - Ti;
-                g00 += vj * dvx * gx;
-                g01 += vj * dvx * gy;
-                g10 += vj * dvy * gx;
-                g11 += vj * dvy * gy;
-                gt0 += vj * dT * gx;
-                gt1 += vj * dT * gy;
-            }
-        }
-    }
     const double rho = A.rho[s].x, r2i = rho * rho;
     double pxx = 0, pxy = 0, pyy = 0;
     if (B.terms & 4u) {
-        const double tr = g00 + g11;
-        pxx = -B.eta * (2.0 * g00 - tr) - B.zeta * tr;
-        pyy = -B.eta * (2.0 * g11 - tr) - B.zeta * tr;
-        pxy = -B.eta * (g01 + g10);
+        const double tr = g.g00 + g.g11;
+        pxx = -B.eta * (2.0 * g.g00 - tr) - B.zeta * tr;
+        pyy = -B.eta * (2.0 * g.g11 - tr) - B.zeta * tr;
+        pxy = -B.eta * (g.g01 + g.g10);
         const double pr = (B.terms & 1u) ? vdw_p(rho, Ti, B) : 0.0;
         const double4 S0 = A.S[s];
         A.S[s] = make_double4((pr + pxx) / r2i, pxy / r2i, (pr + pyy) / r2i, S0.w);
     }
     double qx = 0, qy = 0;
     if (B.terms & 8u) {
-        qx = -B.kcond * gt0;
-        qy = -B.kcond * gt1;
+        qx = -B.kcond * g.gt0;
+        qy = -B.kcond * g.gt1;
         A.Q[s] = make_double2(qx / r2i, qy / r2i);
     }
     if (diag) {
-        A.gradv[s] = make_double4(g00, g01, g10, g11);
+        A.gradv[s] = make_double4(g.g00, g.g01, g.g10, g.g11);
         A.jq[s] = make_double2(qx, qy);
     }
+}
+
+struct Rate64 {
+    double ax, ay, ud;
+};
+
+template <int U64, bool COND>
+__device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K, int s, int ns)
+{
+    const double2 ri = A.r[s], vi = A.vp[s];
+    const double4 Si = A.S[s];
+    const double2 Qi = COND ? A.Q[s] : make_double2(0.0, 0.0);
+    const size_t cap = A.cap;
+    const unsigned *nb = A.nbr + s;
+    Rate64 o;
+    o.ax = o.ay = o.ud = 0.0;
+    for (int k0 = 0; k0 < ns; k0 += U64) {
+        int t[U64];
+        double2 rj[U64], vv[U64], Qj[U64];
+        double4 Sj[U64];
+#pragma unroll
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            rj[u] = A.r[t[u]];
+            vv[u] = A.vp[t[u]];
+            Sj[u] = A.S[t[u]];
+            if (COND) Qj[u] = A.Q[t[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U64; ++u) {
+            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
+            min_image(dx, dy, K);
+            const double r2 = rsq_exact(dx, dy);
+            if (k0 + u < ns && r2 < K.cuth) {
+                const double rr = sqrt(r2);
+                const double q = rr * K.ih, w = 1.0 - q;
+                const double F = K.fh * w * w;
+                const double wx = F * dx, wy = F * dy;
+                const double axx = Si.x + Sj[u].x, axy = Si.y + Sj[u].y, ayy = Si.z + Sj[u].z;
+                const double fx = axx * wx + axy * wy;
+                const double fy = axy * wx + ayy * wy;
+                const double cc = COND ? (Qi.x + Qj[u].x) * wx + (Qi.y + Qj[u].y) * wy : 0.0;
+                const double e = 0.5 * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                const double mj = Sj[u].w;
+                o.ax += mj * fx;
+                o.ay += mj * fy;
+                o.ud += mj * (e + cc);
+            }
+        }
+    }
+    return o;
 }
 
 // P3: rates over the short entries, plus the cohesive part from P1.
@@ -478,55 +639,14 @@ __global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
     if (b < 0) return;
     if (!state_in(A.dyn[b].state, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
-    const bool cond = (B.terms & 8u) != 0;
-    const bool shortr = (B.terms & 13u) != 0;
-    const double2 ri = A.r[s], vi = A.vp[s];
-    const double4 Si = A.S[s];
-    const double2 Qi = cond ? A.Q[s] : make_double2(0.0, 0.0);
-    const int ns = shortr ? A.cnt[s].x : 0;
-    const size_t cap = A.cap;
-    const unsigned *nb = A.nbr + s;
-    const double cut = B.cut2 < B.h2 ? B.cut2 : B.h2;
-    double ax = 0, ay = 0, ud = 0;
-    for (int k0 = 0; k0 < ns; k0 += U64) {
-        int t[U64];
-        double2 rj[U64], vv[U64], Qj[U64];
-        double4 Sj[U64];
-#pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
-#pragma unroll
-        for (int u = 0; u < U64; ++u) {
-            rj[u] = A.r[t[u]];
-            vv[u] = A.vp[t[u]];
-            Sj[u] = A.S[t[u]];
-            Qj[u] = cond ? A.Q[t[u]] : make_double2(0.0, 0.0);
-        }
-#pragma unroll
-        for (int u = 0; u < U64; ++u) {
-            double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            min_image(dx, dy, B);
-            const double r2 = rsq_exact(dx, dy);
-            if (k0 + u < ns && r2 < cut) {
-                const double rr = sqrt(r2);
-                const double q = rr * B.ih, w = 1.0 - q;
-                const double F = B.fh * w * w;
-                const double wx = F * dx, wy = F * dy;
-                const double axx = Si.x + Sj[u].x, axy = Si.y + Sj[u].y, ayy = Si.z + Sj[u].z;
-                const double fx = axx * wx + axy * wy;
-                const double fy = axy * wx + ayy * wy;
-                const double cc = cond ? (Qi.x + Qj[u].x) * wx + (Qi.y + Qj[u].y) * wy : 0.0;
-                const double e = 0.5 * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
-                const double mj = Sj[u].w;
-                ax += mj * fx;
-                ay += mj * fy;
-                ud += mj * (e + cc);
-            }
-        }
-    }
+    const int ns = (B.terms & 13u) ? A.cnt[s].x : 0;
+    Rate64 o;
+    per_box(b, [&](int bu) {
+        const BoxK K = box_k(A.boxes[bu], true);
+        o = (K.terms & 8) ? rates64_walk<U64, true>(A, K, s, ns) : rates64_walk<U64, false>(A, K, s, ns);
+    });
     const double2 ac = A.aco[s];
-    ax += ac.x;
-    ay += ac.y;
-    ud += A.uco[s];
+    const double ax = o.ax + ac.x, ay = o.ay + ac.y, ud = o.ud + A.uco[s];
     if (!isfinite(ax) || !isfinite(ay) || !isfinite(ud)) flag_health(A.dyn, b, HEALTH_NONFINITE);
     A.acc[s] = make_double2(ax, ay);
     A.udot[s] = ud;
@@ -555,10 +675,10 @@ struct BoxF {
     double sclx, scly;  // length of one fixed-point unit
     float cut2, cutl, h2, cuts;
     float ih, ihl, qh, qhl, fh, fhl;
-    int ncx, ncy, sx, sy;
+    int ncx, ncy, sx, sy, terms;
 };
 
-__device__ __forceinline__ BoxF boxf(const BoxDesc &B)
+__device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
 {
     BoxF f;
     f.sclx = B.cwx / FIX_ONE; f.scly = B.cwy / FIX_ONE;
@@ -566,7 +686,15 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B)
     f.cuts = (float)(B.R2list < B.h2 ? B.R2list : B.h2);
     f.ih = (float)B.ih; f.ihl = (float)B.ihl; f.qh = (float)B.qh; f.qhl = (float)B.qhl;
     f.fh = (float)B.fh; f.fhl = (float)B.fhl;
-    f.ncx = B.ncx; f.ncy = B.ncy; f.sx = B.sx; f.sy = B.sy;
+    f.ncx = B.ncx; f.ncy = B.ncy; f.sx = B.sx; f.sy = B.sy; f.terms = B.terms;
+    if (uniform) {
+        f.sclx = wave_uniform(f.sclx); f.scly = wave_uniform(f.scly);
+        f.cut2 = wave_uniform(f.cut2); f.cutl = wave_uniform(f.cutl); f.h2 = wave_uniform(f.h2);
+        f.cuts = wave_uniform(f.cuts); f.ih = wave_uniform(f.ih); f.ihl = wave_uniform(f.ihl);
+        f.qh = wave_uniform(f.qh); f.qhl = wave_uniform(f.qhl); f.fh = wave_uniform(f.fh); f.fhl = wave_uniform(f.fhl);
+        f.ncx = wave_uniform(f.ncx); f.ncy = wave_uniform(f.ncy); f.sx = wave_uniform(f.sx); f.sy = wave_uniform(f.sy);
+        f.terms = wave_uniform(f.terms);
+    }
     return f;
 }
 
@@ -594,26 +722,18 @@ __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const 
     dy = (float)(ddy * F.scly);
 }
 
-// P1 in FP32: rf carries position and mass, vf the predicted velocity; the
-// cohesive rate part (see k_density64) is accumulated with the densities.
-template <int U32>
-__global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
+struct Dens32 {
+    Kahan rho, rhol, gx, gy, ev;
+};
+
+template <int U32, bool COH>
+__device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F, int s, Dens32 &d)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
-    if (s >= A.n) return;
-    const int b = A.slot_box[s];
-    if (b < 0) return;
-    const int st = A.dyn[b].state;
-    if (!state_in(st, state_mask)) return;
-    const BoxDesc &B = A.boxes[b];
-    const BoxF F = boxf(B);
-    const bool coh = (B.terms & 2u) != 0;
     const float4 ri = A.rf[s];
     const float2 vi = A.vf[s];
     const int2 c = A.cnt[s];
     const size_t cap = A.cap;
     const unsigned *nb = A.nbr + s;
-    Kahan rho(F.qh), rhol(F.qhl), gx, gy, ev;
     for (int k0 = 0; k0 < c.x; k0 += U32) {
         int t[U32];
         float4 rj[U32];
@@ -623,7 +743,7 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
-            vv[u] = coh ? A.vf[t[u]] : vi;
+            if (COH) vv[u] = A.vf[t[u]];
         }
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
@@ -634,18 +754,18 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
                 const float rr = sqrtf(r2);
                 if (r2 < F.cutl) {
                     const float q = rr * F.ihl, w = 1.f - q;
-                    rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
-                    if (coh) {
+                    d.rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
+                    if (COH) {
                         const float Fl = F.fhl * w * w;
                         const float fx = Fl * dx, fy = Fl * dy;
-                        gx.add(rj[u].z * fx);
-                        gy.add(rj[u].z * fy);
-                        ev.add(rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy));
+                        d.gx.add(rj[u].z * fx);
+                        d.gy.add(rj[u].z * fy);
+                        d.ev.add(rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy));
                     }
                 }
                 if (r2 < F.h2) {
                     const float q = rr * F.ih, w = 1.f - q;
-                    rho.add(rj[u].z * (F.qh * (1.f + 3.f * q) * (w * w * w)));
+                    d.rho.add(rj[u].z * (F.qh * (1.f + 3.f * q) * (w * w * w)));
                 }
             }
         }
@@ -661,7 +781,7 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
-            vv[u] = coh ? A.vf[t[u]] : vi;
+            if (COH) vv[u] = A.vf[t[u]];
         }
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
@@ -671,18 +791,42 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
             if (k0 + u < c.y && r2 < F.cutl) {
                 const float rr = sqrtf(r2);
                 const float q = rr * F.ihl, w = 1.f - q;
-                rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
-                if (coh) {
+                d.rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
+                if (COH) {
                     const float Fl = F.fhl * w * w;
                     const float fx = Fl * dx, fy = Fl * dy;
-                    gx.add(rj[u].z * fx);
-                    gy.add(rj[u].z * fy);
-                    ev.add(rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy));
+                    d.gx.add(rj[u].z * fx);
+                    d.gy.add(rj[u].z * fy);
+                    d.ev.add(rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy));
                 }
             }
         }
     }
-    const double rd = rho.value(), rl = rhol.value();
+}
+
+// P1 in FP32: rf carries position and mass, vf the predicted velocity; the
+// cohesive rate part (see k_density64) is accumulated with the densities.
+template <int U32>
+__global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    const int st = A.dyn[b].state;
+    if (!state_in(st, state_mask)) return;
+    const BoxDesc &B = A.boxes[b];
+    const bool coh = (B.terms & 2u) != 0;
+    Dens32 d;
+    per_box(b, [&](int bu) {
+        const BoxF F = boxf(A.boxes[bu], true);
+        d.rho = Kahan(F.qh);
+        d.rhol = Kahan(F.qhl);
+        if (F.terms & 2) density32_walk<U32, true>(A, F, s, d);
+        else density32_walk<U32, false>(A, F, s, d);
+    });
+    const float2 vi = A.vf[s];
+    const double rd = d.rho.value(), rl = d.rhol.value();
     const double mi = A.m[s];
     double ui = A.up[s];
     if (st == BOX_INIT) {
@@ -701,32 +845,25 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     A.Q[s] = make_double2(0.0, 0.0);
     A.vq32[s] = make_float4(vi.x, vi.y, 0.f, 0.f);
     A.vt32[s] = make_float2((float)(mi / rd), (float)T);
-    A.aco[s] = coh ? make_double2(-2.0 * B.a * gx.value(), -2.0 * B.a * gy.value()) : make_double2(0.0, 0.0);
-    A.uco[s] = coh ? -B.a * ev.value() : 0.0;
+    A.aco[s] = coh ? make_double2(-2.0 * B.a * d.gx.value(), -2.0 * B.a * d.gy.value()) : make_double2(0.0, 0.0);
+    A.uco[s] = coh ? -B.a * d.ev.value() : 0.0;
     if (!(rd * B.b < 1.0)) flag_health(A.dyn, b, HEALTH_EOS_POLE);
     if (!isfinite(rd) || !isfinite(rl) || !isfinite(ui)) flag_health(A.dyn, b, HEALTH_NONFINITE);
 }
 
-// P2 in FP32; neighbour record: rf, vf, (m/rho, T).  Writes the rate
-// pass's (v, q/rho^2) record.
+struct Grad32 {
+    Kahan g00, g01, g10, g11, gt0, gt1;
+};
+
 template <int U32>
-__global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int diag)
+__device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, int s, Grad32 &g)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
-    if (s >= A.n) return;
-    const int b = A.slot_box[s];
-    if (b < 0) return;
-    if (!state_in(A.dyn[b].state, state_mask)) return;
-    const BoxDesc &B = A.boxes[b];
-    if (!diag && !(B.terms & 12u)) return;
-    const BoxF F = boxf(B);
     const float4 ri = A.rf[s];
     const float2 vi = A.vf[s];
     const float Ti = A.vt32[s].y;
     const int ns = A.cnt[s].x;
     const size_t cap = A.cap;
     const unsigned *nb = A.nbr + s;
-    Kahan g00, g01, g10, g11, gt0, gt1;
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32];
@@ -752,16 +889,33 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
                 const float vj = vt[u].x;
                 const float dvx = vv[u].x - vi.x, dvy = vv[u].y - vi.y;
                 const float dT = vt[u].y - Ti;
-                g00.add(vj * dvx * gx);
-                g01.add(vj * dvx * gy);
-                g10.add(vj * dvy * gx);
-                g11.add(vj * dvy * gy);
-                gt0.add(vj * dT * gx);
-                gt1.add(vj * dT * gy);
+                g.g00.add(vj * dvx * gx);
+                g.g01.add(vj * dvx * gy);
+                g.g10.add(vj * dvy * gx);
+                g.g11.add(vj * dvy * gy);
+                g.gt0.add(vj * dT * gx);
+                g.gt1.add(vj * dT * gy);
             }
         }
     }
-    const double G00 = g00.value(), G01 = g01.value(), G10 = g10.value(), G11 = g11.value();
+}
+
+// P2 in FP32; neighbour record: rf, vf, (m/rho, T).  Writes the rate
+// pass's (v, q/rho^2) record.
+template <int U32>
+__global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int diag)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    if (!state_in(A.dyn[b].state, state_mask)) return;
+    const BoxDesc &B = A.boxes[b];
+    if (!diag && !(B.terms & 12u)) return;
+    Grad32 g;
+    per_box(b, [&](int bu) { grad32_walk<U32>(A, boxf(A.boxes[bu], true), s, g); });
+    const float2 vi = A.vf[s];
+    const double G00 = g.g00.value(), G01 = g.g01.value(), G10 = g.g10.value(), G11 = g.g11.value();
     const double rho = A.rho[s].x, r2i = rho * rho;
     double qx = 0, qy = 0;
     if (B.terms & 4u) {
@@ -776,8 +930,8 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
         A.S32[s] = make_float4((float)Sn.x, (float)Sn.y, (float)Sn.z, 0.f);
     }
     if (B.terms & 8u) {
-        qx = -B.kcond * gt0.value();
-        qy = -B.kcond * gt1.value();
+        qx = -B.kcond * g.gt0.value();
+        qy = -B.kcond * g.gt1.value();
         A.Q[s] = make_double2(qx / r2i, qy / r2i);
     }
     A.vq32[s] = make_float4(vi.x, vi.y, (float)(qx / r2i), (float)(qy / r2i));
@@ -787,27 +941,18 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
     }
 }
 
-// P3 in FP32 over the short entries; neighbour record: rf (m in .z),
-// (v, q/rho^2), S.  Adds the cohesive part from P1.
-template <int U32>
-__global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
+struct Rate32 {
+    Kahan ax, ay, ud;
+};
+
+template <int U32, bool COND>
+__device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, int s, int ns, Rate32 &o)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
-    if (s >= A.n) return;
-    const int b = A.slot_box[s];
-    if (b < 0) return;
-    if (!state_in(A.dyn[b].state, state_mask)) return;
-    const BoxDesc &B = A.boxes[b];
-    const BoxF F = boxf(B);
-    const bool cond = (B.terms & 8u) != 0;
-    const bool shortr = (B.terms & 13u) != 0;
     const float4 ri = A.rf[s];
     const float4 vqi = A.vq32[s];
     const float4 Si = A.S32[s];
-    const int ns = shortr ? A.cnt[s].x : 0;
     const size_t cap = A.cap;
     const unsigned *nb = A.nbr + s;
-    Kahan ax, ay, ud;
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32], Sj[U32], vq[U32];
@@ -832,17 +977,37 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
                 const float axx = Si.x + Sj[u].x, axy = Si.y + Sj[u].y, ayy = Si.z + Sj[u].z;
                 const float fx = axx * wx + axy * wy;
                 const float fy = axy * wx + ayy * wy;
-                const float cc = cond ? (vqi.z + vq[u].z) * wx + (vqi.w + vq[u].w) * wy : 0.f;
+                const float cc = COND ? (vqi.z + vq[u].z) * wx + (vqi.w + vq[u].w) * wy : 0.f;
                 const float mj = rj[u].z;
                 const float e = 0.5f * ((vq[u].x - vqi.x) * fx + (vq[u].y - vqi.y) * fy);
-                ax.add(mj * fx);
-                ay.add(mj * fy);
-                ud.add(mj * (e + cc));
+                o.ax.add(mj * fx);
+                o.ay.add(mj * fy);
+                o.ud.add(mj * (e + cc));
             }
         }
     }
+}
+
+// P3 in FP32 over the short entries; neighbour record: rf (m in .z),
+// (v, q/rho^2), S.  Adds the cohesive part from P1.
+template <int U32>
+__global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = A.slot_box[s];
+    if (b < 0) return;
+    if (!state_in(A.dyn[b].state, state_mask)) return;
+    const BoxDesc &B = A.boxes[b];
+    const int ns = (B.terms & 13u) ? A.cnt[s].x : 0;
+    Rate32 o;
+    per_box(b, [&](int bu) {
+        const BoxF F = boxf(A.boxes[bu], true);
+        if (F.terms & 8) rates32_walk<U32, true>(A, F, s, ns, o);
+        else rates32_walk<U32, false>(A, F, s, ns, o);
+    });
     const double2 ac = A.aco[s];
-    const double axd = ax.value() + ac.x, ayd = ay.value() + ac.y, udd = ud.value() + A.uco[s];
+    const double axd = o.ax.value() + ac.x, ayd = o.ay.value() + ac.y, udd = o.ud.value() + A.uco[s];
     if (!isfinite(axd) || !isfinite(ayd) || !isfinite(udd)) flag_health(A.dyn, b, HEALTH_NONFINITE);
     A.acc[s] = make_double2(axd, ayd);
     A.udot[s] = udd;
