@@ -111,10 +111,11 @@ __device__ __forceinline__ void max_bits(unsigned long long *addr, double x)
 // integers plus whole cells, so the FP32 separation is the exact one
 // rounded once (FP32 coordinates would round each position first and leave
 // several ulps of the coordinate in every separation).  Offsets are clamped
-// at 1.5 cell widths, far beyond the half skin a particle may move between
-// builds before the skin check trips.
+// at 1.25 cell widths, far beyond the half skin a particle may move between
+// builds before the skin check trips.  The clamp keeps a difference of two
+// offsets plus one cell width (at most 3.5 * 2^29) inside an int.
 static constexpr double FIX_ONE = 536870912.0;  // 2^29
-static constexpr double FIX_HALF = 268435456.0;
+static constexpr int FIX_SHIFT = 29;
 
 __device__ __forceinline__ float4 fix_pos(double2 r, int cell, double m, const BoxDesc &B)
 {
@@ -122,8 +123,8 @@ __device__ __forceinline__ float4 fix_pos(double2 r, int cell, double m, const B
     double xl = r.x - cx * B.cwx, yl = r.y - cy * B.cwy;
     xl = xl > B.hLx ? xl - B.Lx : (xl < -B.hLx ? xl + B.Lx : xl);
     yl = yl > B.hLy ? yl - B.Ly : (yl < -B.hLy ? yl + B.Ly : yl);
-    const double fx = fmin(fmax(xl * B.icwx, -1.5), 1.5) * FIX_ONE;
-    const double fy = fmin(fmax(yl * B.icwy, -1.5), 1.5) * FIX_ONE;
+    const double fx = fmin(fmax(xl * B.icwx, -1.25), 1.25) * FIX_ONE;
+    const double fy = fmin(fmax(yl * B.icwy, -1.25), 1.25) * FIX_ONE;
     return make_float4(__int_as_float((int)rint(fx)), __int_as_float((int)rint(fy)), (float)m,
                        __int_as_float(cx | (cy << 16)));
 }
@@ -700,32 +701,34 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
 
 // Separation of rj from ri (fixed-point records): the integer difference
 // plus the cell offset (with the periodic wrap of the cell index) is exact in
-// FP64, and so is the minimum image of an axis with a single cell (period
-// 2^29 units); one rounding to FP32 at the end.
+// an int, and so is the minimum image of an axis with a single cell (period
+// 2^29 units); one FP64 scale and one rounding to FP32 at the end.
 __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const BoxF &F, float &dx, float &dy)
 {
     const int ci = __float_as_int(ri.w), cj = __float_as_int(rj.w);
     int ox = (cj & 0xffff) - (ci & 0xffff), oy = (cj >> 16) - (ci >> 16);
     ox = ox > 1 ? ox - F.ncx : (ox < -1 ? ox + F.ncx : ox);
     oy = oy > 1 ? oy - F.ncy : (oy < -1 ? oy + F.ncy : oy);
-    double ddx = (double)(__float_as_int(rj.x) - __float_as_int(ri.x)) + (double)ox * FIX_ONE;
-    double ddy = (double)(__float_as_int(rj.y) - __float_as_int(ri.y)) + (double)oy * FIX_ONE;
+    int ix = (__float_as_int(rj.x) - __float_as_int(ri.x)) + ox * (1 << FIX_SHIFT);
+    int iy = (__float_as_int(rj.y) - __float_as_int(ri.y)) + oy * (1 << FIX_SHIFT);
     if (F.sx == 0) {
-        ddx = ddx > FIX_HALF ? ddx - FIX_ONE : ddx;
-        ddx = ddx < -FIX_HALF ? ddx + FIX_ONE : ddx;
+        ix = ix > (1 << (FIX_SHIFT - 1)) ? ix - (1 << FIX_SHIFT) : ix;
+        ix = ix < -(1 << (FIX_SHIFT - 1)) ? ix + (1 << FIX_SHIFT) : ix;
     }
     if (F.sy == 0) {
-        ddy = ddy > FIX_HALF ? ddy - FIX_ONE : ddy;
-        ddy = ddy < -FIX_HALF ? ddy + FIX_ONE : ddy;
+        iy = iy > (1 << (FIX_SHIFT - 1)) ? iy - (1 << FIX_SHIFT) : iy;
+        iy = iy < -(1 << (FIX_SHIFT - 1)) ? iy + (1 << FIX_SHIFT) : iy;
     }
-    dx = (float)(ddx * F.sclx);
-    dy = (float)(ddy * F.scly);
+    dx = (float)((double)ix * F.sclx);
+    dy = (float)((double)iy * F.scly);
 }
 
 struct Dens32 {
     Kahan rho, rhol, gx, gy, ev;
 };
 
+// FP32 sums: the terms of one unrolled round (U32 list entries) are added in
+// FP32, and each round's partial sum goes into a Kahan-compensated total.
 template <int U32, bool COH>
 __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F, int s, Dens32 &d)
 {
@@ -745,6 +748,7 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
             rj[u] = A.rf[t[u]];
             if (COH) vv[u] = A.vf[t[u]];
         }
+        float grho = 0.f, grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
@@ -754,20 +758,27 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
                 const float rr = sqrtf(r2);
                 if (r2 < F.cutl) {
                     const float q = rr * F.ihl, w = 1.f - q;
-                    d.rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
+                    grhol += rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w));
                     if (COH) {
                         const float Fl = F.fhl * w * w;
                         const float fx = Fl * dx, fy = Fl * dy;
-                        d.gx.add(rj[u].z * fx);
-                        d.gy.add(rj[u].z * fy);
-                        d.ev.add(rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy));
+                        ggx += rj[u].z * fx;
+                        ggy += rj[u].z * fy;
+                        gev += rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
                     }
                 }
                 if (r2 < F.h2) {
                     const float q = rr * F.ih, w = 1.f - q;
-                    d.rho.add(rj[u].z * (F.qh * (1.f + 3.f * q) * (w * w * w)));
+                    grho += rj[u].z * (F.qh * (1.f + 3.f * q) * (w * w * w));
                 }
             }
+        }
+        d.rho.add(grho);
+        d.rhol.add(grhol);
+        if (COH) {
+            d.gx.add(ggx);
+            d.gy.add(ggy);
+            d.ev.add(gev);
         }
     }
     // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
@@ -783,6 +794,7 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
             rj[u] = A.rf[t[u]];
             if (COH) vv[u] = A.vf[t[u]];
         }
+        float grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
@@ -791,15 +803,21 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
             if (k0 + u < c.y && r2 < F.cutl) {
                 const float rr = sqrtf(r2);
                 const float q = rr * F.ihl, w = 1.f - q;
-                d.rhol.add(rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w)));
+                grhol += rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w));
                 if (COH) {
                     const float Fl = F.fhl * w * w;
                     const float fx = Fl * dx, fy = Fl * dy;
-                    d.gx.add(rj[u].z * fx);
-                    d.gy.add(rj[u].z * fy);
-                    d.ev.add(rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy));
+                    ggx += rj[u].z * fx;
+                    ggy += rj[u].z * fy;
+                    gev += rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
                 }
             }
+        }
+        d.rhol.add(grhol);
+        if (COH) {
+            d.gx.add(ggx);
+            d.gy.add(ggy);
+            d.ev.add(gev);
         }
     }
 }
@@ -876,6 +894,7 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
             vv[u] = A.vf[t[u]];
             vt[u] = A.vt32[t[u]];
         }
+        float a00 = 0.f, a01 = 0.f, a10 = 0.f, a11 = 0.f, at0 = 0.f, at1 = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
@@ -889,14 +908,20 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
                 const float vj = vt[u].x;
                 const float dvx = vv[u].x - vi.x, dvy = vv[u].y - vi.y;
                 const float dT = vt[u].y - Ti;
-                g.g00.add(vj * dvx * gx);
-                g.g01.add(vj * dvx * gy);
-                g.g10.add(vj * dvy * gx);
-                g.g11.add(vj * dvy * gy);
-                g.gt0.add(vj * dT * gx);
-                g.gt1.add(vj * dT * gy);
+                a00 += vj * dvx * gx;
+                a01 += vj * dvx * gy;
+                a10 += vj * dvy * gx;
+                a11 += vj * dvy * gy;
+                at0 += vj * dT * gx;
+                at1 += vj * dT * gy;
             }
         }
+        g.g00.add(a00);
+        g.g01.add(a01);
+        g.g10.add(a10);
+        g.g11.add(a11);
+        g.gt0.add(at0);
+        g.gt1.add(at1);
     }
 }
 
@@ -964,6 +989,7 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
             Sj[u] = A.S32[t[u]];
             vq[u] = A.vq32[t[u]];
         }
+        float px = 0.f, py = 0.f, pu = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
@@ -980,11 +1006,14 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
                 const float cc = COND ? (vqi.z + vq[u].z) * wx + (vqi.w + vq[u].w) * wy : 0.f;
                 const float mj = rj[u].z;
                 const float e = 0.5f * ((vq[u].x - vqi.x) * fx + (vq[u].y - vqi.y) * fy);
-                o.ax.add(mj * fx);
-                o.ay.add(mj * fy);
-                o.ud.add(mj * (e + cc));
+                px += mj * fx;
+                py += mj * fy;
+                pu += mj * (e + cc);
             }
         }
+        o.ax.add(px);
+        o.ay.add(py);
+        o.ud.add(pu);
     }
 }
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from pyticles_amd import _lib, autotune, configs
-from pyticles_amd.device import from_system
+from pyticles_amd.device import DeviceBox, from_system
 from pyticles_amd.serve import Server, latency_summary
 
 pytestmark = pytest.mark.gpu
@@ -17,9 +17,12 @@ def alone(sysm, precision, unroll=0, skin=None):
     prm = dict(sysm.prm, precision=precision)
     if skin is not None:
         prm["skin"] = skin
-    s = configs.System(sysm.r, sysm.v, sysm.m, sysm.T, prm, sysm.steps, sysm.seed)
-    with from_system(s) as b:
+    # the schedule is set before the initial density pass (u = kB T - a rho):
+    # the FP32 form's partial sums follow the unroll factor
+    with DeviceBox(sysm.n, prm) as b:
         b.set_schedule(unroll)
+        b.upload(r=sysm.r, v=sysm.v, m=sysm.m, u=np.zeros(sysm.n))
+        b.init_thermal(sysm.T)
         b.step(sysm.steps)
         return b.download("r", "v", "u", "rho")
 
