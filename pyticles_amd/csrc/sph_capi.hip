@@ -102,7 +102,7 @@ static size_t arena_bytes(size_t cap, int prec, int lcap)
     b += cap * (4 * d2 + 4 * d1 + i1);          // r v vp acc, u m up udot, id
     b += cap * (4 * d2 + 4 * d1 + i1);          // sorted scratch of the same
     b += cap * (2 * i1 + d2 + sizeof(int2));    // pcell slot_box, r_ref, cnt
-    b += cap * (size_t)lcap * sizeof(unsigned); // nbr
+    b += (cap + 63) / 64 * 64 * (size_t)lcap * sizeof(unsigned);  // nbr, in tiles of 64 slots
     b += cap * (3 * d2 + 2 * d1 + 2 * sizeof(double4));  // rho Q jq, T e, S gradv
     b += cap * (2 * d2 + d1);                             // vt, aco, uco
     if (prec == SPH_FP32C) b += cap * (3 * sizeof(float4) + 2 * sizeof(float2));  // rf S32 vq32, vf vt32
@@ -218,7 +218,7 @@ struct Arena {
         rc |= dalloc(&u_s, cap); rc |= dalloc(&m_s, cap); rc |= dalloc(&up_s, cap); rc |= dalloc(&udot_s, cap);
         rc |= dalloc(&id_s, cap); rc |= dalloc(&pcell, cap); rc |= dalloc(&slot_box, cap);
         rc |= dalloc(&r_ref, cap); rc |= dalloc(&cnt, cap);
-        rc |= dalloc(&nbr, (size_t)cap * lcap);
+        rc |= dalloc(&nbr, (size_t)(cap + 63) / 64 * 64 * lcap);  // tiles of 64 slots
         rc |= dalloc(&rho, cap); rc |= dalloc(&Q, cap); rc |= dalloc(&jq, cap);
         rc |= dalloc(&T, cap); rc |= dalloc(&e, cap);
         rc |= dalloc(&S, cap); rc |= dalloc(&gradv, cap);
@@ -437,8 +437,7 @@ struct Arena {
         mark(T_INT, true);
         launch_integrate(A, f, st);
         mark(T_INT, false);
-        launch_box_energy(A, (int)boxes.size(), st);
-        launch_advance(A, (int)boxes.size(), st);
+        launch_box_step(A, (int)boxes.size(), st);
     }
 
     int launch_steps(int64_t count, bool with_build)

@@ -9,7 +9,8 @@
 //                    integer time from the last rates; every pass reads these
 //   last rates       acc (double2), udot
 //   sorted scratch   *_s copies written by the permutation of a build
-//   neighbour list   nbr[k * cap + s] (uint32 slot), cnt[s] = (short, long):
+//   neighbour list   tiles of 64 slots, entry k of slot s at
+//                    ((s/64)*lcap + k)*64 + s%64 (uint32 slot), cnt[s] = (short, long):
 //                    short entries (r < h + skin at build) at k = 0.., long
 //                    entries (h + skin <= r < r_cut + skin) at k = lcap-1 down
 //   derived          rho (double2: rho, rho_lr), T, S (double4: Sxx, Sxy,
@@ -86,7 +87,7 @@ struct ArenaView {
     int *pcell;          // cell of the slot at the last build
     double2 *r_ref;      // position at the last build (skin check)
     int *slot_box;       // box of the slot (-1 = free)
-    unsigned *nbr;       // neighbour lists, nbr[k * cap + s]
+    unsigned *nbr;       // neighbour lists in tiles of 64 slots (list_of)
     int2 *cnt;           // (short, long) entries per slot
     // derived fields
     double2 *rho;
@@ -117,7 +118,7 @@ struct ArenaView {
     int nseg;
     int n;               // slots in use (high-water mark)
     int nkeys;           // total keys
-    int cap;             // slot capacity (stride of nbr)
+    int cap;             // slot capacity
     int lcap;            // list entries per slot
     int pad;
 };

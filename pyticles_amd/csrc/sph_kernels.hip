@@ -42,6 +42,17 @@ __device__ __forceinline__ int xcd_block()
 
 __device__ __forceinline__ int wrapi(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
+// Neighbour lists in tiles of 64 slots (one wavefront): entry k of slot s
+// at ((s / 64) * lcap + k) * 64 + s % 64.  At list step k a wave reads 256
+// consecutive bytes, and everything one wave reads or writes lies in one
+// 256 * lcap byte tile (pages and L2 lines stay with the wave).
+static constexpr int LIST_TILE = 64;
+
+__device__ __forceinline__ const unsigned *list_of(const ArenaView &A, int s)
+{
+    return A.nbr + (((size_t)(s / LIST_TILE) * A.lcap) * LIST_TILE + (s % LIST_TILE));
+}
+
 // Minimum image exactly as neighbour_list.py:110-123 (all '>' tests, then '<').
 template <class Box>
 __device__ __forceinline__ void min_image(double &dx, double &dy, const Box &B)
@@ -234,7 +245,8 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
         A.rf[s] = fix_pos(ri, cell, mi, B);
         A.vf[s] = make_float2((float)vpi.x, (float)vpi.y);
     }
-    const int cap = A.cap, lcap = A.lcap;
+    const int lcap = A.lcap;
+    unsigned *nbw = A.nbr + (list_of(A, s) - A.nbr);
     int ns = 0, nl = 0;
     for_each_cell(B, A.cell_start, cell, [&](int t0, int t1, int, int) {
         for (int t = t0; t < t1; ++t) {
@@ -244,8 +256,8 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
             const double r2 = rsq_exact(dx, dy);
             if (r2 < B.rnl2 && t != s) {
                 if (ns + nl < lcap) {
-                    if (r2 < B.hsk2) A.nbr[(size_t)ns * cap + s] = (unsigned)t;
-                    else A.nbr[(size_t)(lcap - 1 - nl) * cap + s] = (unsigned)t;
+                    if (r2 < B.hsk2) nbw[(size_t)ns * LIST_TILE] = (unsigned)t;
+                    else nbw[(size_t)(lcap - 1 - nl) * LIST_TILE] = (unsigned)t;
                 }
                 if (r2 < B.hsk2) ++ns;
                 else ++nl;
@@ -351,8 +363,7 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
 {
     const double2 ri = A.r[s], vi = A.vp[s];
     const int2 c = A.cnt[s];
-    const size_t cap = A.cap;
-    const unsigned *nb = A.nbr + s;
+    const unsigned *nb = list_of(A, s);
     Dens64 d;
     d.rho = K.qh;
     d.rhol = K.qhl;
@@ -364,7 +375,7 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
         double2 rj[U64], vv[U64];
         double mj[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -404,7 +415,7 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
         double2 rj[U64], vv[U64];
         double mj[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * LIST_TILE] : s;
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -495,15 +506,14 @@ __device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K,
     const double2 ri = A.r[s], vi = A.vp[s];
     const double Ti = A.T[s];
     const int ns = A.cnt[s].x;
-    const size_t cap = A.cap;
-    const unsigned *nb = A.nbr + s;
+    const unsigned *nb = list_of(A, s);
     Grad64 g;
     g.g00 = g.g01 = g.g10 = g.g11 = g.gt0 = g.gt1 = 0.0;
     for (int k0 = 0; k0 < ns; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64], vt[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -585,8 +595,7 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
     const double2 ri = A.r[s], vi = A.vp[s];
     const double4 Si = A.S[s];
     const double2 Qi = COND ? A.Q[s] : make_double2(0.0, 0.0);
-    const size_t cap = A.cap;
-    const unsigned *nb = A.nbr + s;
+    const unsigned *nb = list_of(A, s);
     Rate64 o;
     o.ax = o.ay = o.ud = 0.0;
     for (int k0 = 0; k0 < ns; k0 += U64) {
@@ -594,7 +603,7 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
         double2 rj[U64], vv[U64], Qj[U64];
         double4 Sj[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -735,14 +744,13 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
     const float4 ri = A.rf[s];
     const float2 vi = A.vf[s];
     const int2 c = A.cnt[s];
-    const size_t cap = A.cap;
-    const unsigned *nb = A.nbr + s;
+    const unsigned *nb = list_of(A, s);
     for (int k0 = 0; k0 < c.x; k0 += U32) {
         int t[U32];
         float4 rj[U32];
         float2 vv[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -788,7 +796,7 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
         float4 rj[U32];
         float2 vv[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * cap] : s;
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * LIST_TILE] : s;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -880,14 +888,13 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
     const float2 vi = A.vf[s];
     const float Ti = A.vt32[s].y;
     const int ns = A.cnt[s].x;
-    const size_t cap = A.cap;
-    const unsigned *nb = A.nbr + s;
+    const unsigned *nb = list_of(A, s);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32];
         float2 vv[U32], vt[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -976,13 +983,12 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
     const float4 ri = A.rf[s];
     const float4 vqi = A.vq32[s];
     const float4 Si = A.S32[s];
-    const size_t cap = A.cap;
-    const unsigned *nb = A.nbr + s;
+    const unsigned *nb = list_of(A, s);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32], Sj[U32], vq[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * cap] : s;
+        for (int u = 0; u < U32; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -1099,28 +1105,6 @@ __global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
 // per-box bookkeeping
 // ===========================================================================
 
-__global__ __launch_bounds__(NT) void k_box_energy(ArenaView A, int nbox)
-{
-    const int b = blockIdx.x;
-    if (b >= nbox) return;
-    BoxDyn &D = A.dyn[b];
-    if (D.state != BOX_RUN || !(D.k == 0 || D.k == D.nsteps)) return;
-    const BoxDesc &B = A.boxes[b];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < B.n; i += NT) acc += A.e[B.part_off + i];
-    __shared__ double red[NT];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = NT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        if (D.k == 0) D.E0 = red[0];
-        else D.E1 = red[0];
-    }
-}
-
 // Steps a box may reuse the list it just built: the largest c with
 //   c dt vmax + (c dt)^2 amax <= 0.4 skin    (twice the kinematic bound on
 // the acceleration term, 80 % of the half skin), plus the build step.
@@ -1138,28 +1122,61 @@ __device__ __forceinline__ int64_t reuse_steps(const BoxDesc &B, const BoxDyn &D
     return K < 1 ? 1 : (K > 64 ? 64 : K);
 }
 
+// Per-box step bookkeeping, one block per box.  A running box at k = 0 or
+// k = nsteps first sums its slots' energies m (v^2/2 + u) into E0 / E1
+// (fixed order: four strided partial sums per thread, then a tree).  Then
 // INIT -> RUN (k = 0); RUN: k++ until k == nsteps, then DONE.  Boxes that
 // built this step get their next build step from their own speed and
 // acceleration, so a box's schedule never depends on the other boxes.
-__global__ void k_advance(ArenaView A, int nbox)
+static constexpr int NT_BOX = 1024;
+
+__global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x;
     if (b >= nbox) return;
     BoxDyn &D = A.dyn[b];
-    const bool built = building(D, 0);
-    if (built) {
-        const int64_t K = reuse_steps(A.boxes[b], D);
-        D.next_rebuild = (D.state == BOX_INIT ? 0 : D.k) + K;
+    const int state = D.state;
+    const int64_t k = D.k, K = D.nsteps;
+    const bool capture = state == BOX_RUN && (k == 0 || k == K);
+    __shared__ double red[NT_BOX];
+    if (capture) {
+        const BoxDesc &B = A.boxes[b];
+        const double *e = A.e + B.part_off;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int i = threadIdx.x;
+        for (; i + 3 * NT_BOX < B.n; i += 4 * NT_BOX) {
+            a0 += e[i];
+            a1 += e[i + NT_BOX];
+            a2 += e[i + 2 * NT_BOX];
+            a3 += e[i + 3 * NT_BOX];
+        }
+        for (; i < B.n; i += NT_BOX) a0 += e[i];
+        red[threadIdx.x] = (a0 + a1) + (a2 + a3);
+        __syncthreads();
+        for (int off = NT_BOX / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+    }
+    __syncthreads();  // every thread has read D before thread 0 updates it
+    if (threadIdx.x != 0) return;
+    if (capture) {
+        if (k == 0) D.E0 = red[0];
+        else D.E1 = red[0];
+    }
+    if (building(D, 0)) {
+        const int64_t nK = reuse_steps(A.boxes[b], D);
+        D.next_rebuild = (state == BOX_INIT ? 0 : k) + nK;
         D.builds += 1;
         D.vmax2 = 0ull;
         D.amax2 = 0ull;
     }
-    if (D.state == BOX_INIT) {
-        D.state = D.nsteps > 0 ? BOX_RUN : BOX_DONE;
+    if (state == BOX_INIT) {
+        D.state = K > 0 ? BOX_RUN : BOX_DONE;
         D.k = 0;
-    } else if (D.state == BOX_RUN) {
-        if (D.k >= D.nsteps) D.state = BOX_DONE;
-        else D.k += 1;
+    } else if (state == BOX_RUN) {
+        if (k >= K) D.state = BOX_DONE;
+        else D.k = k + 1;
     }
 }
 
@@ -1466,13 +1483,9 @@ void launch_integrate(const ArenaView &A, int fp32, hipStream_t st)
 {
     hipLaunchKernelGGL(k_integrate, grid_for(A.n), dim3(NT), 0, st, A, fp32);
 }
-void launch_box_energy(const ArenaView &A, int nbox, hipStream_t st)
+void launch_box_step(const ArenaView &A, int nbox, hipStream_t st)
 {
-    if (nbox > 0) hipLaunchKernelGGL(k_box_energy, dim3(nbox), dim3(NT), 0, st, A, nbox);
-}
-void launch_advance(const ArenaView &A, int nbox, hipStream_t st)
-{
-    if (nbox > 0) hipLaunchKernelGGL(k_advance, grid_for(nbox), dim3(NT), 0, st, A, nbox);
+    if (nbox > 0) hipLaunchKernelGGL(k_box_step, dim3(nbox), dim3(NT_BOX), 0, st, A, nbox);
 }
 void launch_upload(const ArenaView &A, int off, int n, int field, const double *src, hipStream_t st)
 {

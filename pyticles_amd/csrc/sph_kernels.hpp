@@ -19,8 +19,7 @@ void launch_density(const ArenaView &A, int fp32, int state_mask, int unroll, hi
 void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, int unroll, hipStream_t st);
 void launch_rates(const ArenaView &A, int fp32, int state_mask, int unroll, hipStream_t st);
 void launch_integrate(const ArenaView &A, int fp32, hipStream_t st);
-void launch_box_energy(const ArenaView &A, int nbox, hipStream_t st);
-void launch_advance(const ArenaView &A, int nbox, hipStream_t st);
+void launch_box_step(const ArenaView &A, int nbox, hipStream_t st);
 void launch_upload(const ArenaView &A, int off, int n, int field, const double *src, hipStream_t st);
 void launch_init_slots(const ArenaView &A, int off, int n, hipStream_t st);
 void launch_download(const ArenaView &A, int box, int n, int field, double *dst, hipStream_t st);
