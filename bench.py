@@ -34,19 +34,22 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-# algorithmic HBM bytes per particle-step of each pass (fixed accounting:
-# every per-particle array a pass must read or write counted once at its
-# storage width; neighbour re-reads not counted; sort/permute counted).  The
-# rates pass reads r, v, m, rho, rho_lr, u, grad v, J_q; its leapfrog writes
-# (r', v', u', 40 B) are counted on the kick/drift kernel.
+# Algorithmic HBM bytes per particle of each kernel (fixed accounting: every
+# per-particle array a logical pass must read or write counted once at its
+# storage width; neighbour re-reads and list words not counted; sort/permute
+# counted).  Per logical pass: build 124 (FP32C 116), density r, m -> rho,
+# rho_lr 40 (28), gradients 104 (72), rates 112 (76), leapfrog writes 40.
+# The density kernel also computes the cohesive part of the rates, so it
+# takes over the rates' velocity read (16 B, FP32C 8 B).
 B_PS = {
-    ("fp64", True): dict(build=124, density=40, gradients=104, rates=112, integrate=40),
-    ("fp64", False): dict(build=124, density=40, gradients=0, rates=64, integrate=40),
-    ("fp32c", True): dict(build=116, density=28, gradients=72, rates=76, integrate=40),
-    ("fp32c", False): dict(build=116, density=28, gradients=0, rates=52, integrate=40),
+    ("fp64", True): dict(build=124, density=56, gradients=104, rates=96, integrate=40),
+    ("fp64", False): dict(build=124, density=56, gradients=0, rates=48, integrate=40),
+    ("fp32c", True): dict(build=116, density=36, gradients=72, rates=68, integrate=40),
+    ("fp32c", False): dict(build=116, density=36, gradients=0, rates=44, integrate=40),
 }
+KERNEL = {"fp64": dict(density="k_density64", gradients="k_grad64", rates="k_rates64"),
+          "fp32c": dict(density="k_density32", gradients="k_grad32", rates="k_rates32")}
 HBM_PEAK_GBS = 8000.0
-RATES_KERNEL = {"fp64": "k_rates64", "fp32c": "k_rates32"}
 
 
 def workload(name, precision):
@@ -74,11 +77,37 @@ def schedule(s, cls):
     return sc["unroll"], sc
 
 
+def device_box(s, unroll):
+    """A box holding s with the launch schedule set before its first pass."""
+    from pyticles_amd.device import DeviceBox
+    b = DeviceBox(s.n, s.prm)
+    b.set_schedule(unroll)
+    b.upload(r=s.r, v=s.v, m=s.m, u=np.zeros(s.n))
+    b.init_thermal(s.T)
+    return b
+
+
+def pcie_inclusive(s, steps, unroll):
+    """Particle-steps/s of a whole job with host buffers: upload of r, v, m,
+    the initial density pass, the steps and the download of r, v, u (device
+    allocation excluded)."""
+    from pyticles_amd.device import DeviceBox
+    b = DeviceBox(s.n, s.prm)
+    b.set_schedule(unroll)
+    t0 = time.perf_counter()
+    b.upload(r=s.r, v=s.v, m=s.m, u=np.zeros(s.n))
+    b.init_thermal(s.T)
+    b.step(steps)
+    b.download("r", "v", "u")
+    el = time.perf_counter() - t0
+    b.close()
+    return {"value": s.n * steps / el, "unit": "particle-steps/s", "seconds": el,
+            "includes": "host->device r, v, m; initial density pass; %d steps; device->host r, v, u" % steps}
+
+
 def run_gpu(s, steps, warmup, timing=True, unroll=0):
     from pyticles_amd import _lib
-    from pyticles_amd.device import from_system
-    b = from_system(s)
-    b.set_schedule(unroll)
+    b = device_box(s, unroll)
     if warmup:
         b.step(warmup)
     _lib.check(_lib.lib.sph_ctx_set_timing(b.ctx, 1 if timing else 0))
@@ -116,13 +145,11 @@ def error_vs_oracle(s, r=None, v=None, u=None, unroll=0):
     FP64 oracle, at the initial state of s or at the state (r, v, u):
     max|x - x_ref| / max|x_ref| per field, and the RMS ratio as <field>_rms."""
     from oracle import oracle as O
-    from pyticles_amd.device import from_system
     r = s.r if r is None else r
     v = s.v if v is None else v
     u = O.init_u(s.r, s.m, s.T, s.prm) if u is None else u
     ref = O.derivatives(r, v, s.m, u, s.prm, terms=s.prm["terms"])
-    with from_system(s) as b:
-        b.set_schedule(unroll)
+    with device_box(s, unroll) as b:
         b.upload(r=r, v=v, u=u)
         b.rates(s.prm["terms"])
         got = b.download("rho", "rho_lr", "acc", "udot")
@@ -216,7 +243,6 @@ def leg_c4(nbox=512, plain_every=4):
     """Batched FP32C (and FP64) arena over all boxes vs the plain FP64 form,
     one box per launch sequence, on every plain_every-th box."""
     from pyticles_amd import _lib, configs
-    from pyticles_amd.device import from_system
     boxes = [configs.c4_box(b) for b in range(nbox)]
     steps = boxes[0].steps
     ps = sum(b.n for b in boxes) * steps
@@ -236,7 +262,8 @@ def leg_c4(nbox=512, plain_every=4):
     log("c4 plain fp64, %d boxes one at a time" % len(sample))
     for i in sample:
         sb = configs.c4_box(i, precision=_lib.FP64)
-        b = from_system(sb)
+        un, _ = schedule(sb, "box")  # the tuned single-box schedule
+        b = device_box(sb, un)
         b.step(1)
         t = time.perf_counter()
         b.step(steps)
@@ -245,7 +272,8 @@ def leg_c4(nbox=512, plain_every=4):
         b.close()
     out["plain_fp64_one_box_per_launch"] = {
         "value": pss / el, "unit": "particle-steps/s", "seconds": el,
-        "sample": "boxes %s (every %dth of %d), %d steps each, timed per box excluding set-up" % (
+        "sample": "boxes %s (every %dth of %d), %d steps each, tuned single-box schedule, timed per box "
+                  "excluding set-up" % (
             "0,%d,...,%d" % (plain_every, sample[-1]), plain_every, nbox, steps)}
     out["batched_fp32c_over_plain_fp64"] = out["batched_fp32c"]["value"] / out["plain_fp64_one_box_per_launch"]["value"]
     out["batched_fp64_over_plain_fp64"] = out["batched_fp64"]["value"] / out["plain_fp64_one_box_per_launch"]["value"]
@@ -297,14 +325,17 @@ def tune_all(path):
     return res
 
 
-def load_traffic(key):
+def load_traffic(key, kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (profiles/traffic.json: {workload_precision: {kernel: bytes}})."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
     try:
-        return json.load(open(p)).get(key)
-    except Exception:
+        entry = json.load(open(p)).get(key)
+    except (OSError, ValueError):
         return None
+    return entry.get(kernel) if isinstance(entry, dict) else None
 
 
 def main():
@@ -360,14 +391,16 @@ def main():
     value = s.n * args.steps / el
     evals = args.steps + 1
     per = {k: tm[k + "_ms"] / evals for k in ("build", "density", "gradients", "rates", "integrate")}
-    rates_s = per["rates"] * 1e-3
-    alg_bytes = bps["rates"] * s.n
-    achieved = alg_bytes / rates_s / 1e9
-    traffic = load_traffic("%s_%s" % (args.workload, args.precision))
+    # the dominant pair-pass kernel by measured time
+    dom = max(("density", "gradients", "rates"), key=lambda k: per[k])
+    alg_bytes = bps[dom] * s.n
+    achieved = alg_bytes / (per[dom] * 1e-3) / 1e9
+    kname = KERNEL[args.precision][dom]
+    traffic = load_traffic("%s_%s" % (args.workload, args.precision), kname)
     roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,
-                kernel=RATES_KERNEL[args.precision], kernel_avg_ms=round(per["rates"], 5),
-                alg_bytes_per_launch=alg_bytes)
+                kernel=kname, kernel_avg_ms=round(per[dom], 5), alg_bytes_per_launch=alg_bytes,
+                alg_bytes_per_particle=bps[dom])
     step_bytes = sum(bps.values())
     line = {
         "metric": "particle-steps/s",
@@ -391,6 +424,7 @@ def main():
         "passes_ms_per_eval": {k: round(v, 5) for k, v in per.items()},
         "step_roofline_frac": round(value * step_bytes / (HBM_PEAK_GBS * 1e9), 5),
         "energy_drift_rel": abs(E1 - E0) / abs(E0),
+        "pcie_inclusive": pcie_inclusive(s, args.steps, unroll),
     }
     legs = [] if args.no_extra else [x for x in args.legs.split(",") if x]
     is_head = args.workload == "c2" and args.precision == "fp64"
