@@ -150,6 +150,34 @@ int sph_energy(sph_box *box, double *ke, double *ue, double *etot);
  * Mirrors the p.timing dictionary (particles.py:164-169,359). */
 int sph_box_timing(sph_box *box, double *out, int n, int reset);
 
+/* ---- rendering ------------------------------------------------------------ */
+/* SPH interpolation of a particle field onto grid points (x[i], y[j]):
+ * Z[i][j] = sum_p a_p m_p / rho_p W(|dr|, h_p) over the particles with
+ * |dr| < cutoff, dr = (x_p - x[i], y[j] - y_p) under the minimum image with
+ * periods (bounds[1], bounds[3]) (r[n][2]; bounds = xmin, xmax, ymin, ymax).
+ * Replaces interpolate.splash_map_2d (interpolate.py:39-88) as a per-pixel
+ * gather over a cell list (no atomics); equal to the reference's
+ * per-particle scatter when cutoff >= max h + pixel width, since its
+ * half-open pixel ranges then drop only pixels of zero weight. */
+int sph_splash_map_2d(sph_ctx *ctx, int64_t n, const double *r, const double *m, const double *rho,
+                      const double *a, const double *h, int nx, const double *x, int ny, const double *y,
+                      const double *bounds, double cutoff, double *Z);
+
+/* ---- trajectory snapshots ----------------------------------------------- */
+/* Pinned host memory (hipHostMalloc) for asynchronous snapshots. */
+int sph_host_alloc(int64_t bytes, void **out);
+int sph_host_free(void *p);
+/* Start an asynchronous snapshot of r[n][2], v[n][2], u[n], m[n] in the
+ * caller's particle order (any pointer may be NULL): the state is copied on
+ * the device in stream order, then to the host buffers on a copy stream,
+ * so steps enqueued afterwards overlap the transfer.  Buffers from
+ * sph_host_alloc make the copy asynchronous.  A snapshot still in flight is
+ * waited for first.  Replaces the state gather of spam_nc.write_step
+ * (spam_nc.py:121-145). */
+int sph_box_snapshot(sph_box *box, double *r, double *v, double *u, double *m);
+/* Wait until the last snapshot has landed in its host buffers. */
+int sph_box_snapshot_wait(sph_box *box);
+
 /* ---- batching and serving ----------------------------------------------- */
 /* An arena holds many independent boxes in one device-resident particle
  * pool; one step launch advances every running box by one step.  No

@@ -35,6 +35,9 @@
  *                                by shear-flow and hot-spot physics
  *   orc_run                      leapfrog defined here (the reference has none);
  *                                checked by energy drift and step-size order
+ *   orc_splash_map_2d            pinned: lattice rendering at particle sites is
+ *                                exactly 1 (sum_j W / rho = rho / rho), linear
+ *                                in a, brute-force pixel sums on tiny cases
  */
 #include <math.h>
 #include <stdint.h>
@@ -614,4 +617,47 @@ int orc_num_threads(void)
 #else
     return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* SPH scatter to a grid: interpolate.splash_map_2d (interpolate.py:39-88),  */
+/* restated loop by loop.  Every particle adds a_i m_i / rho_i W(s, h_i) to  */
+/* the pixels of its half-open pixel ranges [floor((r - c - min)/res),       */
+/* floor((r + c - min)/res)) (wrapped into the grid) whose pixel coordinate  */
+/* lies within the cutoff under the minimum image with periods (xmax, ymax); */
+/* Z is [nx][ny] (Z[xpx][ypx]).                                             */
+void orc_splash_map_2d(int64_t n, const double *r, const double *m, const double *rho,
+                       const double *a, const double *h, int nx, const double *x, int ny,
+                       const double *y, const double *bounds, double cutoff, double *Z)
+{
+    const double xmin = bounds[0], xmax = bounds[1], ymin = bounds[2], ymax = bounds[3];
+    const double res = x[1] - x[0];
+    memset(Z, 0, sizeof(double) * (size_t)nx * ny);
+    for (int64_t i = 0; i < n; ++i) {
+        const int xlo = (int)floor((r[2 * i] - cutoff - xmin) / res);
+        const int ylo = (int)floor((r[2 * i + 1] - cutoff - ymin) / res);
+        const int xhi = (int)floor((r[2 * i] + cutoff - xmin) / res);
+        const int yhi = (int)floor((r[2 * i + 1] + cutoff - ymin) / res);
+        for (int yp0 = ylo; yp0 < yhi; ++yp0) {
+            for (int xp0 = xlo; xp0 < xhi; ++xp0) {
+                int yp = yp0, xp = xp0;
+                if (yp >= ny) yp -= ny;
+                if (yp < 0) yp += ny;
+                if (xp >= nx) xp -= nx;
+                if (xp < 0) xp += nx;
+                double dx = r[2 * i] - x[xp], dy = y[yp] - r[2 * i + 1];
+                orc_min_image(&dx, &dy, xmax, ymax);
+                const double rsq = pair_rsq(dx, dy);
+                if (rsq < cutoff * cutoff) {
+                    const double s = sqrt(rsq);
+                    double w = 0.;
+                    if (s < h[i]) {  /* spkernel.lucy_w, 2-D (spkernel.py:54-63) */
+                        const double t = 1. - s / h[i];
+                        w = lucy_norm(h[i], 2) * (1. + 3. * s / h[i]) * (t * t * t);
+                    }
+                    Z[(size_t)xp * ny + yp] += a[i] * m[i] / rho[i] * w;
+                }
+            }
+        }
+    }
 }

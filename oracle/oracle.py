@@ -51,6 +51,8 @@ def lib():
             "orc_run": (ctypes.c_int, [i64, D, D, D, D, P, ctypes.c_uint, i64, D]),
             "orc_init_u": (None, [i64, D, D, D, P, D]),
             "orc_num_threads": (ctypes.c_int, []),
+            "orc_splash_map_2d": (None, [i64, D, D, D, D, D, ctypes.c_int, D, ctypes.c_int, D, D,
+                                         ctypes.c_double, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -184,3 +186,15 @@ def run(r, v, m, u, prm, nsteps, terms=ALL_TERMS, energy=False):
     if rc != 0:
         raise MemoryError("oracle allocation failed")
     return (r, v, u, E) if energy else (r, v, u)
+
+
+def splash_map_2d(x, y, r, m, rho, a, h, bounds, cutoff):
+    """interpolate.splash_map_2d restated (Z[nx, ny])."""
+    r, m, rho, a = _c(r), _c(m), _c(rho), _c(a)
+    n = r.shape[0]
+    h = _c(np.broadcast_to(np.asarray(h, dtype=np.float64), (n,)))
+    x, y = _c(x), _c(y)
+    Z = np.empty((x.size, y.size))
+    lib().orc_splash_map_2d(n, _d(r), _d(m), _d(rho), _d(a), _d(h), x.size, _d(x), y.size, _d(y),
+                            _d(_c(bounds)), float(cutoff), _d(Z))
+    return Z

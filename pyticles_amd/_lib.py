@@ -91,6 +91,12 @@ _SIGS = {
     "sph_arena_set_schedule": [_P, ctypes.c_int],
     "sph_box_set_schedule": [_P, ctypes.c_int],
     "sph_arena_footprint": [_I64, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)],
+    "sph_host_alloc": [_I64, ctypes.POINTER(_P)],
+    "sph_host_free": [_P],
+    "sph_box_snapshot": [_P, _D, _D, _D, _D],
+    "sph_box_snapshot_wait": [_P],
+    "sph_splash_map_2d": [_P, _I64, _D, _D, _D, _D, _D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_double,
+                          _D],
 }
 _RESTYPES = {"sph_last_error": ctypes.c_char_p, "sph_version": ctypes.c_char_p}
 

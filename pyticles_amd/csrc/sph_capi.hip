@@ -157,6 +157,14 @@ struct Arena {
     // staging
     double *d_stage = nullptr;
     size_t stage_bytes = 0;
+
+    // asynchronous snapshots: state copied on the device in stream order,
+    // then to the host on a copy stream
+    double *d_snap = nullptr;
+    size_t snap_cap = 0;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_snap_ready = nullptr, ev_snap_done = nullptr;
+    bool snap_pending = false;
     double *d_e3 = nullptr;
 
     // pair export buffers
@@ -261,6 +269,15 @@ struct Arena {
         dfree(keys_in); dfree(k_out); dfree(v_out); dfree(k_alt); dfree(v_alt);
         if (sort_tmp) (void)hipFree(sort_tmp);
         sort_tmp = nullptr;
+        if (snap_pending) (void)hipEventSynchronize(ev_snap_done);
+        snap_pending = false;
+        if (ev_snap_ready) (void)hipEventDestroy(ev_snap_ready);
+        if (ev_snap_done) (void)hipEventDestroy(ev_snap_done);
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
+        ev_snap_ready = ev_snap_done = nullptr;
+        copy_stream = nullptr;
+        dfree(d_snap);
+        snap_cap = 0;
         dfree(cell_start); dfree(d_boxes); dfree(d_dyn); dfree(d_segs); dfree(d_stage); dfree(d_e3);
         dfree(d_inv); dfree(d_cnt); dfree(d_off); dfree(d_grand); dfree(d_jbuf); dfree(d_iap);
         if (scan_tmp) (void)hipFree(scan_tmp);
@@ -1249,6 +1266,142 @@ int sph_box_set_graph(sph_box *box, int on)
     box->ar.use_graph = on ? 1 : 0;
     box->ar.invalidate_graph();
     return SPH_OK;
+}
+
+int sph_host_alloc(int64_t bytes, void **out)
+{
+    if (!out || bytes < 0) return fail(SPH_ERR_ARG, "invalid argument");
+    *out = nullptr;
+    if (bytes == 0) return SPH_OK;
+    if (hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return fail(SPH_ERR_NOMEM, "hipHostMalloc failed");
+    }
+    return SPH_OK;
+}
+
+int sph_host_free(void *p)
+{
+    if (p) HIPCHK(hipHostFree(p));
+    return SPH_OK;
+}
+
+int sph_box_snapshot_wait(sph_box *box)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    Arena &A = box->ar;
+    if (A.snap_pending) {
+        HIPCHK(hipEventSynchronize(A.ev_snap_done));
+        A.snap_pending = false;
+    }
+    return SPH_OK;
+}
+
+int sph_box_snapshot(sph_box *box, double *r, double *v, double *u, double *m)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    Arena &A = box->ar;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    if (int rc = sph_box_snapshot_wait(box)) return rc;  // the staging buffer is reused
+    const int b = box->b, n = A.boxes[b].n;
+    if (!A.copy_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&A.copy_stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&A.ev_snap_ready, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&A.ev_snap_done, hipEventDisableTiming));
+    }
+    const size_t need = (size_t)6 * n;
+    if (need > A.snap_cap) {
+        dfree(A.d_snap);
+        A.snap_cap = 0;
+        if (dalloc(&A.d_snap, need)) return SPH_ERR_NOMEM;
+        A.snap_cap = need;
+    }
+    struct F { double *dst; int field, nc; size_t off; } fs[] = {
+        {r, 0, 2, 0}, {v, 1, 2, (size_t)2 * n}, {u, 3, 1, (size_t)4 * n}, {m, 2, 1, (size_t)5 * n}};
+    ArenaView V = A.view();
+    for (auto &f : fs)
+        if (f.dst) launch_download(V, b, n, f.field, A.d_snap + f.off, A.ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(A.ev_snap_ready, A.ctx->stream));
+    HIPCHK(hipStreamWaitEvent(A.copy_stream, A.ev_snap_ready, 0));
+    for (auto &f : fs)
+        if (f.dst)
+            HIPCHK(hipMemcpyAsync(f.dst, A.d_snap + f.off, sizeof(double) * f.nc * (size_t)n,
+                                  hipMemcpyDeviceToHost, A.copy_stream));
+    HIPCHK(hipEventRecord(A.ev_snap_done, A.copy_stream));
+    A.snap_pending = true;
+    return SPH_OK;
+}
+
+int sph_splash_map_2d(sph_ctx *ctx, int64_t n, const double *r, const double *m, const double *rho,
+                      const double *a, const double *h, int nx, const double *x, int ny, const double *y,
+                      const double *bounds, double cutoff, double *Z)
+{
+    if (!ctx || n < 0 || nx < 1 || ny < 1 || !bounds || !x || !y || !Z || !(cutoff > 0) ||
+        (n > 0 && (!r || !m || !rho || !a || !h)))
+        return fail(SPH_ERR_ARG, "invalid argument");
+    if (n > (1ll << 30) || (long long)nx * ny > (1ll << 31))
+        return fail(SPH_ERR_ARG, "too many particles or pixels");
+    const double Lx = bounds[1], Ly = bounds[3];  // periods as interpolate.py:79 uses them
+    if (!(Lx > 0) || !(Ly > 0)) return fail(SPH_ERR_ARG, "bounds need xmax, ymax > 0");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    SplashGrid G;
+    G.Lx = Lx; G.Ly = Ly; G.cutoff = cutoff;
+    G.ncx = (int)std::floor(Lx / (cutoff * (1.0 + 1e-9)));
+    G.ncy = (int)std::floor(Ly / (cutoff * (1.0 + 1e-9)));
+    G.ncx = G.ncx < 3 ? 1 : std::min(G.ncx, 4096);
+    G.ncy = G.ncy < 3 ? 1 : std::min(G.ncy, 4096);
+    G.cwx = Lx / G.ncx; G.cwy = Ly / G.ncy;
+    const int ncell = G.ncx * G.ncy;
+    const size_t np = (size_t)nx * ny;
+    double *d_in = nullptr, *d_xy = nullptr, *d_Z = nullptr;
+    double4 *d_rec = nullptr, *d_rec2 = nullptr;
+    unsigned *d_keys = nullptr;
+    int *d_cs = nullptr;
+    void *d_tmp = nullptr;
+    const size_t N = (size_t)std::max<int64_t>(n, 1);
+    const size_t tmp_bytes = radix_sort_temp_bytes((int)N);
+    int rc = 0;
+    rc |= dalloc(&d_in, 6 * N);  // r (2n), m, rho, a, h
+    rc |= dalloc(&d_xy, (size_t)nx + ny);
+    rc |= dalloc(&d_Z, np);
+    rc |= dalloc(&d_rec, N);
+    rc |= dalloc(&d_rec2, N);
+    rc |= dalloc(&d_keys, 5 * N);
+    rc |= dalloc(&d_cs, (size_t)ncell + 1);
+    rc |= dalloc((char **)&d_tmp, tmp_bytes);
+    if (!rc) {
+        const size_t b = sizeof(double) * n;
+        if (n > 0) {
+            if (hipMemcpyAsync(d_in, r, 2 * b, hipMemcpyHostToDevice, st) ||
+                hipMemcpyAsync(d_in + 2 * n, m, b, hipMemcpyHostToDevice, st) ||
+                hipMemcpyAsync(d_in + 3 * n, rho, b, hipMemcpyHostToDevice, st) ||
+                hipMemcpyAsync(d_in + 4 * n, a, b, hipMemcpyHostToDevice, st) ||
+                hipMemcpyAsync(d_in + 5 * n, h, b, hipMemcpyHostToDevice, st))
+                rc = fail(SPH_ERR_HIP, "hipMemcpyAsync failed");
+        }
+        if (!rc && (hipMemcpyAsync(d_xy, x, sizeof(double) * nx, hipMemcpyHostToDevice, st) ||
+                    hipMemcpyAsync(d_xy + nx, y, sizeof(double) * ny, hipMemcpyHostToDevice, st)))
+            rc = fail(SPH_ERR_HIP, "hipMemcpyAsync failed");
+        if (!rc) {
+            unsigned *k_in = d_keys, *k_out = d_keys + N, *v_out = d_keys + 2 * N, *k_alt = d_keys + 3 * N,
+                     *v_alt = d_keys + 4 * N;
+            launch_splash(n, d_in, d_in + 2 * n, d_in + 3 * n, d_in + 4 * n, d_in + 5 * n, G, d_rec, d_rec2, k_in,
+                          k_out, v_out, k_alt, v_alt, d_tmp, bits_for(ncell), d_cs, nx, ny, d_xy, d_xy + nx, d_Z,
+                          st);
+            if (hipGetLastError() != hipSuccess) rc = fail(SPH_ERR_HIP, "splash launch failed");
+            if (!rc && hipMemcpyAsync(Z, d_Z, sizeof(double) * np, hipMemcpyDeviceToHost, st))
+                rc = fail(SPH_ERR_HIP, "hipMemcpyAsync failed");
+            if (!rc && hipStreamSynchronize(st)) rc = fail(SPH_ERR_HIP, "hipStreamSynchronize failed");
+        }
+    } else {
+        rc = fail(SPH_ERR_NOMEM, "device allocation failed");
+    }
+    (void)hipStreamSynchronize(st);
+    dfree(d_in); dfree(d_xy); dfree(d_Z); dfree(d_rec); dfree(d_rec2); dfree(d_keys); dfree(d_cs);
+    if (d_tmp) (void)hipFree(d_tmp);
+    return rc;
 }
 
 static int set_schedule(Arena &A, int unroll)

@@ -123,4 +123,10 @@ struct ArenaView {
     int pad;
 };
 
+// Grid of the splash rendering: particle cells at least one cutoff wide.
+struct SplashGrid {
+    double Lx, Ly, cwx, cwy, cutoff;
+    int ncx, ncy;
+};
+
 }  // namespace sph

@@ -20,10 +20,12 @@
 #include <math.h>
 #include "sph_common.hpp"
 #include "sph_kernels.hpp"
+#include "sph_sort.hpp"
 
 namespace sph {
 
 static constexpr int NT = 256;  // work-items per block (4 waves)
+static constexpr double SPH_PI = 3.14159265358979323846;  // math.pi (spkernel.py:15)
 // The pair passes are templates on the number of list entries whose loads
 // are in flight per lane (U64 / U32 = 1, 2 or 4); the schedule picks one at
 // launch time.  Defaults: 2 for FP64, whose wider registers cut occupancy,
@@ -1431,6 +1433,100 @@ __global__ __launch_bounds__(NT) void k_energy_now(ArenaView A, int box, double 
 }
 
 // ===========================================================================
+// SPH scatter to a grid (interpolate.splash_map_2d) as a per-pixel gather
+// ===========================================================================
+
+// Particle record (x, y wrapped into the periods, w = a m / rho, h) and its
+// cell key.
+__global__ __launch_bounds__(NT) void k_splash_prep(long long n, const double *__restrict__ r,
+                                                     const double *__restrict__ m,
+                                                     const double *__restrict__ rho,
+                                                     const double *__restrict__ a,
+                                                     const double *__restrict__ h, SplashGrid G,
+                                                     double4 *__restrict__ rec, unsigned *__restrict__ keys)
+{
+    const long long i = (long long)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    double x = r[2 * i], y = r[2 * i + 1];
+    x -= G.Lx * floor(x / G.Lx);
+    y -= G.Ly * floor(y / G.Ly);
+    int cx = (int)(x / G.cwx), cy = (int)(y / G.cwy);
+    cx = cx < 0 ? 0 : (cx >= G.ncx ? G.ncx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= G.ncy ? G.ncy - 1 : cy);
+    rec[i] = make_double4(r[2 * i], r[2 * i + 1], a[i] * m[i] / rho[i], h[i]);
+    keys[i] = (unsigned)(cy * G.ncx + cx);
+}
+
+__global__ __launch_bounds__(NT) void k_splash_gather(long long n, const unsigned *__restrict__ src,
+                                                       const double4 *__restrict__ rec,
+                                                       double4 *__restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    out[i] = rec[src[i]];
+}
+
+// cs[c] = first sorted particle with key >= c, c in [0, ncell]
+__global__ __launch_bounds__(NT) void k_splash_cells(long long n, const unsigned *__restrict__ keys, int ncell,
+                                                      int *__restrict__ cs)
+{
+    const long long i = (long long)blockIdx.x * NT + threadIdx.x;
+    if (i > n) return;
+    const int k = i == n ? ncell : (int)keys[i];
+    const int kp = i == 0 ? -1 : (int)keys[i - 1];
+    for (int c = kp + 1; c <= k; ++c) cs[c] = (int)i;
+}
+
+// One thread per pixel (i, j) at (x[i], y[j]); Z[i * ny + j] sums
+// w_p W(s, h_p) over particles within the cutoff (minimum image with periods
+// Lx, Ly, dr = (x_p - x, y - y_p) as interpolate.py:76).  Cells are at least
+// the cutoff wide, so the 3x3 stencil of the pixel's cell holds them all
+// (every cell when an axis has fewer than three).
+__global__ __launch_bounds__(NT) void k_splash_pixels(int nx, int ny, const double *__restrict__ x,
+                                                       const double *__restrict__ y, SplashGrid G,
+                                                       const double4 *__restrict__ rec,
+                                                       const int *__restrict__ cs, double *__restrict__ Z)
+{
+    const long long pix = (long long)blockIdx.x * NT + threadIdx.x;
+    if (pix >= (long long)nx * ny) return;
+    const int i = (int)(pix / ny), j = (int)(pix % ny);
+    const double px = x[i], py = y[j];
+    double wx = px - G.Lx * floor(px / G.Lx), wy = py - G.Ly * floor(py / G.Ly);
+    int cx = (int)(wx / G.cwx), cy = (int)(wy / G.cwy);
+    cx = cx < 0 ? 0 : (cx >= G.ncx ? G.ncx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= G.ncy ? G.ncy - 1 : cy);
+    const int sx = G.ncx >= 3 ? 1 : 0, sy = G.ncy >= 3 ? 1 : 0;
+    const int x0 = sx ? -1 : 0, x1 = sx ? 1 : G.ncx - 1, y0 = sy ? -1 : 0, y1 = sy ? 1 : G.ncy - 1;
+    const double c2 = G.cutoff * G.cutoff;
+    const double hLx = G.Lx / 2., hLy = G.Ly / 2.;
+    double z = 0.0;
+    for (int oy = y0; oy <= y1; ++oy) {
+        const int yy = sy ? wrapi(cy + oy, G.ncy) : oy;
+        for (int ox = x0; ox <= x1; ++ox) {
+            const int xx = sx ? wrapi(cx + ox, G.ncx) : ox;
+            const int c = yy * G.ncx + xx;
+            for (int t = cs[c]; t < cs[c + 1]; ++t) {
+                const double4 p = rec[t];
+                double dx = p.x - px, dy = py - p.y;
+                dx = dx > hLx ? dx - G.Lx : dx;  // neighbour_list.py:110-123 order
+                dy = dy > hLy ? dy - G.Ly : dy;
+                dx = dx < -hLx ? dx + G.Lx : dx;
+                dy = dy < -hLy ? dy + G.Ly : dy;
+                const double rsq = rsq_exact(dx, dy);
+                if (rsq < c2) {
+                    const double sr = sqrt(rsq);
+                    if (sr < p.w) {  // spkernel.lucy_w, 2-D
+                        const double q = sr / p.w, tt = 1.0 - q;
+                        z += p.z * ((5.0 / (SPH_PI * p.w * p.w)) * (1.0 + 3.0 * q) * (tt * tt * tt));
+                    }
+                }
+            }
+        }
+    }
+    Z[pix] = z;
+}
+
+// ===========================================================================
 // host launchers
 // ===========================================================================
 
@@ -1518,6 +1614,20 @@ void launch_separations(const ArenaView &A, int box, const int *inv, long long n
                         hipStream_t st)
 {
     hipLaunchKernelGGL(k_separations, grid_for(np), dim3(NT), 0, st, A, box, inv, np, iap, drij, rij, rsq, dv);
+}
+void launch_splash(long long n, const double *r, const double *m, const double *rho, const double *a,
+                   const double *h, const SplashGrid &G, double4 *rec, double4 *rec_sorted, unsigned *keys,
+                   unsigned *k_out, unsigned *v_out, unsigned *k_alt, unsigned *v_alt, void *sort_tmp, int key_bits,
+                   int *cs, int nx, int ny, const double *x, const double *y, double *Z, hipStream_t st)
+{
+    if (n > 0) {
+        hipLaunchKernelGGL(k_splash_prep, grid_for(n), dim3(NT), 0, st, n, r, m, rho, a, h, G, rec, keys);
+        radix_sort_pairs(keys, k_out, v_out, k_alt, v_alt, (int)n, key_bits, sort_tmp, st);
+        hipLaunchKernelGGL(k_splash_gather, grid_for(n), dim3(NT), 0, st, n, v_out, rec, rec_sorted);
+    }
+    hipLaunchKernelGGL(k_splash_cells, grid_for(n + 1), dim3(NT), 0, st, n, k_out, G.ncx * G.ncy, cs);
+    const long long np = (long long)nx * ny;
+    hipLaunchKernelGGL(k_splash_pixels, grid_for(np), dim3(NT), 0, st, nx, ny, x, y, G, rec_sorted, cs, Z);
 }
 void launch_lucy(long long np, const double *r, const double *dx, double h, int dim, double *w,
                  double *dwdx, hipStream_t st)

@@ -106,6 +106,15 @@ class DeviceBox:
     def set_graph(self, on):
         _lib.check(_lib.lib.sph_box_set_graph(self._h, 1 if on else 0))
 
+    def snapshot(self, bufs):
+        """Start an asynchronous copy of r, v, u, m into bufs (a PinnedState);
+        steps enqueued afterwards overlap the transfer."""
+        _lib.check(_lib.lib.sph_box_snapshot(self._h, _lib.dptr(bufs.r), _lib.dptr(bufs.v), _lib.dptr(bufs.u),
+                                             _lib.dptr(bufs.m)))
+
+    def snapshot_wait(self):
+        _lib.check(_lib.lib.sph_box_snapshot_wait(self._h))
+
     def set_schedule(self, unroll):
         """List entries in flight per lane in the pair passes (1, 2, 4; 0 = default)."""
         _lib.check(_lib.lib.sph_box_set_schedule(self._h, int(unroll)))
@@ -141,6 +150,38 @@ class DeviceBox:
         _lib.check(_lib.lib.sph_separations(self._h, nip, _lib.i64ptr(iap), _lib.dptr(drij), _lib.dptr(rij),
                                             _lib.dptr(rsq), _lib.dptr(dv)))
         return drij, rij, rsq, dv
+
+
+class PinnedState:
+    """r[n][2], v[n][2], u[n], m[n] in pinned host memory (hipHostMalloc),
+    the target of DeviceBox.snapshot."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        nbytes = 8 * 6 * self.n
+        self._p = _lib._P()
+        _lib.check(_lib.lib.sph_host_alloc(nbytes, ctypes.byref(self._p)))
+        if nbytes:
+            buf = (ctypes.c_double * (6 * self.n)).from_address(self._p.value)
+            flat = np.ctypeslib.as_array(buf)
+        else:
+            flat = np.empty(0)
+        self.r = flat[: 2 * n].reshape(n, 2)
+        self.v = flat[2 * n: 4 * n].reshape(n, 2)
+        self.u = flat[4 * n: 5 * n]
+        self.m = flat[5 * n: 6 * n]
+
+    def close(self):
+        if self._p:
+            self.r = self.v = self.u = self.m = None
+            _lib.lib.sph_host_free(self._p)
+            self._p = _lib._P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def from_system(sysm, device=0):

@@ -375,3 +375,40 @@ def test_leapfrog_free_particles_exact():
     r2, v2, u2 = O.run(r, v, np.ones(2), np.array([-1.0, -1.0]), prm, 150)
     np.testing.assert_array_equal(v2, v)
     np.testing.assert_allclose(r2, np.mod(r + 1.5 * v, 60.0), rtol=1e-12)
+
+
+# ---------------------------------------------------------------- splash rendering
+def test_splash_lattice_sites_render_one():
+    # at a lattice site sum_j m_j/rho_j W = (sum_j W)/rho = 1 (self term included)
+    ns, L = 16, 16.0
+    r = configs.lattice(ns * ns, L, None, jitter=0)
+    rho0 = json.load(open(os.path.join(GOLD, "lattice.json")))["rho_h2"]
+    x = np.arange(32) * 0.5
+    Z = O.splash_map_2d(x, x, r, np.ones(ns * ns), np.full(ns * ns, rho0), np.ones(ns * ns), 2.0,
+                        (0, L, 0, L), 2.6)
+    np.testing.assert_allclose(Z[1::2, 1::2], 1.0, rtol=1e-14)
+    assert 0.9 < Z.min() <= Z.max() < 1.1
+
+
+def test_splash_brute_force_and_linearity():
+    rng = np.random.default_rng(4)
+    n, L = 40, 8.0
+    r = rng.uniform(0, L, size=(n, 2))
+    m, rho, a = rng.uniform(0.5, 1.5, n), rng.uniform(0.8, 1.2, n), rng.normal(size=n)
+    x = np.arange(16) * 0.5
+    y = np.arange(16) * 0.5
+    h, cut = 1.5, 2.1  # cutoff >= h + res: the half-open pixel ranges drop only zero-weight pixels
+    Z = O.splash_map_2d(x, y, r, m, rho, a, h, (0, L, 0, L), cut)
+    # brute force over every pixel and particle (minimum image)
+    ref = np.zeros((16, 16))
+    for i in range(16):
+        for j in range(16):
+            d = np.stack([r[:, 0] - x[i], y[j] - r[:, 1]], axis=1)
+            d = np.where(d > L / 2, d - L, np.where(d < -L / 2, d + L, d))
+            s = np.sqrt((d ** 2).sum(1))
+            q = s / h
+            w = np.where(s < h, 5 / (math.pi * h * h) * (1 + 3 * q) * (1 - q) ** 3, 0.0)
+            ref[i, j] = np.sum(np.where(s < cut, a * m / rho * w, 0.0))
+    np.testing.assert_allclose(Z, ref, rtol=1e-12, atol=1e-14)
+    Z2 = O.splash_map_2d(x, y, r, m, rho, 3.0 * a, h, (0, L, 0, L), cut)
+    np.testing.assert_allclose(Z2, 3.0 * Z, rtol=1e-13, atol=1e-15)
