@@ -208,6 +208,8 @@ int sph_arena_remove(sph_arena *ar, int box_id);
 /* Occupancy: particles in use, capacity, running boxes, total boxes. */
 int sph_arena_stats(sph_arena *ar, int64_t *used, int64_t *capacity,
                     int *running, int *boxes);
+/* Per-pass GPU time of the arena's steps, as sph_box_timing. */
+int sph_arena_timing(sph_arena *ar, double *out, int n, int reset);
 /* Replay the step as a captured HIP graph (default on) or launch directly. */
 int sph_arena_set_graph(sph_arena *ar, int on);
 int sph_box_set_graph(sph_box *box, int on);

@@ -88,6 +88,7 @@ _SIGS = {
     "sph_arena_stats": [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "sph_arena_set_graph": [_P, ctypes.c_int],
+    "sph_arena_timing": [_P, _D, ctypes.c_int, ctypes.c_int],
     "sph_arena_set_schedule": [_P, ctypes.c_int],
     "sph_box_set_schedule": [_P, ctypes.c_int],
     "sph_arena_footprint": [_I64, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)],

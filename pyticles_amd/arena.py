@@ -74,6 +74,14 @@ class Arena:
     def set_graph(self, on):
         _lib.check(_lib.lib.sph_arena_set_graph(self._h, 1 if on else 0))
 
+    def timing(self, reset=False):
+        """Accumulated per-pass GPU ms while context timing is on (keys as
+        device.DeviceBox.timing)."""
+        out = np.zeros(7)
+        _lib.check(_lib.lib.sph_arena_timing(self._h, _lib.dptr(out), 7, 1 if reset else 0))
+        keys = ("build_ms", "density_ms", "gradients_ms", "rates_ms", "pairs_ms", "steps", "integrate_ms")
+        return dict(zip(keys, out))
+
     def set_schedule(self, unroll):
         """List entries in flight per lane in the pair passes (1, 2, 4; 0 = default)."""
         _lib.check(_lib.lib.sph_arena_set_schedule(self._h, int(unroll)))

@@ -424,7 +424,7 @@ struct Arena {
         if (hwm == 0) return;
         launch_keys(A, force_mask, st);
         radix_sort_pairs(keys_in, k_out, v_out, k_alt, v_alt, hwm, bits_for(nkeys), sort_tmp, st);
-        launch_permute(A, st);
+        launch_permute(A, force_mask, st);
         launch_cell_start(A, st);
         launch_build_lists(A, force_mask, fp32() ? 1 : 0, st);
     }
@@ -1090,10 +1090,9 @@ int sph_energy(sph_box *box, double *ke, double *ue, double *etot)
     return SPH_OK;
 }
 
-int sph_box_timing(sph_box *box, double *out, int n, int reset)
+static int read_timing(Arena &A, double *out, int n, int reset)
 {
-    if (!box || n < 0 || n > 7 || (n > 0 && !out)) return fail(SPH_ERR_ARG, "invalid argument");
-    Arena &A = box->ar;
+    if (n < 0 || n > 7 || (n > 0 && !out)) return fail(SPH_ERR_ARG, "invalid argument");
     const double vals[7] = {A.t_acc[T_SORT], A.t_acc[T_DENS], A.t_acc[T_GRAD], A.t_acc[T_RATES],
                             A.t_acc[T_PAIRS], A.steps_done, A.t_acc[T_INT]};
     for (int i = 0; i < n; ++i) out[i] = vals[i];
@@ -1104,7 +1103,19 @@ int sph_box_timing(sph_box *box, double *out, int n, int reset)
     return SPH_OK;
 }
 
+int sph_box_timing(sph_box *box, double *out, int n, int reset)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    return read_timing(box->ar, out, n, reset);
+}
+
 // ---- batching and serving ------------------------------------------------------
+
+int sph_arena_timing(sph_arena *ar, double *out, int n, int reset)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    return read_timing(ar->ar, out, n, reset);
+}
 
 int sph_arena_create(sph_ctx *ctx, int64_t particle_capacity, int precision, sph_arena **out)
 {

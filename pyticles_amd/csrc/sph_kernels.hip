@@ -142,6 +142,42 @@ __device__ __forceinline__ float4 fix_pos(double2 r, int cell, double m, const B
                        __int_as_float(cx | (cy << 16)));
 }
 
+// Wave-uniform values: a value the lanes agree on, moved to a scalar
+// register (readfirstlane), and a waterfall over the boxes of a wave.
+__device__ __forceinline__ double wave_uniform(double x)
+{
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ float wave_uniform(float x)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Runs f(bu) once for each distinct box bu among the active lanes, on the
+// lanes of that box, with bu wave-uniform (a waterfall over the boxes of the
+// wave: one round for all waves but those straddling a box boundary).
+template <class F>
+__device__ __forceinline__ void per_box(int b, F &&f)
+{
+    bool done = false;
+    for (;;) {
+        if (!done) {
+            const int bu = __builtin_amdgcn_readfirstlane(b);
+            if (b == bu) {
+                f(bu);
+                done = true;
+            }
+        }
+        if (__ballot(!done) == 0ull) break;
+    }
+}
+
 // ===========================================================================
 // build: binning, permutation, cell starts, neighbour lists
 // ===========================================================================
@@ -175,14 +211,14 @@ __global__ __launch_bounds__(NT) void k_keys(ArenaView A, int force_mask)
     A.keys_in[s] = (unsigned)key;
 }
 
-// Gather every per-slot quantity into sorted scratch (identity for boxes
-// that do not build, whose keys are constant).
-__global__ __launch_bounds__(NT) void k_permute(ArenaView A)
+// Gather every per-slot quantity of a building box into sorted scratch
+// (the order of the other boxes is the identity and they are left alone).
+__global__ __launch_bounds__(NT) void k_permute(ArenaView A, int force_mask)
 {
     const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
     const int b = A.slot_box[s];
-    if (b < 0) return;
+    if (b < 0 || !building(A.dyn[b], force_mask)) return;
     const int src = (int)A.vals[s];
     A.r_s[s] = A.r[src];
     A.v_s[s] = A.v[src];
@@ -250,19 +286,47 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
     const int lcap = A.lcap;
     unsigned *nbw = A.nbr + (list_of(A, s) - A.nbr);
     int ns = 0, nl = 0;
-    for_each_cell(B, A.cell_start, cell, [&](int t0, int t1, int, int) {
-        for (int t = t0; t < t1; ++t) {
-            const double2 rj = A.r_s[t];
-            double dx = rj.x - ri.x, dy = rj.y - ri.y;
-            min_image(dx, dy, B);
-            const double r2 = rsq_exact(dx, dy);
-            if (r2 < B.rnl2 && t != s) {
-                if (ns + nl < lcap) {
-                    if (r2 < B.hsk2) nbw[(size_t)ns * LIST_TILE] = (unsigned)t;
-                    else nbw[(size_t)(lcap - 1 - nl) * LIST_TILE] = (unsigned)t;
+    // Box constants wave-uniform (scalar registers); each stencil cell's
+    // periodic image is applied to r_i once per cell, so the candidate loop
+    // needs no minimum image (axes with a single cell keep it).
+    per_box(b, [&](int bu) {
+        const BoxDesc &Bu = A.boxes[bu];
+        const double rnl2 = wave_uniform(Bu.rnl2), hsk2 = wave_uniform(Bu.hsk2);
+        const double Lx = wave_uniform(Bu.Lx), Ly = wave_uniform(Bu.Ly);
+        const double hLx = wave_uniform(Bu.hLx), hLy = wave_uniform(Bu.hLy);
+        const int ncx = wave_uniform(Bu.ncx), ncy = wave_uniform(Bu.ncy);
+        const int sx = wave_uniform(Bu.sx), sy = wave_uniform(Bu.sy);
+        const int *cs = A.cell_start + wave_uniform(Bu.key_base);
+        const int cx = cell % ncx, cy = cell / ncx;
+        for (int oy = -sy; oy <= sy; ++oy) {
+            const int yr = cy + oy;
+            const int yy = yr < 0 ? yr + ncy : (yr >= ncy ? yr - ncy : yr);
+            const double ry = ri.y - (yr < 0 ? -Ly : (yr >= ncy ? Ly : 0.0));
+            for (int ox = -sx; ox <= sx; ++ox) {
+                const int xr = cx + ox;
+                const int xx = xr < 0 ? xr + ncx : (xr >= ncx ? xr - ncx : xr);
+                const double rx = ri.x - (xr < 0 ? -Lx : (xr >= ncx ? Lx : 0.0));
+                const int c = yy * ncx + xx;
+                const int t1 = cs[c + 1];
+                for (int t = cs[c]; t < t1; ++t) {
+                    const double2 rj = A.r_s[t];
+                    double dx = rj.x - rx, dy = rj.y - ry;
+                    if (sx == 0) {
+                        dx = dx > hLx ? dx - Lx : dx;
+                        dx = dx < -hLx ? dx + Lx : dx;
+                    }
+                    if (sy == 0) {
+                        dy = dy > hLy ? dy - Ly : dy;
+                        dy = dy < -hLy ? dy + Ly : dy;
+                    }
+                    const double r2 = rsq_exact(dx, dy);
+                    if (r2 < rnl2 && t != s) {
+                        const bool sh = r2 < hsk2;
+                        if (ns + nl < lcap) nbw[(size_t)(sh ? ns : lcap - 1 - nl) * LIST_TILE] = (unsigned)t;
+                        ns += sh ? 1 : 0;
+                        nl += sh ? 0 : 1;
+                    }
                 }
-                if (r2 < B.hsk2) ++ns;
-                else ++nl;
             }
         }
     });
@@ -289,40 +353,6 @@ struct BoxK {
     double ih, ihl, qh, qhl, fh, fhl, munif;
     int terms;
 };
-
-__device__ __forceinline__ double wave_uniform(double x)
-{
-    const long long v = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffll));
-    const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-__device__ __forceinline__ float wave_uniform(float x)
-{
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
-}
-
-__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
-
-// Runs f(bu) once for each distinct box bu among the active lanes, on the
-// lanes of that box, with bu wave-uniform (a waterfall over the boxes of the
-// wave: one round for all waves but those straddling a box boundary).
-template <class F>
-__device__ __forceinline__ void per_box(int b, F &&f)
-{
-    bool done = false;
-    for (;;) {
-        if (!done) {
-            const int bu = __builtin_amdgcn_readfirstlane(b);
-            if (b == bu) {
-                f(bu);
-                done = true;
-            }
-        }
-        if (__ballot(!done) == 0ull) break;
-    }
-}
 
 __device__ __forceinline__ BoxK box_k(const BoxDesc &B, bool uniform)
 {
@@ -1536,9 +1566,9 @@ void launch_keys(const ArenaView &A, int force_mask, hipStream_t st)
 {
     hipLaunchKernelGGL(k_keys, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
 }
-void launch_permute(const ArenaView &A, hipStream_t st)
+void launch_permute(const ArenaView &A, int force_mask, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_permute, grid_for(A.n), dim3(NT), 0, st, A);
+    hipLaunchKernelGGL(k_permute, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
 }
 void launch_cell_start(const ArenaView &A, hipStream_t st)
 {
