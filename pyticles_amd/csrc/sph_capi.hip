@@ -138,6 +138,7 @@ struct Arena {
     double2 *vt = nullptr, *aco = nullptr;
     double *uco = nullptr;
     unsigned *keys_in = nullptr, *k_out = nullptr, *v_out = nullptr, *k_alt = nullptr, *v_alt = nullptr;
+    int *plan_n = nullptr, *box_coff = nullptr;  // box_coff sized box_cap
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int *cell_start = nullptr;
@@ -204,6 +205,7 @@ struct Arena {
         A.vt = vt; A.aco = aco; A.uco = uco;
         A.cell_start = cell_start;
         A.keys = k_out; A.vals = v_out; A.keys_in = keys_in;
+        A.plan_n = plan_n; A.box_coff = box_coff;
         A.boxes = d_boxes; A.dyn = d_dyn; A.segs = d_segs;
         A.nseg = (int)segs.size();
         A.n = hwm;
@@ -240,6 +242,7 @@ struct Arena {
         sort_tmp_bytes = radix_sort_temp_bytes(cap);
         rc |= dalloc((char **)&sort_tmp, sort_tmp_bytes);
         rc |= dalloc(&d_e3, 3);
+        rc |= dalloc(&plan_n, 2);
         if (rc) return SPH_ERR_NOMEM;
         // zero-fill the state so that unused slots hold finite values
         size_t z2 = sizeof(double2) * cap, z1 = sizeof(double) * cap;
@@ -267,6 +270,7 @@ struct Arena {
         dfree(rho); dfree(Q); dfree(jq); dfree(T); dfree(e); dfree(S); dfree(gradv);
         dfree(rf); dfree(S32); dfree(vf); dfree(vq32); dfree(vt32); dfree(vt); dfree(aco); dfree(uco);
         dfree(keys_in); dfree(k_out); dfree(v_out); dfree(k_alt); dfree(v_alt);
+        dfree(plan_n); dfree(box_coff);
         if (sort_tmp) (void)hipFree(sort_tmp);
         sort_tmp = nullptr;
         if (snap_pending) (void)hipEventSynchronize(ev_snap_done);
@@ -294,7 +298,8 @@ struct Arena {
             int nc = std::max(nbox, box_cap * 2);
             BoxDesc *nb = nullptr;
             BoxDyn *nd = nullptr;
-            if (dalloc(&nb, nc) || dalloc(&nd, nc)) return SPH_ERR_NOMEM;
+            int *nco = nullptr;
+            if (dalloc(&nb, nc) || dalloc(&nd, nc) || dalloc(&nco, nc)) return SPH_ERR_NOMEM;
             HIPCHK(hipMemsetAsync(nd, 0, sizeof(BoxDyn) * nc, ctx->stream));
             if (box_cap > 0) {
                 HIPCHK(hipMemcpyAsync(nd, d_dyn, sizeof(BoxDyn) * box_cap, hipMemcpyDeviceToDevice, ctx->stream));
@@ -302,8 +307,10 @@ struct Arena {
             HIPCHK(hipStreamSynchronize(ctx->stream));
             dfree(d_boxes);
             dfree(d_dyn);
+            dfree(box_coff);
             d_boxes = nb;
             d_dyn = nd;
+            box_coff = nco;
             BoxDyn *nh = nullptr;
             HIPCHK(hipHostMalloc((void **)&nh, sizeof(BoxDyn) * nc, hipHostMallocDefault));
             if (h_dyn) (void)hipHostFree(h_dyn);
@@ -361,6 +368,7 @@ struct Arena {
                                   ctx->stream));
         HIPCHK(hipMemcpyAsync(d_segs, segs.data(), sizeof(SegDesc) * segs.size(), hipMemcpyHostToDevice,
                               ctx->stream));
+        launch_slot_box(view(), ctx->stream);
         // host-side vectors must outlive the async copies
         HIPCHK(hipStreamSynchronize(ctx->stream));
         // the slot count, key count and segment count are baked into
@@ -422,10 +430,13 @@ struct Arena {
         hipStream_t st = ctx->stream;
         ArenaView A = view();
         if (hwm == 0) return;
+        // the sort runs over the M building slots (M read on the device);
+        // the other kernels run over the slots and skip non-building boxes
+        launch_build_plan(A, force_mask, st);
         launch_keys(A, force_mask, st);
-        radix_sort_pairs(keys_in, k_out, v_out, k_alt, v_alt, hwm, bits_for(nkeys), sort_tmp, st);
+        radix_sort_pairs(keys_in, k_out, v_out, k_alt, v_alt, hwm, bits_for(nkeys), sort_tmp, st, plan_n + 1);
         launch_permute(A, force_mask, st);
-        launch_cell_start(A, st);
+        launch_cell_start(A, force_mask, st);
         launch_build_lists(A, force_mask, fp32() ? 1 : 0, st);
     }
 

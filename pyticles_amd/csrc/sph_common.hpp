@@ -111,6 +111,10 @@ struct ArenaView {
     unsigned *keys;      // sorted keys (final buffer of the radix sort)
     unsigned *vals;      // source slot of each sorted slot
     unsigned *keys_in;   // unsorted keys written by the key pass
+    // build plan: plan_n[1] = building slots M; box_coff[b] = compact
+    // position of building box b's first slot (its slots numbered on)
+    int *plan_n;
+    int *box_coff;
     // boxes
     const BoxDesc *boxes;
     BoxDyn *dyn;

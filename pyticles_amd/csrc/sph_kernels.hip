@@ -182,9 +182,57 @@ __device__ __forceinline__ void per_box(int b, F &&f)
 // build: binning, permutation, cell starts, neighbour lists
 // ===========================================================================
 
-// key = box key base + cell (row major) for building boxes; other boxes and
-// free ranges keep one constant key, so the stable sort leaves them in place.
-__global__ __launch_bounds__(NT) void k_keys(ArenaView A, int force_mask)
+// The build stage sorts the building boxes only.  A plan numbers their slots
+// consecutively, box after box in slot order (box b's first slot gets the
+// compact position box_coff[b]); the radix sort runs over those M compact
+// positions, so boxes that keep their lists cost it nothing.  Keys are global
+// (box key base + cell, box key ranges increase with the slot offset), so
+// after a stable sort the j-th sorted element belongs at the j-th building
+// slot.  The other kernels run over slots (a wave = one 64-slot tile) and
+// skip slots of boxes that do not build.
+
+static constexpr int PLAN_THREADS = 1024;
+
+// plan_n[1] = M (building slots); box_coff[b] for each building box b.
+__global__ __launch_bounds__(PLAN_THREADS) void k_build_plan(ArenaView A, int force_mask)
+{
+    __shared__ int ss[PLAN_THREADS];
+    const int tid = threadIdx.x;
+    const int per = (A.nseg + PLAN_THREADS - 1) / PLAN_THREADS;
+    const int g0 = tid * per, g1 = min(g0 + per, A.nseg);
+    int sz = 0;
+    for (int g = g0; g < g1; ++g) {
+        const SegDesc sg = A.segs[g];
+        if (sg.box >= 0 && building(A.dyn[sg.box], force_mask)) sz += sg.len;
+    }
+    ss[tid] = sz;
+    __syncthreads();
+    for (int off = 1; off < PLAN_THREADS; off <<= 1) {  // inclusive scan
+        const int c = tid >= off ? ss[tid - off] : 0;
+        __syncthreads();
+        ss[tid] += c;
+        __syncthreads();
+    }
+    int pos = ss[tid] - sz;
+    for (int g = g0; g < g1; ++g) {
+        const SegDesc sg = A.segs[g];
+        if (sg.box >= 0 && building(A.dyn[sg.box], force_mask)) {
+            A.box_coff[sg.box] = pos;
+            pos += sg.len;
+        }
+    }
+    if (tid == PLAN_THREADS - 1) A.plan_n[1] = ss[tid];
+}
+
+// Box of slot s if it builds in this launch, else -1.
+__device__ __forceinline__ int building_box(const ArenaView &A, int s, int force_mask)
+{
+    const int b = A.slot_box[s];
+    return (b >= 0 && building(A.dyn[b], force_mask)) ? b : -1;
+}
+
+// Slot owners, written when the layout changes (free slots: -1).
+__global__ __launch_bounds__(NT) void k_slot_box(ArenaView A)
 {
     const int s = blockIdx.x * NT + threadIdx.x;
     if (s >= A.n) return;
@@ -194,32 +242,37 @@ __global__ __launch_bounds__(NT) void k_keys(ArenaView A, int force_mask)
         if (A.segs[mid].off <= s) lo = mid;
         else hi = mid - 1;
     }
-    const SegDesc sg = A.segs[lo];
-    int key = sg.key, box = -1;
-    if (sg.box >= 0) {
-        box = sg.box;
-        if (building(A.dyn[box], force_mask)) {
-            const BoxDesc &B = A.boxes[box];
-            const double2 r = A.r[s];
-            int cx = (int)(r.x * B.icwx), cy = (int)(r.y * B.icwy);
-            cx = cx < 0 ? 0 : (cx >= B.ncx ? B.ncx - 1 : cx);
-            cy = cy < 0 ? 0 : (cy >= B.ncy ? B.ncy - 1 : cy);
-            key = B.key_base + cy * B.ncx + cx;
-        }
-    }
-    A.slot_box[s] = box;
-    A.keys_in[s] = (unsigned)key;
+    A.slot_box[s] = A.segs[lo].box;
 }
 
-// Gather every per-slot quantity of a building box into sorted scratch
-// (the order of the other boxes is the identity and they are left alone).
+// key = box key base + current cell (row major) at the slot's compact
+// position.
+__global__ __launch_bounds__(NT) void k_keys(ArenaView A, int force_mask)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = building_box(A, s, force_mask);
+    if (b < 0) return;
+    const BoxDesc &B = A.boxes[b];
+    const double2 r = A.r[s];
+    int cx = (int)(r.x * B.icwx), cy = (int)(r.y * B.icwy);
+    cx = cx < 0 ? 0 : (cx >= B.ncx ? B.ncx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= B.ncy ? B.ncy - 1 : cy);
+    A.keys_in[A.box_coff[b] + (s - B.part_off)] = (unsigned)(B.key_base + cy * B.ncx + cx);
+}
+
+// Sorted compact position j (slot s of box b) takes the state of the slot at
+// compact position vals[j] (same box: its key lies in the box's key range)
+// into sorted scratch and records its cell.
 __global__ __launch_bounds__(NT) void k_permute(ArenaView A, int force_mask)
 {
     const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
-    const int b = A.slot_box[s];
-    if (b < 0 || !building(A.dyn[b], force_mask)) return;
-    const int src = (int)A.vals[s];
+    const int b = building_box(A, s, force_mask);
+    if (b < 0) return;
+    const BoxDesc &B = A.boxes[b];
+    const int c0 = A.box_coff[b], j = c0 + (s - B.part_off);
+    const int src = B.part_off + ((int)A.vals[j] - c0);
     A.r_s[s] = A.r[src];
     A.v_s[s] = A.v[src];
     A.vp_s[s] = A.vp[src];
@@ -229,24 +282,29 @@ __global__ __launch_bounds__(NT) void k_permute(ArenaView A, int force_mask)
     A.up_s[s] = A.up[src];
     A.udot_s[s] = A.udot[src];
     A.id_s[s] = A.id[src];
+    const int cell = (int)A.keys[j] - B.key_base;
+    A.pcell[s] = (cell < 0 || cell >= B.ncx * B.ncy) ? 0 : cell;
 }
 
-// cell_start[k] = first sorted slot with key >= k, for k in [0, nkeys].
-__global__ __launch_bounds__(NT) void k_cell_start(ArenaView A)
+// cell_start over the key range of each building box: cell_start[c] = first
+// slot whose key >= c, and the end marker key_base + ncx*ncy = the slot past
+// the box (the next segment's first slot: a building next box writes the same
+// value there).
+__global__ __launch_bounds__(NT) void k_cell_start(ArenaView A, int force_mask)
 {
-    const int s = blockIdx.x * NT + threadIdx.x;
-    if (s > A.n) return;
-    if (A.n == 0) {
-        for (int c = 0; c <= A.nkeys; ++c) A.cell_start[c] = 0;
-        return;
+    const int s = xcd_block() * NT + threadIdx.x;
+    if (s >= A.n) return;
+    const int b = building_box(A, s, force_mask);
+    if (b < 0) return;
+    const BoxDesc &B = A.boxes[b];
+    const int i = s - B.part_off, j = A.box_coff[b] + i;
+    const int key = (int)A.keys[j];
+    const int kp = i == 0 ? B.key_base - 1 : (int)A.keys[j - 1];
+    for (int c = kp + 1; c <= key; ++c) A.cell_start[c] = s;
+    if (i == B.n - 1) {
+        const int kend = B.key_base + B.ncx * B.ncy;
+        for (int c = key + 1; c <= kend; ++c) A.cell_start[c] = s + 1;
     }
-    if (s == A.n) {
-        for (int c = (int)A.keys[A.n - 1] + 1; c <= A.nkeys; ++c) A.cell_start[c] = A.n;
-        return;
-    }
-    const int k = (int)A.keys[s];
-    const int kp = s == 0 ? -1 : (int)A.keys[s - 1];
-    for (int c = kp + 1; c <= k; ++c) A.cell_start[c] = s;
 }
 
 // Neighbour list of a building slot: every other slot of the 3x3 cell
@@ -254,14 +312,18 @@ __global__ __launch_bounds__(NT) void k_cell_start(ArenaView A)
 // from the front, the rest from the back.  The sorted state is copied back
 // into the state arrays, the build position kept for the skin check, and the
 // box's largest speed and acceleration recorded for its next schedule.
+//
+// A wave covers one 64-slot list tile, so every tile is written by one wave.
+// (Over compact positions, which are not tile-aligned, two waves, often on
+// two XCDs with their own L2, wrote most tiles and the kernel ran 2.4x
+// slower.)
 __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask, int fp32)
 {
     const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
-    const int b = A.slot_box[s];
+    const int b = building_box(A, s, force_mask);
     if (b < 0) return;
     BoxDyn &D = A.dyn[b];
-    if (!building(D, force_mask)) return;
     const BoxDesc &B = A.boxes[b];
     const double2 ri = A.r_s[s];
     const double2 vpi = A.vp_s[s], ai = A.acc_s[s];
@@ -276,9 +338,7 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
     A.udot[s] = A.udot_s[s];
     A.id[s] = A.id_s[s];
     A.r_ref[s] = ri;
-    int cell = (int)A.keys[s] - B.key_base;
-    cell = (cell < 0 || cell >= B.ncx * B.ncy) ? 0 : cell;
-    A.pcell[s] = cell;
+    const int cell = A.pcell[s];
     if (fp32) {
         A.rf[s] = fix_pos(ri, cell, mi, B);
         A.vf[s] = make_float2((float)vpi.x, (float)vpi.y);
@@ -1562,6 +1622,14 @@ __global__ __launch_bounds__(NT) void k_splash_pixels(int nx, int ny, const doub
 
 static inline dim3 grid_for(long long n) { return dim3((unsigned)((n + NT - 1) / NT > 0 ? (n + NT - 1) / NT : 1)); }
 
+void launch_slot_box(const ArenaView &A, hipStream_t st)
+{
+    if (A.n > 0) hipLaunchKernelGGL(k_slot_box, grid_for(A.n), dim3(NT), 0, st, A);
+}
+void launch_build_plan(const ArenaView &A, int force_mask, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_build_plan, dim3(1), dim3(PLAN_THREADS), 0, st, A, force_mask);
+}
 void launch_keys(const ArenaView &A, int force_mask, hipStream_t st)
 {
     hipLaunchKernelGGL(k_keys, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
@@ -1570,9 +1638,9 @@ void launch_permute(const ArenaView &A, int force_mask, hipStream_t st)
 {
     hipLaunchKernelGGL(k_permute, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
 }
-void launch_cell_start(const ArenaView &A, hipStream_t st)
+void launch_cell_start(const ArenaView &A, int force_mask, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cell_start, grid_for((long long)A.n + 1), dim3(NT), 0, st, A);
+    hipLaunchKernelGGL(k_cell_start, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
 }
 void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_t st)
 {

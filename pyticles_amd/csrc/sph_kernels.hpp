@@ -11,9 +11,13 @@ constexpr int MASK_RUN = (1 << BOX_RUN);
 constexpr int MASK_INIT = (1 << BOX_INIT);
 constexpr int MASK_ALL = (1 << BOX_INIT) | (1 << BOX_RUN) | (1 << BOX_DONE);
 
+// build stage: slot owners (on a layout change), then plan, keys, sort,
+// permutation, cell starts and lists of the building boxes
+void launch_slot_box(const ArenaView &A, hipStream_t st);
+void launch_build_plan(const ArenaView &A, int force_mask, hipStream_t st);
 void launch_keys(const ArenaView &A, int force_mask, hipStream_t st);
 void launch_permute(const ArenaView &A, int force_mask, hipStream_t st);
-void launch_cell_start(const ArenaView &A, hipStream_t st);
+void launch_cell_start(const ArenaView &A, int force_mask, hipStream_t st);
 void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_t st);
 void launch_density(const ArenaView &A, int fp32, int state_mask, int unroll, hipStream_t st);
 void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, int unroll, hipStream_t st);

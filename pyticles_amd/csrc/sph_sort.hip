@@ -56,12 +56,18 @@ __device__ unsigned block_exscan256(unsigned x, unsigned *lds /*[256]*/, unsigne
 }
 
 // hist[d * nblocks + blk] = count of digit d in tile blk.
+// n_dev (may be null): element count on the device, at most n.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const unsigned *__restrict__ keys, int n,
-                                                         int shift, unsigned *__restrict__ hist,
-                                                         int nblocks)
+                                                         const int *__restrict__ n_dev, int shift,
+                                                         unsigned *__restrict__ hist, int nblocks)
 {
     __shared__ unsigned wc[RS_WAVES][RS_BINS];
     const int tid = threadIdx.x, w = tid >> 6;
+    if (n_dev) n = min(n, *n_dev);
+    if ((int)blockIdx.x * RS_TILE >= n) {  // empty tile
+        for (int d = tid; d < RS_BINS; d += RS_THREADS) hist[(size_t)d * nblocks + blockIdx.x] = 0;
+        return;
+    }
     for (int i = tid; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wc[0][0])[i] = 0;
     __syncthreads();
     const int base = blockIdx.x * RS_TILE;
@@ -109,14 +115,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scan_digits(unsigned *__restr
 // Stable scatter of one tile.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const unsigned *__restrict__ kin, const unsigned *__restrict__ vin,
-    unsigned *__restrict__ kout, unsigned *__restrict__ vout, int n, int shift,
-    const unsigned *__restrict__ hist, const unsigned *__restrict__ tot, int nblocks,
+    unsigned *__restrict__ kout, unsigned *__restrict__ vout, int n, const int *__restrict__ n_dev,
+    int shift, const unsigned *__restrict__ hist, const unsigned *__restrict__ tot, int nblocks,
     int identity_vals)
 {
     __shared__ unsigned base[RS_BINS];
     __shared__ unsigned wc[RS_WAVES][RS_BINS];
     __shared__ unsigned lds[256];
     const int tid = threadIdx.x, w = tid >> 6;
+    if (n_dev) n = min(n, *n_dev);
+    if ((int)blockIdx.x * RS_TILE >= n) return;  // empty tile (uniform over the block)
     {
         unsigned total;
         const unsigned pre = block_exscan256(tot[tid], lds, &total);
@@ -162,9 +170,11 @@ size_t radix_sort_temp_bytes(int n)
 // Sorts (keys_in, implicit identity values) by the low key_bits bits.
 // Result lands in (k_out, v_out); k_alt/v_alt are ping-pong buffers.  The
 // input keys are left untouched.  temp holds radix_sort_temp_bytes(n).
+// With n_dev the element count is read on the device (min(n, *n_dev)): the
+// grid covers n and tiles past the count exit at once.
 void radix_sort_pairs(const unsigned *keys_in, unsigned *k_out, unsigned *v_out,
                       unsigned *k_alt, unsigned *v_alt, int n, int key_bits, void *temp,
-                      hipStream_t stream)
+                      hipStream_t stream, const int *n_dev)
 {
     if (n <= 0) return;
     const int nb = (n + RS_TILE - 1) / RS_TILE;
@@ -180,10 +190,10 @@ void radix_sort_pairs(const unsigned *keys_in, unsigned *k_out, unsigned *v_out,
         unsigned *ko = last_to_out ? k_out : k_alt;
         unsigned *vo = last_to_out ? v_out : v_alt;
         const int shift = 8 * p;
-        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, stream, kin, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, stream, kin, n, n_dev, shift, hist, nb);
         hipLaunchKernelGGL(k_rs_scan_digits, dim3(RS_BINS), dim3(RS_THREADS), 0, stream, hist, nb, tot);
         hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, stream, kin, vin, ko, vo, n,
-                           shift, hist, tot, nb, p == 0 ? 1 : 0);
+                           n_dev, shift, hist, tot, nb, p == 0 ? 1 : 0);
         kin = ko;
         vin = vo;
     }
