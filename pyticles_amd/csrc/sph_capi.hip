@@ -569,6 +569,9 @@ int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
     B.rnl2 = (rcut + p->skin) * (rcut + p->skin);
     B.hsk2 = (p->h + p->skin) * (p->h + p->skin);
     B.hskin2 = 0.25 * p->skin * p->skin;
+    // listed pairs are closer than r_cut + skin at the build and each end
+    // moves at most skin / 2 before the next, so within r_cut + 2 skin
+    B.rin = 2.0 * (rcut + p->skin);
     // cells cover both the pair-export radius and the list radius
     const double Rc = std::max(std::sqrt(B.R2list), rcut + p->skin) * (1.0 + 1e-9);
     int ncx = (int)std::floor(p->Lx / Rc), ncy = (int)std::floor(p->Ly / Rc);

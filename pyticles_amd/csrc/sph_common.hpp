@@ -48,7 +48,9 @@ struct BoxDesc {
     double eta, zeta, kcond;       // transport coefficients
     double dt, T0;                 // time step, initial temperature
     double skin, rnl2, hsk2, hskin2;  // list skin, (r_cut+skin)^2, (h+skin)^2, (skin/2)^2
-    double munif, pad_d;           // the common mass when all masses are equal, else 0
+    double munif;                  // the common mass when all masses are equal, else 0
+    double rin;                    // interior margin 2 (r_cut + skin): a particle at least this far
+                                   // from every side has no listed neighbour across a side
     int ncx, ncy, sx, sy;          // cell grid and stencil half widths (0/1)
     int key_base, part_off, n, terms;
     int precision, pad0, pad1, pad2;

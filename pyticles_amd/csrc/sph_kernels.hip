@@ -410,7 +410,7 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
 struct BoxK {
     double Lx, Ly, hLx, hLy;
     double cut2, h2, hl2, cutl, cuth;  // cutl = min(cut2, hl2), cuth = min(cut2, h2)
-    double ih, ihl, qh, qhl, fh, fhl, munif;
+    double ih, ihl, qh, qhl, fh, fhl, munif, rin;
     int terms;
 };
 
@@ -423,6 +423,7 @@ __device__ __forceinline__ BoxK box_k(const BoxDesc &B, bool uniform)
     k.cuth = B.cut2 < B.h2 ? B.cut2 : B.h2;
     k.ih = B.ih; k.ihl = B.ihl; k.qh = B.qh; k.qhl = B.qhl; k.fh = B.fh; k.fhl = B.fhl;
     k.munif = B.munif;
+    k.rin = B.rin;
     k.terms = B.terms;
     if (uniform) {
         k.Lx = wave_uniform(k.Lx); k.Ly = wave_uniform(k.Ly); k.hLx = wave_uniform(k.hLx); k.hLy = wave_uniform(k.hLy);
@@ -430,6 +431,7 @@ __device__ __forceinline__ BoxK box_k(const BoxDesc &B, bool uniform)
         k.cutl = wave_uniform(k.cutl); k.cuth = wave_uniform(k.cuth);
         k.ih = wave_uniform(k.ih); k.ihl = wave_uniform(k.ihl); k.qh = wave_uniform(k.qh); k.qhl = wave_uniform(k.qhl);
         k.fh = wave_uniform(k.fh); k.fhl = wave_uniform(k.fhl); k.munif = wave_uniform(k.munif);
+        k.rin = wave_uniform(k.rin);
         k.terms = wave_uniform(k.terms);
     }
     return k;
@@ -438,6 +440,32 @@ __device__ __forceinline__ BoxK box_k(const BoxDesc &B, bool uniform)
 // ===========================================================================
 // FP64 form
 // ===========================================================================
+
+// True when no active lane's particle lies within rin of a side: then no
+// listed neighbour is across a side, the raw separation r_j - r_i is the
+// minimum image, and the walks skip the image test (same bits: the test
+// would leave every such separation alone).
+__device__ __forceinline__ bool wave_interior(double2 r, const BoxK &K)
+{
+    const bool in = r.x >= K.rin && r.x <= K.Lx - K.rin && r.y >= K.rin && r.y <= K.Ly - K.rin;
+    return __ballot(!in) == 0ull;
+}
+
+// sqrt(x) for x >= 0 in a few FMAs: the hardware reciprocal square root,
+// one Goldschmidt step for (sqrt x, 1 / (2 sqrt x)) and one Newton
+// correction of the root, within about 1 ulp of the rounded root (pair
+// separations only; x below 1e-280 returns 0).
+__device__ __forceinline__ double sqrt_pair(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    const double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return x >= 1e-280 ? g : 0.0;
+}
 
 __device__ __forceinline__ double vdw_p(double rho, double T, const BoxDesc &B)
 {
@@ -450,7 +478,7 @@ struct Dens64 {
 
 // P1 list walk.  COH: accumulate the cohesive terms (needs velocities);
 // UNIM: all masses equal (no mass fetch).
-template <int U64, bool COH, bool UNIM>
+template <int U64, bool COH, bool UNIM, bool WRAP>
 __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK &K, int s)
 {
     const double2 ri = A.r[s], vi = A.vp[s];
@@ -461,13 +489,15 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
     d.rhol = K.qhl;
     d.gx = d.gy = d.ev = 0.0;
     // U64 entries per round: all their loads are issued before any use so
-    // several neighbour fetches are in flight per wavefront
+    // several neighbour fetches are in flight per wavefront.  Rounds past the
+    // count re-read the last entry (a valid neighbour whose terms are masked
+    // off), so the list loads take no branch.
     for (int k0 = 0; k0 < c.x; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64];
         double mj[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
+        for (int u = 0; u < U64; ++u) t[u] = (int)nb[(size_t)min(k0 + u, c.x - 1) * LIST_TILE];
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -477,11 +507,11 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            min_image(dx, dy, K);
+            if (WRAP) min_image(dx, dy, K);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < c.x && r2 < K.cut2) {
                 const double m = UNIM ? K.munif : mj[u];
-                const double rr = sqrt(r2);
+                const double rr = sqrt_pair(r2);
                 if (r2 < K.hl2) {
                     const double q = rr * K.ihl, w = 1.0 - q;
                     d.rhol += m * (K.qhl * (1.0 + 3.0 * q) * (w * w * w));
@@ -507,7 +537,7 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
         double2 rj[U64], vv[U64];
         double mj[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * LIST_TILE] : s;
+        for (int u = 0; u < U64; ++u) t[u] = (int)nb[(size_t)(A.lcap - 1 - min(k0 + u, c.y - 1)) * LIST_TILE];
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -517,11 +547,11 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            min_image(dx, dy, K);
+            if (WRAP) min_image(dx, dy, K);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < c.y && r2 < K.cutl) {
                 const double m = UNIM ? K.munif : mj[u];
-                const double rr = sqrt(r2);
+                const double rr = sqrt_pair(r2);
                 const double q = rr * K.ihl, w = 1.0 - q;
                 d.rhol += m * (K.qhl * (1.0 + 3.0 * q) * (w * w * w));
                 if (COH) {
@@ -560,8 +590,13 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
     per_box(b, [&](int bu) {
         const BoxK K = box_k(A.boxes[bu], true);
         const bool coh = (K.terms & 2) != 0, unim = K.munif > 0.0;
-        if (coh) d = unim ? density64_walk<U64, true, true>(A, K, s) : density64_walk<U64, true, false>(A, K, s);
-        else d = unim ? density64_walk<U64, false, true>(A, K, s) : density64_walk<U64, false, false>(A, K, s);
+        if (wave_interior(A.r[s], K)) {
+            if (coh) d = unim ? density64_walk<U64, true, true, false>(A, K, s) : density64_walk<U64, true, false, false>(A, K, s);
+            else d = unim ? density64_walk<U64, false, true, false>(A, K, s) : density64_walk<U64, false, false, false>(A, K, s);
+        } else {
+            if (coh) d = unim ? density64_walk<U64, true, true, true>(A, K, s) : density64_walk<U64, true, false, true>(A, K, s);
+            else d = unim ? density64_walk<U64, false, true, true>(A, K, s) : density64_walk<U64, false, false, true>(A, K, s);
+        }
     });
     const BoxDesc &B = A.boxes[b];
     const bool coh = (B.terms & 2u) != 0;
@@ -592,7 +627,7 @@ struct Grad64 {
     double g00, g01, g10, g11, gt0, gt1;
 };
 
-template <int U64>
+template <int U64, bool WRAP>
 __device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K, int s)
 {
     const double2 ri = A.r[s], vi = A.vp[s];
@@ -605,7 +640,7 @@ __device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K,
         int t[U64];
         double2 rj[U64], vv[U64], vt[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
+        for (int u = 0; u < U64; ++u) t[u] = (int)nb[(size_t)min(k0 + u, ns - 1) * LIST_TILE];
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -615,10 +650,10 @@ __device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K,
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            min_image(dx, dy, K);
+            if (WRAP) min_image(dx, dy, K);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < ns && r2 < K.cuth) {
-                const double rr = sqrt(r2);
+                const double rr = sqrt_pair(r2);
                 const double q = rr * K.ih, w = 1.0 - q;
                 const double F = K.fh * w * w;
                 const double gx = -F * dx, gy = -F * dy;
@@ -652,7 +687,10 @@ __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int 
     const BoxDesc &B = A.boxes[b];
     if (!diag && !(B.terms & 12u)) return;
     Grad64 g;
-    per_box(b, [&](int bu) { g = grad64_walk<U64>(A, box_k(A.boxes[bu], true), s); });
+    per_box(b, [&](int bu) {
+        const BoxK K = box_k(A.boxes[bu], true);
+        g = wave_interior(A.r[s], K) ? grad64_walk<U64, false>(A, K, s) : grad64_walk<U64, true>(A, K, s);
+    });
     const double Ti = A.T[s];
     const double rho = A.rho[s].x, r2i = rho * rho;
     double pxx = 0, pxy = 0, pyy = 0;
@@ -681,7 +719,7 @@ struct Rate64 {
     double ax, ay, ud;
 };
 
-template <int U64, bool COND>
+template <int U64, bool COND, bool WRAP>
 __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K, int s, int ns)
 {
     const double2 ri = A.r[s], vi = A.vp[s];
@@ -695,7 +733,7 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
         double2 rj[U64], vv[U64], Qj[U64];
         double4 Sj[U64];
 #pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
+        for (int u = 0; u < U64; ++u) t[u] = (int)nb[(size_t)min(k0 + u, ns - 1) * LIST_TILE];
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -706,10 +744,10 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            min_image(dx, dy, K);
+            if (WRAP) min_image(dx, dy, K);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < ns && r2 < K.cuth) {
-                const double rr = sqrt(r2);
+                const double rr = sqrt_pair(r2);
                 const double q = rr * K.ih, w = 1.0 - q;
                 const double F = K.fh * w * w;
                 const double wx = F * dx, wy = F * dy;
@@ -745,7 +783,10 @@ __global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
     Rate64 o;
     per_box(b, [&](int bu) {
         const BoxK K = box_k(A.boxes[bu], true);
-        o = (K.terms & 8) ? rates64_walk<U64, true>(A, K, s, ns) : rates64_walk<U64, false>(A, K, s, ns);
+        if (wave_interior(A.r[s], K))
+            o = (K.terms & 8) ? rates64_walk<U64, true, false>(A, K, s, ns) : rates64_walk<U64, false, false>(A, K, s, ns);
+        else
+            o = (K.terms & 8) ? rates64_walk<U64, true, true>(A, K, s, ns) : rates64_walk<U64, false, true>(A, K, s, ns);
     });
     const double2 ac = A.aco[s];
     const double ax = o.ax + ac.x, ay = o.ay + ac.y, ud = o.ud + A.uco[s];
@@ -842,7 +883,7 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
         float4 rj[U32];
         float2 vv[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.x ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
+        for (int u = 0; u < U32; ++u) t[u] = (int)nb[(size_t)min(k0 + u, c.x - 1) * LIST_TILE];
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -888,7 +929,7 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
         float4 rj[U32];
         float2 vv[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < c.y ? (int)nb[(size_t)(A.lcap - 1 - (k0 + u)) * LIST_TILE] : s;
+        for (int u = 0; u < U32; ++u) t[u] = (int)nb[(size_t)(A.lcap - 1 - min(k0 + u, c.y - 1)) * LIST_TILE];
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -986,7 +1027,7 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
         float4 rj[U32];
         float2 vv[U32], vt[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
+        for (int u = 0; u < U32; ++u) t[u] = (int)nb[(size_t)min(k0 + u, ns - 1) * LIST_TILE];
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -1080,7 +1121,7 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
         int t[U32];
         float4 rj[U32], Sj[U32], vq[U32];
 #pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = k0 + u < ns ? (int)nb[(size_t)(k0 + u) * LIST_TILE] : s;
+        for (int u = 0; u < U32; ++u) t[u] = (int)nb[(size_t)min(k0 + u, ns - 1) * LIST_TILE];
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
