@@ -845,19 +845,22 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
 // plus the cell offset (with the periodic wrap of the cell index) is exact in
 // an int, and so is the minimum image of an axis with a single cell (period
 // 2^29 units); one FP64 scale and one rounding to FP32 at the end.
+template <bool WRAP>
 __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const BoxF &F, float &dx, float &dy)
 {
     const int ci = __float_as_int(ri.w), cj = __float_as_int(rj.w);
     int ox = (cj & 0xffff) - (ci & 0xffff), oy = (cj >> 16) - (ci >> 16);
-    ox = ox > 1 ? ox - F.ncx : (ox < -1 ? ox + F.ncx : ox);
-    oy = oy > 1 ? oy - F.ncy : (oy < -1 ? oy + F.ncy : oy);
+    if (WRAP) {
+        ox = ox > 1 ? ox - F.ncx : (ox < -1 ? ox + F.ncx : ox);
+        oy = oy > 1 ? oy - F.ncy : (oy < -1 ? oy + F.ncy : oy);
+    }
     int ix = (__float_as_int(rj.x) - __float_as_int(ri.x)) + ox * (1 << FIX_SHIFT);
     int iy = (__float_as_int(rj.y) - __float_as_int(ri.y)) + oy * (1 << FIX_SHIFT);
-    if (F.sx == 0) {
+    if (WRAP && F.sx == 0) {
         ix = ix > (1 << (FIX_SHIFT - 1)) ? ix - (1 << FIX_SHIFT) : ix;
         ix = ix < -(1 << (FIX_SHIFT - 1)) ? ix + (1 << FIX_SHIFT) : ix;
     }
-    if (F.sy == 0) {
+    if (WRAP && F.sy == 0) {
         iy = iy > (1 << (FIX_SHIFT - 1)) ? iy - (1 << FIX_SHIFT) : iy;
         iy = iy < -(1 << (FIX_SHIFT - 1)) ? iy + (1 << FIX_SHIFT) : iy;
     }
@@ -865,13 +868,28 @@ __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const 
     dy = (float)((double)iy * F.scly);
 }
 
+// True when every active lane's build cell is at least one cell from each
+// side: the stencil of such a cell needs no wrap, so every listed
+// neighbour's cell offset is already -1, 0 or 1 and sep32<false> gives the
+// same separations as sep32<true>.
+__device__ __forceinline__ bool wave_interior32(const float4 &ri, const BoxF &F)
+{
+    const int c = __float_as_int(ri.w), cx = c & 0xffff, cy = c >> 16;
+    const bool in = cx >= 1 && cx <= F.ncx - 2 && cy >= 1 && cy <= F.ncy - 2;
+    return __ballot(!in) == 0ull;
+}
+
 struct Dens32 {
     Kahan rho, rhol, gx, gy, ev;
 };
 
+// The FP32 walks take pair distances from the hardware square root
+// (v_sqrt_f32, about 1 ulp; the correctly rounded sequence costs ~14 more
+// instructions per entry and moved no error against the FP64 oracle).
+
 // FP32 sums: the terms of one unrolled round (U32 list entries) are added in
 // FP32, and each round's partial sum goes into a Kahan-compensated total.
-template <int U32, bool COH>
+template <int U32, bool COH, bool WRAP>
 __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F, int s, Dens32 &d)
 {
     const float4 ri = A.rf[s];
@@ -893,10 +911,10 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
-            sep32(ri, rj[u], F, dx, dy);
+            sep32<WRAP>(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
             if (k0 + u < c.x && r2 < F.cut2) {
-                const float rr = sqrtf(r2);
+                const float rr = __builtin_amdgcn_sqrtf(r2);
                 if (r2 < F.cutl) {
                     const float q = rr * F.ihl, w = 1.f - q;
                     grhol += rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w));
@@ -939,10 +957,10 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
-            sep32(ri, rj[u], F, dx, dy);
+            sep32<WRAP>(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
             if (k0 + u < c.y && r2 < F.cutl) {
-                const float rr = sqrtf(r2);
+                const float rr = __builtin_amdgcn_sqrtf(r2);
                 const float q = rr * F.ihl, w = 1.f - q;
                 grhol += rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w));
                 if (COH) {
@@ -981,8 +999,13 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
         const BoxF F = boxf(A.boxes[bu], true);
         d.rho = Kahan(F.qh);
         d.rhol = Kahan(F.qhl);
-        if (F.terms & 2) density32_walk<U32, true>(A, F, s, d);
-        else density32_walk<U32, false>(A, F, s, d);
+        if (wave_interior32(A.rf[s], F)) {
+            if (F.terms & 2) density32_walk<U32, true, false>(A, F, s, d);
+            else density32_walk<U32, false, false>(A, F, s, d);
+        } else {
+            if (F.terms & 2) density32_walk<U32, true, true>(A, F, s, d);
+            else density32_walk<U32, false, true>(A, F, s, d);
+        }
     });
     const float2 vi = A.vf[s];
     const double rd = d.rho.value(), rl = d.rhol.value();
@@ -1014,7 +1037,7 @@ struct Grad32 {
     Kahan g00, g01, g10, g11, gt0, gt1;
 };
 
-template <int U32>
+template <int U32, bool WRAP>
 __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, int s, Grad32 &g)
 {
     const float4 ri = A.rf[s];
@@ -1038,10 +1061,10 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
-            sep32(ri, rj[u], F, dx, dy);
+            sep32<WRAP>(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
             if (k0 + u < ns && r2 < F.cuts) {
-                const float rr = sqrtf(r2);
+                const float rr = __builtin_amdgcn_sqrtf(r2);
                 const float q = rr * F.ih, w = 1.f - q;
                 const float Fk = F.fh * w * w;
                 const float gx = -Fk * dx, gy = -Fk * dy;
@@ -1078,7 +1101,11 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
     const BoxDesc &B = A.boxes[b];
     if (!diag && !(B.terms & 12u)) return;
     Grad32 g;
-    per_box(b, [&](int bu) { grad32_walk<U32>(A, boxf(A.boxes[bu], true), s, g); });
+    per_box(b, [&](int bu) {
+        const BoxF F = boxf(A.boxes[bu], true);
+        if (wave_interior32(A.rf[s], F)) grad32_walk<U32, false>(A, F, s, g);
+        else grad32_walk<U32, true>(A, F, s, g);
+    });
     const float2 vi = A.vf[s];
     const double G00 = g.g00.value(), G01 = g.g01.value(), G10 = g.g10.value(), G11 = g.g11.value();
     const double rho = A.rho[s].x, r2i = rho * rho;
@@ -1110,7 +1137,7 @@ struct Rate32 {
     Kahan ax, ay, ud;
 };
 
-template <int U32, bool COND>
+template <int U32, bool COND, bool WRAP>
 __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, int s, int ns, Rate32 &o)
 {
     const float4 ri = A.rf[s];
@@ -1132,10 +1159,10 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
-            sep32(ri, rj[u], F, dx, dy);
+            sep32<WRAP>(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
             if (k0 + u < ns && r2 < F.cuts) {
-                const float rr = sqrtf(r2);
+                const float rr = __builtin_amdgcn_sqrtf(r2);
                 const float q = rr * F.ih, w = 1.f - q;
                 const float Fk = F.fh * w * w;
                 const float wx = Fk * dx, wy = Fk * dy;
@@ -1171,8 +1198,13 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
     Rate32 o;
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
-        if (F.terms & 8) rates32_walk<U32, true>(A, F, s, ns, o);
-        else rates32_walk<U32, false>(A, F, s, ns, o);
+        if (wave_interior32(A.rf[s], F)) {
+            if (F.terms & 8) rates32_walk<U32, true, false>(A, F, s, ns, o);
+            else rates32_walk<U32, false, false>(A, F, s, ns, o);
+        } else {
+            if (F.terms & 8) rates32_walk<U32, true, true>(A, F, s, ns, o);
+            else rates32_walk<U32, false, true>(A, F, s, ns, o);
+        }
     });
     const double2 ac = A.aco[s];
     const double axd = o.ax.value() + ac.x, ayd = o.ay.value() + ac.y, udd = o.ud.value() + A.uco[s];
