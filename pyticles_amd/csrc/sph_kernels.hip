@@ -410,7 +410,7 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
 struct BoxK {
     double Lx, Ly, hLx, hLy;
     double cut2, h2, hl2, cutl, cuth;  // cutl = min(cut2, hl2), cuth = min(cut2, h2)
-    double ih, ihl, qh, qhl, fh, fhl, munif, rin;
+    double ih, ihl, qh, qhl, fh, fhl, munif, rin, iLx, iLy;
     int terms;
 };
 
@@ -424,6 +424,8 @@ __device__ __forceinline__ BoxK box_k(const BoxDesc &B, bool uniform)
     k.ih = B.ih; k.ihl = B.ihl; k.qh = B.qh; k.qhl = B.qhl; k.fh = B.fh; k.fhl = B.fhl;
     k.munif = B.munif;
     k.rin = B.rin;
+    k.iLx = 1.0 / B.Lx;
+    k.iLy = 1.0 / B.Ly;
     k.terms = B.terms;
     if (uniform) {
         k.Lx = wave_uniform(k.Lx); k.Ly = wave_uniform(k.Ly); k.hLx = wave_uniform(k.hLx); k.hLy = wave_uniform(k.hLy);
@@ -431,7 +433,7 @@ __device__ __forceinline__ BoxK box_k(const BoxDesc &B, bool uniform)
         k.cutl = wave_uniform(k.cutl); k.cuth = wave_uniform(k.cuth);
         k.ih = wave_uniform(k.ih); k.ihl = wave_uniform(k.ihl); k.qh = wave_uniform(k.qh); k.qhl = wave_uniform(k.qhl);
         k.fh = wave_uniform(k.fh); k.fhl = wave_uniform(k.fhl); k.munif = wave_uniform(k.munif);
-        k.rin = wave_uniform(k.rin);
+        k.rin = wave_uniform(k.rin); k.iLx = wave_uniform(k.iLx); k.iLy = wave_uniform(k.iLy);
         k.terms = wave_uniform(k.terms);
     }
     return k;
@@ -449,6 +451,17 @@ __device__ __forceinline__ bool wave_interior(double2 r, const BoxK &K)
 {
     const bool in = r.x >= K.rin && r.x <= K.Lx - K.rin && r.y >= K.rin && r.y <= K.Ly - K.rin;
     return __ballot(!in) == 0ull;
+}
+
+// Minimum image of a listed pair's raw separation.  A listed pair is closer
+// than rin / 2, so the raw separation lies within rin / 2 of 0 or of +-L,
+// where rint(dx / L) is exactly 0 or +-1 and dx - k L is one rounding of
+// the same difference min_image forms: the same bits in 3 operations per
+// axis instead of a compare and two selects per side.
+__device__ __forceinline__ void min_image_listed(double &dx, double &dy, const BoxK &K)
+{
+    dx = fma(-K.Lx, rint(dx * K.iLx), dx);
+    dy = fma(-K.Ly, rint(dy * K.iLy), dy);
 }
 
 // sqrt(x) for x >= 0 in a few FMAs: the hardware reciprocal square root,
@@ -507,7 +520,7 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            if (WRAP) min_image(dx, dy, K);
+            if (WRAP) min_image_listed(dx, dy, K);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < c.x && r2 < K.cut2) {
                 const double m = UNIM ? K.munif : mj[u];
@@ -547,7 +560,7 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            if (WRAP) min_image(dx, dy, K);
+            if (WRAP) min_image_listed(dx, dy, K);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < c.y && r2 < K.cutl) {
                 const double m = UNIM ? K.munif : mj[u];
@@ -650,7 +663,7 @@ __device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K,
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            if (WRAP) min_image(dx, dy, K);
+            if (WRAP) min_image_listed(dx, dy, K);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < ns && r2 < K.cuth) {
                 const double rr = sqrt_pair(r2);
@@ -744,7 +757,7 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
-            if (WRAP) min_image(dx, dy, K);
+            if (WRAP) min_image_listed(dx, dy, K);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < ns && r2 < K.cuth) {
                 const double rr = sqrt_pair(r2);
@@ -845,7 +858,9 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
 // plus the cell offset (with the periodic wrap of the cell index) is exact in
 // an int, and so is the minimum image of an axis with a single cell (period
 // 2^29 units); one FP64 scale and one rounding to FP32 at the end.
-template <bool WRAP>
+// WRAP: 0 = interior wave (no wrap), 1 = wrap of cell offsets, 2 = also the
+// fixed-point wrap of single-cell axes.
+template <int WRAP>
 __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const BoxF &F, float &dx, float &dy)
 {
     const int ci = __float_as_int(ri.w), cj = __float_as_int(rj.w);
@@ -856,11 +871,11 @@ __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const 
     }
     int ix = (__float_as_int(rj.x) - __float_as_int(ri.x)) + ox * (1 << FIX_SHIFT);
     int iy = (__float_as_int(rj.y) - __float_as_int(ri.y)) + oy * (1 << FIX_SHIFT);
-    if (WRAP && F.sx == 0) {
+    if (WRAP == 2 && F.sx == 0) {
         ix = ix > (1 << (FIX_SHIFT - 1)) ? ix - (1 << FIX_SHIFT) : ix;
         ix = ix < -(1 << (FIX_SHIFT - 1)) ? ix + (1 << FIX_SHIFT) : ix;
     }
-    if (WRAP && F.sy == 0) {
+    if (WRAP == 2 && F.sy == 0) {
         iy = iy > (1 << (FIX_SHIFT - 1)) ? iy - (1 << FIX_SHIFT) : iy;
         iy = iy < -(1 << (FIX_SHIFT - 1)) ? iy + (1 << FIX_SHIFT) : iy;
     }
@@ -868,15 +883,17 @@ __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const 
     dy = (float)((double)iy * F.scly);
 }
 
-// True when every active lane's build cell is at least one cell from each
-// side: the stencil of such a cell needs no wrap, so every listed
-// neighbour's cell offset is already -1, 0 or 1 and sep32<false> gives the
-// same separations as sep32<true>.
-__device__ __forceinline__ bool wave_interior32(const float4 &ri, const BoxF &F)
+// Wrap mode of sep32 for the active lanes: 0 when every lane's build cell
+// is at least one cell from each side (the stencil of such a cell needs no
+// wrap, so every listed neighbour's cell offset is already -1, 0 or 1 and
+// the separations are the same as with the wrap), 2 for a box with a
+// single-cell axis, else 1.
+__device__ __forceinline__ int wave_wrap32(const float4 &ri, const BoxF &F)
 {
     const int c = __float_as_int(ri.w), cx = c & 0xffff, cy = c >> 16;
     const bool in = cx >= 1 && cx <= F.ncx - 2 && cy >= 1 && cy <= F.ncy - 2;
-    return __ballot(!in) == 0ull;
+    if (__ballot(!in) == 0ull) return 0;
+    return (F.sx == 0 || F.sy == 0) ? 2 : 1;
 }
 
 struct Dens32 {
@@ -889,7 +906,7 @@ struct Dens32 {
 
 // FP32 sums: the terms of one unrolled round (U32 list entries) are added in
 // FP32, and each round's partial sum goes into a Kahan-compensated total.
-template <int U32, bool COH, bool WRAP>
+template <int U32, bool COH, int WRAP>
 __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F, int s, Dens32 &d)
 {
     const float4 ri = A.rf[s];
@@ -999,12 +1016,15 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
         const BoxF F = boxf(A.boxes[bu], true);
         d.rho = Kahan(F.qh);
         d.rhol = Kahan(F.qhl);
-        if (wave_interior32(A.rf[s], F)) {
-            if (F.terms & 2) density32_walk<U32, true, false>(A, F, s, d);
-            else density32_walk<U32, false, false>(A, F, s, d);
+        const int wrap = wave_wrap32(A.rf[s], F);
+        if (F.terms & 2) {
+            if (wrap == 0) density32_walk<U32, true, 0>(A, F, s, d);
+            else if (wrap == 1) density32_walk<U32, true, 1>(A, F, s, d);
+            else density32_walk<U32, true, 2>(A, F, s, d);
         } else {
-            if (F.terms & 2) density32_walk<U32, true, true>(A, F, s, d);
-            else density32_walk<U32, false, true>(A, F, s, d);
+            if (wrap == 0) density32_walk<U32, false, 0>(A, F, s, d);
+            else if (wrap == 1) density32_walk<U32, false, 1>(A, F, s, d);
+            else density32_walk<U32, false, 2>(A, F, s, d);
         }
     });
     const float2 vi = A.vf[s];
@@ -1037,7 +1057,7 @@ struct Grad32 {
     Kahan g00, g01, g10, g11, gt0, gt1;
 };
 
-template <int U32, bool WRAP>
+template <int U32, int WRAP>
 __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, int s, Grad32 &g)
 {
     const float4 ri = A.rf[s];
@@ -1103,8 +1123,10 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
     Grad32 g;
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
-        if (wave_interior32(A.rf[s], F)) grad32_walk<U32, false>(A, F, s, g);
-        else grad32_walk<U32, true>(A, F, s, g);
+        const int wrap = wave_wrap32(A.rf[s], F);
+        if (wrap == 0) grad32_walk<U32, 0>(A, F, s, g);
+        else if (wrap == 1) grad32_walk<U32, 1>(A, F, s, g);
+        else grad32_walk<U32, 2>(A, F, s, g);
     });
     const float2 vi = A.vf[s];
     const double G00 = g.g00.value(), G01 = g.g01.value(), G10 = g.g10.value(), G11 = g.g11.value();
@@ -1137,7 +1159,7 @@ struct Rate32 {
     Kahan ax, ay, ud;
 };
 
-template <int U32, bool COND, bool WRAP>
+template <int U32, bool COND, int WRAP>
 __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, int s, int ns, Rate32 &o)
 {
     const float4 ri = A.rf[s];
@@ -1198,12 +1220,15 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
     Rate32 o;
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
-        if (wave_interior32(A.rf[s], F)) {
-            if (F.terms & 8) rates32_walk<U32, true, false>(A, F, s, ns, o);
-            else rates32_walk<U32, false, false>(A, F, s, ns, o);
+        const int wrap = wave_wrap32(A.rf[s], F);
+        if (F.terms & 8) {
+            if (wrap == 0) rates32_walk<U32, true, 0>(A, F, s, ns, o);
+            else if (wrap == 1) rates32_walk<U32, true, 1>(A, F, s, ns, o);
+            else rates32_walk<U32, true, 2>(A, F, s, ns, o);
         } else {
-            if (F.terms & 8) rates32_walk<U32, true, true>(A, F, s, ns, o);
-            else rates32_walk<U32, false, true>(A, F, s, ns, o);
+            if (wrap == 0) rates32_walk<U32, false, 0>(A, F, s, ns, o);
+            else if (wrap == 1) rates32_walk<U32, false, 1>(A, F, s, ns, o);
+            else rates32_walk<U32, false, 2>(A, F, s, ns, o);
         }
     });
     const double2 ac = A.aco[s];
