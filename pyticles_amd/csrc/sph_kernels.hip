@@ -307,6 +307,60 @@ __global__ __launch_bounds__(NT) void k_cell_start(ArenaView A, int force_mask)
     }
 }
 
+// Candidate scan of one building slot over the stencil cells of its build
+// cell, in stencil order and slot order within a cell (the list order).
+// Four candidate positions are loaded before any is tested; SINGLE: a box
+// with a single-cell axis, whose candidates need the minimum image.
+template <bool SINGLE>
+__device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu, int s, double2 ri, int cell,
+                                          unsigned *nbw, int &ns, int &nl)
+{
+    const int lcap = A.lcap;
+    const double rnl2 = wave_uniform(Bu.rnl2), hsk2 = wave_uniform(Bu.hsk2);
+    const double Lx = wave_uniform(Bu.Lx), Ly = wave_uniform(Bu.Ly);
+    const double hLx = wave_uniform(Bu.hLx), hLy = wave_uniform(Bu.hLy);
+    const int ncx = wave_uniform(Bu.ncx), ncy = wave_uniform(Bu.ncy);
+    const int sx = wave_uniform(Bu.sx), sy = wave_uniform(Bu.sy);
+    const int *cs = A.cell_start + wave_uniform(Bu.key_base);
+    const int cx = cell % ncx, cy = cell / ncx;
+    for (int oy = -sy; oy <= sy; ++oy) {
+        const int yr = cy + oy;
+        const int yy = yr < 0 ? yr + ncy : (yr >= ncy ? yr - ncy : yr);
+        const double ry = ri.y - (yr < 0 ? -Ly : (yr >= ncy ? Ly : 0.0));
+        for (int ox = -sx; ox <= sx; ++ox) {
+            const int xr = cx + ox;
+            const int xx = xr < 0 ? xr + ncx : (xr >= ncx ? xr - ncx : xr);
+            const double rx = ri.x - (xr < 0 ? -Lx : (xr >= ncx ? Lx : 0.0));
+            const int c = yy * ncx + xx;
+            const int t0 = cs[c], t1 = cs[c + 1];
+            for (int t = t0; t < t1; t += 4) {
+                double2 rj[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) rj[u] = A.r_s[min(t + u, t1 - 1)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    double dx = rj[u].x - rx, dy = rj[u].y - ry;
+                    if (SINGLE && sx == 0) {
+                        dx = dx > hLx ? dx - Lx : dx;
+                        dx = dx < -hLx ? dx + Lx : dx;
+                    }
+                    if (SINGLE && sy == 0) {
+                        dy = dy > hLy ? dy - Ly : dy;
+                        dy = dy < -hLy ? dy + Ly : dy;
+                    }
+                    const double r2 = rsq_exact(dx, dy);
+                    if (t + u < t1 && r2 < rnl2 && t + u != s) {
+                        const bool sh = r2 < hsk2;
+                        if (ns + nl < lcap) nbw[(size_t)(sh ? ns : lcap - 1 - nl) * LIST_TILE] = (unsigned)(t + u);
+                        ns += sh ? 1 : 0;
+                        nl += sh ? 0 : 1;
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Neighbour list of a building slot: every other slot of the 3x3 cell
 // stencil with r^2 < (r_cut + skin)^2, short entries (r^2 < (h + skin)^2)
 // from the front, the rest from the back.  The sorted state is copied back
@@ -351,44 +405,8 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
     // needs no minimum image (axes with a single cell keep it).
     per_box(b, [&](int bu) {
         const BoxDesc &Bu = A.boxes[bu];
-        const double rnl2 = wave_uniform(Bu.rnl2), hsk2 = wave_uniform(Bu.hsk2);
-        const double Lx = wave_uniform(Bu.Lx), Ly = wave_uniform(Bu.Ly);
-        const double hLx = wave_uniform(Bu.hLx), hLy = wave_uniform(Bu.hLy);
-        const int ncx = wave_uniform(Bu.ncx), ncy = wave_uniform(Bu.ncy);
-        const int sx = wave_uniform(Bu.sx), sy = wave_uniform(Bu.sy);
-        const int *cs = A.cell_start + wave_uniform(Bu.key_base);
-        const int cx = cell % ncx, cy = cell / ncx;
-        for (int oy = -sy; oy <= sy; ++oy) {
-            const int yr = cy + oy;
-            const int yy = yr < 0 ? yr + ncy : (yr >= ncy ? yr - ncy : yr);
-            const double ry = ri.y - (yr < 0 ? -Ly : (yr >= ncy ? Ly : 0.0));
-            for (int ox = -sx; ox <= sx; ++ox) {
-                const int xr = cx + ox;
-                const int xx = xr < 0 ? xr + ncx : (xr >= ncx ? xr - ncx : xr);
-                const double rx = ri.x - (xr < 0 ? -Lx : (xr >= ncx ? Lx : 0.0));
-                const int c = yy * ncx + xx;
-                const int t1 = cs[c + 1];
-                for (int t = cs[c]; t < t1; ++t) {
-                    const double2 rj = A.r_s[t];
-                    double dx = rj.x - rx, dy = rj.y - ry;
-                    if (sx == 0) {
-                        dx = dx > hLx ? dx - Lx : dx;
-                        dx = dx < -hLx ? dx + Lx : dx;
-                    }
-                    if (sy == 0) {
-                        dy = dy > hLy ? dy - Ly : dy;
-                        dy = dy < -hLy ? dy + Ly : dy;
-                    }
-                    const double r2 = rsq_exact(dx, dy);
-                    if (r2 < rnl2 && t != s) {
-                        const bool sh = r2 < hsk2;
-                        if (ns + nl < lcap) nbw[(size_t)(sh ? ns : lcap - 1 - nl) * LIST_TILE] = (unsigned)t;
-                        ns += sh ? 1 : 0;
-                        nl += sh ? 0 : 1;
-                    }
-                }
-            }
-        }
+        if (wave_uniform(Bu.sx) == 0 || wave_uniform(Bu.sy) == 0) list_walk<true>(A, Bu, s, ri, cell, nbw, ns, nl);
+        else list_walk<false>(A, Bu, s, ri, cell, nbw, ns, nl);
     });
     if (ns + nl > lcap) {
         flag_health(A.dyn, b, HEALTH_LISTCAP);
