@@ -105,21 +105,32 @@ def pcie_inclusive(s, steps, unroll):
             "includes": "host->device r, v, m; initial density pass; %d steps; device->host r, v, u" % steps}
 
 
-def run_gpu(s, steps, warmup, timing=True, unroll=0):
+def run_gpu(s, steps, warmup, timing=True, unroll=0, kernel_steps=200):
+    """Warm up, then time `steps` steps as the library runs them (captured
+    step graphs replayed, no events).  With timing, a second region of
+    min(steps, kernel_steps) steps follows with HIP events recorded on the
+    context stream around every pass launch (direct launches: events cannot
+    sit between the kernels of a replayed graph); its per-pass times are
+    returned in tm with tm["steps"] = the steps of that region."""
     from pyticles_amd import _lib
     b = device_box(s, unroll)
     if warmup:
         b.step(warmup)
-    _lib.check(_lib.lib.sph_ctx_set_timing(b.ctx, 1 if timing else 0))
-    b.timing(reset=True)
     b.energy()  # synchronises the stream
     E0 = b.energy()[2]
     t0 = time.perf_counter()
     b.step(steps)  # returns after the stream is drained (health check)
     t1 = time.perf_counter()
     E1 = b.energy()[2]
-    tm = b.timing()
-    _lib.check(_lib.lib.sph_ctx_set_timing(b.ctx, 0))
+    tm = None
+    if timing:
+        ks = min(steps, kernel_steps)
+        _lib.check(_lib.lib.sph_ctx_set_timing(b.ctx, 1))
+        b.timing(reset=True)
+        b.step(ks)
+        tm = b.timing()
+        tm["steps"] = ks
+        _lib.check(_lib.lib.sph_ctx_set_timing(b.ctx, 0))
     return b, t1 - t0, tm, (E0, E1)
 
 
@@ -390,7 +401,8 @@ def main():
     b.close()
     value = s.n * args.steps / el
     evals = args.steps + 1
-    per = {k: tm[k + "_ms"] / evals for k in ("build", "density", "gradients", "rates", "integrate")}
+    kevals = int(tm["steps"]) + 1  # rate evaluations of the kernel-timing region
+    per = {k: tm[k + "_ms"] / kevals for k in ("build", "density", "gradients", "rates", "integrate")}
     # the dominant pair-pass kernel by measured time
     dom = max(("density", "gradients", "rates"), key=lambda k: per[k])
     alg_bytes = bps[dom] * s.n
@@ -400,7 +412,9 @@ def main():
     roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,
                 kernel=kname, kernel_avg_ms=round(per[dom], 5), alg_bytes_per_launch=alg_bytes,
-                alg_bytes_per_particle=bps[dom])
+                alg_bytes_per_particle=bps[dom],
+                timing="HIP events on the context stream around every launch, over %d further steps right "
+                       "after the timed region (direct launches)" % int(tm["steps"]))
     step_bytes = sum(bps.values())
     line = {
         "metric": "particle-steps/s",
