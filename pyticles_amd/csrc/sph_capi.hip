@@ -220,6 +220,7 @@ struct Arena {
         cap = capacity;
         precision = prec;
         lcap = list_entries;
+        if (lcap <= 0 || lcap % 4) return fail(SPH_ERR_ARG, "list entries must be a positive multiple of 4");
         int rc = 0;
         rc |= dalloc(&r, cap); rc |= dalloc(&v, cap); rc |= dalloc(&vp, cap); rc |= dalloc(&acc, cap);
         rc |= dalloc(&u, cap); rc |= dalloc(&m, cap); rc |= dalloc(&up, cap); rc |= dalloc(&udot, cap);

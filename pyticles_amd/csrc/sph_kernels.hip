@@ -44,15 +44,60 @@ __device__ __forceinline__ int xcd_block()
 
 __device__ __forceinline__ int wrapi(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
-// Neighbour lists in tiles of 64 slots (one wavefront): entry k of slot s
-// at ((s / 64) * lcap + k) * 64 + s % 64.  At list step k a wave reads 256
-// consecutive bytes, and everything one wave reads or writes lies in one
-// 256 * lcap byte tile (pages and L2 lines stay with the wave).
+// Neighbour lists in tiles of 64 slots (one wavefront), entries in groups
+// of four.  Short entry k sits at position P = k, long entry k at
+// P = lcap - 1 - k (lcap a multiple of 4), and position P of slot s at word
+//   ((s / 64) * (lcap / 4) + P / 4) * 256 + (s % 64) * 4 + P % 4.
+// A slot's four entries of a group are one 16-byte word and a wave's 64
+// groups of one row are 1 KB contiguous: the walks fetch up to four entries
+// per lane with one load and the build stores four at a time.  Everything a
+// wave reads or writes lies in one 256 * lcap byte tile.
 static constexpr int LIST_TILE = 64;
+static constexpr int LIST_GROUP = 4;
+static constexpr int LIST_ROW = LIST_TILE * LIST_GROUP;  // words per group row of a tile
 
-__device__ __forceinline__ const unsigned *list_of(const ArenaView &A, int s)
+__device__ __forceinline__ unsigned *list_base(const ArenaView &A, int s)
 {
-    return A.nbr + (((size_t)(s / LIST_TILE) * A.lcap) * LIST_TILE + (s % LIST_TILE));
+    return A.nbr + (size_t)(s / LIST_TILE) * (A.lcap / LIST_GROUP) * LIST_ROW + (s % LIST_TILE) * LIST_GROUP;
+}
+
+// Entries at positions P0 .. P0 + U - 1 of a slot (U = 1, 2 or 4, P0 a
+// multiple of U): one load.
+template <int U>
+__device__ __forceinline__ void load_entries(const unsigned *base, int P0, unsigned (&w)[U])
+{
+    const unsigned *p = base + (size_t)(P0 / LIST_GROUP) * LIST_ROW + (P0 % LIST_GROUP);
+    if constexpr (U == 4) {
+        const uint4 g = *(const uint4 *)p;
+        w[0] = g.x; w[1] = g.y; w[2] = g.z; w[3] = g.w;
+    } else if constexpr (U == 2) {
+        const uint2 g = *(const uint2 *)p;
+        w[0] = g.x; w[1] = g.y;
+    } else {
+        w[0] = *p;
+    }
+}
+
+// Slots of the round of short entries k0 .. k0 + U - 1 (k0 a multiple of U);
+// entries past the count n read as s (a valid record whose terms the
+// callers mask off), so the gathers need no branch.
+template <int U>
+__device__ __forceinline__ void short_round(const unsigned *base, int k0, int n, int s, int (&t)[U])
+{
+    unsigned w[U];
+    load_entries<U>(base, k0, w);
+#pragma unroll
+    for (int u = 0; u < U; ++u) t[u] = k0 + u < n ? (int)w[u] : s;
+}
+
+// The same for long entries k0 .. k0 + U - 1, at positions lcap - 1 - k.
+template <int U>
+__device__ __forceinline__ void long_round(const unsigned *base, int lcap, int k0, int n, int s, int (&t)[U])
+{
+    unsigned w[U];
+    load_entries<U>(base, lcap - k0 - U, w);
+#pragma unroll
+    for (int u = 0; u < U; ++u) t[u] = k0 + u < n ? (int)w[U - 1 - u] : s;
 }
 
 // Minimum image exactly as neighbour_list.py:110-123 (all '>' tests, then '<').
@@ -311,9 +356,13 @@ __global__ __launch_bounds__(NT) void k_cell_start(ArenaView A, int force_mask)
 // cell, in stencil order and slot order within a cell (the list order).
 // Four candidate positions are loaded before any is tested; SINGLE: a box
 // with a single-cell axis, whose candidates need the minimum image.
+// Entries collect in two four-entry buffers (short from position 0 up, long
+// from lcap - 1 down) stored a group at a time; the partial groups are
+// stored at the end, merged when they share a group.  Entries that do not
+// fit (ns + nl = lcap) set lost.
 template <bool SINGLE>
 __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu, int s, double2 ri, int cell,
-                                          unsigned *nbw, int &ns, int &nl)
+                                          unsigned *lb, int &ns, int &nl, bool &lost)
 {
     const int lcap = A.lcap;
     const double rnl2 = wave_uniform(Bu.rnl2), hsk2 = wave_uniform(Bu.hsk2);
@@ -323,6 +372,7 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
     const int sx = wave_uniform(Bu.sx), sy = wave_uniform(Bu.sy);
     const int *cs = A.cell_start + wave_uniform(Bu.key_base);
     const int cx = cell % ncx, cy = cell / ncx;
+    unsigned s0 = 0, s1 = 0, s2 = 0, s3 = 0, l0 = 0, l1 = 0, l2 = 0, l3 = 0;
     for (int oy = -sy; oy <= sy; ++oy) {
         const int yr = cy + oy;
         const int yy = yr < 0 ? yr + ncy : (yr >= ncy ? yr - ncy : yr);
@@ -350,14 +400,42 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
                     }
                     const double r2 = rsq_exact(dx, dy);
                     if (t + u < t1 && r2 < rnl2 && t + u != s) {
-                        const bool sh = r2 < hsk2;
-                        if (ns + nl < lcap) nbw[(size_t)(sh ? ns : lcap - 1 - nl) * LIST_TILE] = (unsigned)(t + u);
-                        ns += sh ? 1 : 0;
-                        nl += sh ? 0 : 1;
+                        const unsigned e = (unsigned)(t + u);
+                        if (ns + nl >= lcap) {
+                            lost = true;
+                        } else if (r2 < hsk2) {  // short: offset ns % 4 of group ns / 4
+                            const int o = ns & 3;
+                            s0 = o == 0 ? e : s0;
+                            s1 = o == 1 ? e : s1;
+                            s2 = o == 2 ? e : s2;
+                            s3 = o == 3 ? e : s3;
+                            ++ns;
+                            if (o == 3) *(uint4 *)(lb + (size_t)((ns - 4) >> 2) * LIST_ROW) = make_uint4(s0, s1, s2, s3);
+                        } else {  // long entry nl at position lcap - 1 - nl: offset 3 - nl % 4
+                            const int o = 3 - (nl & 3);
+                            l0 = o == 0 ? e : l0;
+                            l1 = o == 1 ? e : l1;
+                            l2 = o == 2 ? e : l2;
+                            l3 = o == 3 ? e : l3;
+                            ++nl;
+                            if (o == 0) *(uint4 *)(lb + (size_t)((lcap - nl) >> 2) * LIST_ROW) = make_uint4(l0, l1, l2, l3);
+                        }
                     }
                 }
             }
         }
+    }
+    // partial groups: short offsets [0, ns % 4) of group ns / 4, long
+    // offsets [4 - nl % 4, 4) of group (lcap - nl) / 4; with ns + nl <= lcap
+    // they never overlap, and a shared group is stored once
+    const int ps = ns & 3, pl = nl & 3;
+    const int gs = ns >> 2, gl = (lcap - nl) >> 2;
+    if (ps && pl && gs == gl) {
+        *(uint4 *)(lb + (size_t)gs * LIST_ROW) =
+            make_uint4(0 < ps ? s0 : l0, 1 < ps ? s1 : l1, 2 < ps ? s2 : l2, 3 < ps ? s3 : l3);
+    } else {
+        if (ps) *(uint4 *)(lb + (size_t)gs * LIST_ROW) = make_uint4(s0, s1, s2, s3);
+        if (pl) *(uint4 *)(lb + (size_t)gl * LIST_ROW) = make_uint4(l0, l1, l2, l3);
     }
 }
 
@@ -397,23 +475,18 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
         A.rf[s] = fix_pos(ri, cell, mi, B);
         A.vf[s] = make_float2((float)vpi.x, (float)vpi.y);
     }
-    const int lcap = A.lcap;
-    unsigned *nbw = A.nbr + (list_of(A, s) - A.nbr);
+    unsigned *lb = list_base(A, s);
     int ns = 0, nl = 0;
+    bool lost = false;
     // Box constants wave-uniform (scalar registers); each stencil cell's
     // periodic image is applied to r_i once per cell, so the candidate loop
     // needs no minimum image (axes with a single cell keep it).
     per_box(b, [&](int bu) {
         const BoxDesc &Bu = A.boxes[bu];
-        if (wave_uniform(Bu.sx) == 0 || wave_uniform(Bu.sy) == 0) list_walk<true>(A, Bu, s, ri, cell, nbw, ns, nl);
-        else list_walk<false>(A, Bu, s, ri, cell, nbw, ns, nl);
+        if (wave_uniform(Bu.sx) == 0 || wave_uniform(Bu.sy) == 0) list_walk<true>(A, Bu, s, ri, cell, lb, ns, nl, lost);
+        else list_walk<false>(A, Bu, s, ri, cell, lb, ns, nl, lost);
     });
-    if (ns + nl > lcap) {
-        flag_health(A.dyn, b, HEALTH_LISTCAP);
-        const int keep = ns < lcap ? ns : lcap;
-        nl = lcap - keep;
-        ns = keep;
-    }
+    if (lost) flag_health(A.dyn, b, HEALTH_LISTCAP);
     A.cnt[s] = make_int2(ns, nl);
     max_bits(&D.vmax2, vpi.x * vpi.x + vpi.y * vpi.y);
     max_bits(&D.amax2, ai.x * ai.x + ai.y * ai.y);
@@ -514,21 +587,19 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
 {
     const double2 ri = A.r[s], vi = A.vp[s];
     const int2 c = A.cnt[s];
-    const unsigned *nb = list_of(A, s);
+    const unsigned *nb = list_base(A, s);
     Dens64 d;
     d.rho = K.qh;
     d.rhol = K.qhl;
     d.gx = d.gy = d.ev = 0.0;
     // U64 entries per round: all their loads are issued before any use so
-    // several neighbour fetches are in flight per wavefront.  Rounds past the
-    // count re-read the last entry (a valid neighbour whose terms are masked
-    // off), so the list loads take no branch.
+    // several neighbour fetches are in flight per wavefront; one load brings
+    // a round's list entries.
     for (int k0 = 0; k0 < c.x; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64];
         double mj[U64];
-#pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = (int)nb[(size_t)min(k0 + u, c.x - 1) * LIST_TILE];
+        short_round<U64>(nb, k0, c.x, s, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -567,8 +638,7 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
         int t[U64];
         double2 rj[U64], vv[U64];
         double mj[U64];
-#pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = (int)nb[(size_t)(A.lcap - 1 - min(k0 + u, c.y - 1)) * LIST_TILE];
+        long_round<U64>(nb, A.lcap, k0, c.y, s, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -664,14 +734,13 @@ __device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K,
     const double2 ri = A.r[s], vi = A.vp[s];
     const double Ti = A.T[s];
     const int ns = A.cnt[s].x;
-    const unsigned *nb = list_of(A, s);
+    const unsigned *nb = list_base(A, s);
     Grad64 g;
     g.g00 = g.g01 = g.g10 = g.g11 = g.gt0 = g.gt1 = 0.0;
     for (int k0 = 0; k0 < ns; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64], vt[U64];
-#pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = (int)nb[(size_t)min(k0 + u, ns - 1) * LIST_TILE];
+        short_round<U64>(nb, k0, ns, s, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -756,15 +825,14 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
     const double2 ri = A.r[s], vi = A.vp[s];
     const double4 Si = A.S[s];
     const double2 Qi = COND ? A.Q[s] : make_double2(0.0, 0.0);
-    const unsigned *nb = list_of(A, s);
+    const unsigned *nb = list_base(A, s);
     Rate64 o;
     o.ax = o.ay = o.ud = 0.0;
     for (int k0 = 0; k0 < ns; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64], Qj[U64];
         double4 Sj[U64];
-#pragma unroll
-        for (int u = 0; u < U64; ++u) t[u] = (int)nb[(size_t)min(k0 + u, ns - 1) * LIST_TILE];
+        short_round<U64>(nb, k0, ns, s, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -930,13 +998,12 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
     const float4 ri = A.rf[s];
     const float2 vi = A.vf[s];
     const int2 c = A.cnt[s];
-    const unsigned *nb = list_of(A, s);
+    const unsigned *nb = list_base(A, s);
     for (int k0 = 0; k0 < c.x; k0 += U32) {
         int t[U32];
         float4 rj[U32];
         float2 vv[U32];
-#pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = (int)nb[(size_t)min(k0 + u, c.x - 1) * LIST_TILE];
+        short_round<U32>(nb, k0, c.x, s, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -981,8 +1048,7 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
         int t[U32];
         float4 rj[U32];
         float2 vv[U32];
-#pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = (int)nb[(size_t)(A.lcap - 1 - min(k0 + u, c.y - 1)) * LIST_TILE];
+        long_round<U32>(nb, A.lcap, k0, c.y, s, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -1082,13 +1148,12 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
     const float2 vi = A.vf[s];
     const float Ti = A.vt32[s].y;
     const int ns = A.cnt[s].x;
-    const unsigned *nb = list_of(A, s);
+    const unsigned *nb = list_base(A, s);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32];
         float2 vv[U32], vt[U32];
-#pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = (int)nb[(size_t)min(k0 + u, ns - 1) * LIST_TILE];
+        short_round<U32>(nb, k0, ns, s, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
@@ -1183,12 +1248,11 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
     const float4 ri = A.rf[s];
     const float4 vqi = A.vq32[s];
     const float4 Si = A.S32[s];
-    const unsigned *nb = list_of(A, s);
+    const unsigned *nb = list_base(A, s);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32], Sj[U32], vq[U32];
-#pragma unroll
-        for (int u = 0; u < U32; ++u) t[u] = (int)nb[(size_t)min(k0 + u, ns - 1) * LIST_TILE];
+        short_round<U32>(nb, k0, ns, s, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
