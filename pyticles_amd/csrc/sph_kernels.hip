@@ -22,6 +22,7 @@
 #include "sph_kernels.hpp"
 #include "sph_sort.hpp"
 
+
 namespace sph {
 
 static constexpr int NT = 256;  // work-items per block (4 waves)
@@ -1193,8 +1194,10 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
 
 // P2 in FP32; neighbour record: rf, vf, (m/rho, T).  Writes the rate
 // pass's (v, q/rho^2) record.
+// Five waves per SIMD (94 VGPRs, no spill) instead of the compiler's four:
+// 4% faster at C4.  The other pass kernels gained nothing or spilled.
 template <int U32>
-__global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int diag)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_grad32(ArenaView A, int state_mask, int diag)
 {
     const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
