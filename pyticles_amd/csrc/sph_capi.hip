@@ -105,7 +105,7 @@ static size_t arena_bytes(size_t cap, int prec, int lcap)
     b += (cap + 63) / 64 * 64 * (size_t)lcap * sizeof(unsigned);  // nbr, in tiles of 64 slots
     b += cap * (3 * d2 + 2 * d1 + 2 * sizeof(double4));  // rho Q jq, T e, S gradv
     b += cap * (2 * d2 + d1);                             // vt, aco, uco
-    if (prec == SPH_FP32C) b += cap * (3 * sizeof(float4) + 2 * sizeof(float2));  // rf S32 vq32, vf vt32
+    if (prec == SPH_FP32C) b += cap * (4 * sizeof(float4) + sizeof(float2));  // rf S32 vq32 vt32, vf
     b += cap * 5 * sizeof(unsigned);             // sort keys / values
     b += radix_sort_temp_bytes((int)cap);
     return b;
@@ -133,7 +133,8 @@ struct Arena {
     double *T = nullptr, *e = nullptr;
     double4 *S = nullptr, *gradv = nullptr;
     float4 *rf = nullptr, *S32 = nullptr;
-    float2 *vf = nullptr, *vt32 = nullptr;
+    float2 *vf = nullptr;
+    float4 *vt32 = nullptr;
     float4 *vq32 = nullptr;
     double2 *vt = nullptr, *aco = nullptr;
     double *uco = nullptr;

@@ -18,7 +18,7 @@
 //                    vt (double2: m/rho, T), aco / uco (cohesive rates)
 //   FP32 mirrors     rf (int x, y fixed point relative to the build cell
 //                    corner, float m, cell column|row), vf (float2 v),
-//                    S32 (float4), vq32 (float4: v, q/rho^2), vt32 (m/rho, T)
+//                    S32 (float4), vq32 (float4: v, q/rho^2), vt32 (float4: v, m/rho, T)
 // A box is a contiguous slot range [part_off, part_off + n) plus a key range
 // [key_base, key_base + ncx*ncy] of cells; free slot ranges own one key each
 // so a stable sort over the whole arena never moves a particle across boxes.
@@ -107,7 +107,7 @@ struct ArenaView {
     float2 *vf;
     float4 *S32;
     float4 *vq32;        // (v, q / rho^2): neighbour record of the rate pass
-    float2 *vt32;        // (m / rho, T)
+    float4 *vt32;        // (v, m / rho, T): the gradient pass's neighbour record
     // cells and sort
     int *cell_start;
     unsigned *keys;      // sorted keys (final buffer of the radix sort)

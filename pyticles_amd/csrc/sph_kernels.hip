@@ -1131,7 +1131,7 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     A.S32[s] = make_float4((float)ps, 0.f, (float)ps, 0.f);
     A.Q[s] = make_double2(0.0, 0.0);
     A.vq32[s] = make_float4(vi.x, vi.y, 0.f, 0.f);
-    A.vt32[s] = make_float2((float)(mi / rd), (float)T);
+    A.vt32[s] = make_float4(vi.x, vi.y, (float)(mi / rd), (float)T);
     A.aco[s] = coh ? make_double2(-2.0 * B.a * d.gx.value(), -2.0 * B.a * d.gy.value()) : make_double2(0.0, 0.0);
     A.uco[s] = coh ? -B.a * d.ev.value() : 0.0;
     if (!(rd * B.b < 1.0)) flag_health(A.dyn, b, HEALTH_EOS_POLE);
@@ -1146,19 +1146,18 @@ template <int U32, int WRAP>
 __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, int s, Grad32 &g)
 {
     const float4 ri = A.rf[s];
-    const float2 vi = A.vf[s];
-    const float Ti = A.vt32[s].y;
+    const float4 vti = A.vt32[s];
+    const float2 vi = make_float2(vti.x, vti.y);
+    const float Ti = vti.w;
     const int ns = A.cnt[s].x;
     const unsigned *nb = list_base(A, s);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
-        float4 rj[U32];
-        float2 vv[U32], vt[U32];
+        float4 rj[U32], vt[U32];
         short_round<U32>(nb, k0, ns, s, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
-            vv[u] = A.vf[t[u]];
             vt[u] = A.vt32[t[u]];
         }
         float a00 = 0.f, a01 = 0.f, a10 = 0.f, a11 = 0.f, at0 = 0.f, at1 = 0.f;
@@ -1172,9 +1171,9 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
                 const float q = rr * F.ih, w = 1.f - q;
                 const float Fk = F.fh * w * w;
                 const float gx = -Fk * dx, gy = -Fk * dy;
-                const float vj = vt[u].x;
-                const float dvx = vv[u].x - vi.x, dvy = vv[u].y - vi.y;
-                const float dT = vt[u].y - Ti;
+                const float vj = vt[u].z;
+                const float dvx = vt[u].x - vi.x, dvy = vt[u].y - vi.y;
+                const float dT = vt[u].w - Ti;
                 a00 += vj * dvx * gx;
                 a01 += vj * dvx * gy;
                 a10 += vj * dvy * gx;
@@ -1192,7 +1191,7 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
     }
 }
 
-// P2 in FP32; neighbour record: rf, vf, (m/rho, T).  Writes the rate
+// P2 in FP32; neighbour record: rf and (v, m/rho, T) from P1.  Writes the rate
 // pass's (v, q/rho^2) record.
 // Five waves per SIMD (94 VGPRs, no spill) instead of the compiler's four:
 // 4% faster at C4.  The other pass kernels gained nothing or spilled.
