@@ -77,6 +77,14 @@ const char *sph_version(void);
 /* Per-pass GPU timing with HIP events (off by default; on disables graph
  * replay so every pass can be bracketed). */
 int sph_ctx_set_timing(sph_ctx *ctx, int on);
+/* FP32C position mirror: boxes whose sides are all <= L (default 1024) keep
+ * box-relative 32-bit fixed-point positions with the velocity in the same
+ * 16-byte neighbour record (resolution L / 2^32); larger boxes keep
+ * build-cell-relative positions (2^29 units per cell).  Applies to boxes
+ * created or re-parameterised afterwards, process-wide; L = 0 selects the
+ * cell-relative form for every box.  No counterpart in the reference. */
+int sph_set_fp32_box_relative_max(double L);
+int sph_get_fp32_box_relative_max(double *L);
 
 /* ---- one box ------------------------------------------------------------ */
 /* Device-resident state of n particles.  Replaces the arrays of

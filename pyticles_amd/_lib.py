@@ -58,6 +58,8 @@ _SIGS = {
     "sph_last_error": [],
     "sph_version": [],
     "sph_ctx_set_timing": [_P, ctypes.c_int],
+    "sph_set_fp32_box_relative_max": [ctypes.c_double],
+    "sph_get_fp32_box_relative_max": [_D],
     "sph_box_create": [_P, _I64, ctypes.POINTER(Params), ctypes.POINTER(_P)],
     "sph_box_destroy": [_P],
     "sph_box_set_params": [_P, ctypes.POINTER(Params)],

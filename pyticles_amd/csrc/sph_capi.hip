@@ -56,6 +56,9 @@ void dfree(T *&p)
     p = nullptr;
 }
 
+// Largest box side with the box-relative FP32C mirror (sph_set_fp32_box_relative_max).
+double g_fixbox_max_l = FIXBOX_MAX_L;
+
 int bits_for(int nkeys)
 {
     int b = 1;
@@ -574,6 +577,9 @@ int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
     // listed pairs are closer than r_cut + skin at the build and each end
     // moves at most skin / 2 before the next, so within r_cut + 2 skin
     B.rin = 2.0 * (rcut + p->skin);
+    // FP32C boxes up to FIXBOX_MAX_L per side mirror positions as box-relative
+    // fixed point (2^32 units per side)
+    B.fixbox = (p->precision == SPH_FP32C && p->Lx <= g_fixbox_max_l && p->Ly <= g_fixbox_max_l) ? 1 : 0;
     // cells cover both the pair-export radius and the list radius
     const double Rc = std::max(std::sqrt(B.R2list), rcut + p->skin) * (1.0 + 1e-9);
     int ncx = (int)std::floor(p->Lx / Rc), ncy = (int)std::floor(p->Ly / Rc);
@@ -767,6 +773,20 @@ int sph_ctx_destroy(sph_ctx *ctx)
     dfree(ctx->d_scratch);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    return SPH_OK;
+}
+
+int sph_set_fp32_box_relative_max(double L)
+{
+    if (!(L >= 0.0) || !std::isfinite(L)) return fail(SPH_ERR_ARG, "invalid box side (need a finite L >= 0)");
+    g_fixbox_max_l = L;
+    return SPH_OK;
+}
+
+int sph_get_fp32_box_relative_max(double *L)
+{
+    if (!L) return fail(SPH_ERR_ARG, "null argument");
+    *L = g_fixbox_max_l;
     return SPH_OK;
 }
 

@@ -37,6 +37,11 @@ enum Health : int {
     HEALTH_LISTCAP = 8,     // a particle had more neighbours than the list holds
 };
 
+// Largest box side whose FP32C position mirror is box-relative fixed point
+// (resolution FIXBOX_MAX_L / 2^32 = 2.4e-7); larger boxes use the
+// build-cell-relative mirror.
+constexpr double FIXBOX_MAX_L = 1024.0;
+
 // Static description of one box (host-owned, uploaded when it changes).
 struct BoxDesc {
     double Lx, Ly, hLx, hLy;       // sides and half sides (minimum image)
@@ -53,7 +58,9 @@ struct BoxDesc {
                                    // from every side has no listed neighbour across a side
     int ncx, ncy, sx, sy;          // cell grid and stencil half widths (0/1)
     int key_base, part_off, n, terms;
-    int precision, pad0, pad1, pad2;
+    int precision;
+    int fixbox;                    // FP32C: box-relative 32-bit fixed-point mirror (see fix_rec)
+    int pad1, pad2;
 };
 
 // Dynamic per-box state (device-owned; the host writes single entries

@@ -176,8 +176,20 @@ __device__ __forceinline__ void max_bits(unsigned long long *addr, double x)
 static constexpr double FIX_ONE = 536870912.0;  // 2^29
 static constexpr int FIX_SHIFT = 29;
 
-__device__ __forceinline__ float4 fix_pos(double2 r, int cell, double m, const BoxDesc &B)
+// FP32 neighbour record of a particle.  In a box with fixbox set the record
+// is (x, y, vx, vy) with x, y box-relative fixed point, 2^32 units per side:
+// the 32-bit difference of two coordinates, read as a signed int, is the
+// minimum-image separation in units exactly, so the record needs no cell
+// and carries the velocity instead (one 16-byte gather per neighbour in the
+// density pass).  Otherwise (larger boxes) it is the build-cell-relative
+// form below: (x, y, m, cell).
+__device__ __forceinline__ float4 fix_rec(double2 r, double2 v, int cell, double m, const BoxDesc &B)
 {
+    if (B.fixbox) {
+        const double ux = rint(r.x * (4294967296.0 / B.Lx)), uy = rint(r.y * (4294967296.0 / B.Ly));
+        return make_float4(__uint_as_float((unsigned)(unsigned long long)(long long)ux),
+                           __uint_as_float((unsigned)(unsigned long long)(long long)uy), (float)v.x, (float)v.y);
+    }
     const int cx = cell % B.ncx, cy = cell / B.ncx;
     double xl = r.x - cx * B.cwx, yl = r.y - cy * B.cwy;
     xl = xl > B.hLx ? xl - B.Lx : (xl < -B.hLx ? xl + B.Lx : xl);
@@ -473,7 +485,7 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
     A.r_ref[s] = ri;
     const int cell = A.pcell[s];
     if (fp32) {
-        A.rf[s] = fix_pos(ri, cell, mi, B);
+        A.rf[s] = fix_rec(ri, vpi, cell, mi, B);
         A.vf[s] = make_float2((float)vpi.x, (float)vpi.y);
     }
     unsigned *lb = list_base(A, s);
@@ -917,14 +929,22 @@ struct Kahan {
 struct BoxF {
     double sclx, scly;  // length of one fixed-point unit
     float cut2, cutl, h2, cuts;
-    float ih, ihl, qh, qhl, fh, fhl;
-    int ncx, ncy, sx, sy, terms;
+    float ih, ihl, qh, qhl, fh, fhl, munif;
+    int ncx, ncy, sx, sy, terms, fixbox;
 };
 
 __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
 {
     BoxF f;
-    f.sclx = B.cwx / FIX_ONE; f.scly = B.cwy / FIX_ONE;
+    if (B.fixbox) {
+        f.sclx = B.Lx / 4294967296.0;
+        f.scly = B.Ly / 4294967296.0;
+    } else {
+        f.sclx = B.cwx / FIX_ONE;
+        f.scly = B.cwy / FIX_ONE;
+    }
+    f.munif = (float)B.munif;
+    f.fixbox = B.fixbox;
     f.cut2 = (float)B.cut2; f.cutl = (float)(B.cut2 < B.hl2 ? B.cut2 : B.hl2); f.h2 = (float)B.h2;
     f.cuts = (float)(B.R2list < B.h2 ? B.R2list : B.h2);
     f.ih = (float)B.ih; f.ihl = (float)B.ihl; f.qh = (float)B.qh; f.qhl = (float)B.qhl;
@@ -937,6 +957,7 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
         f.qh = wave_uniform(f.qh); f.qhl = wave_uniform(f.qhl); f.fh = wave_uniform(f.fh); f.fhl = wave_uniform(f.fhl);
         f.ncx = wave_uniform(f.ncx); f.ncy = wave_uniform(f.ncy); f.sx = wave_uniform(f.sx); f.sy = wave_uniform(f.sy);
         f.terms = wave_uniform(f.terms);
+        f.munif = wave_uniform(f.munif); f.fixbox = wave_uniform(f.fixbox);
     }
     return f;
 }
@@ -946,10 +967,18 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
 // an int, and so is the minimum image of an axis with a single cell (period
 // 2^29 units); one FP64 scale and one rounding to FP32 at the end.
 // WRAP: 0 = interior wave (no wrap), 1 = wrap of cell offsets, 2 = also the
-// fixed-point wrap of single-cell axes.
+// fixed-point wrap of single-cell axes, 3 = box-relative records (the 32-bit
+// difference is the minimum image).
 template <int WRAP>
 __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const BoxF &F, float &dx, float &dy)
 {
+    if constexpr (WRAP == 3) {
+        const int ix = (int)(__float_as_uint(rj.x) - __float_as_uint(ri.x));
+        const int iy = (int)(__float_as_uint(rj.y) - __float_as_uint(ri.y));
+        dx = (float)((double)ix * F.sclx);
+        dy = (float)((double)iy * F.scly);
+        return;
+    }
     const int ci = __float_as_int(ri.w), cj = __float_as_int(rj.w);
     int ox = (cj & 0xffff) - (ci & 0xffff), oy = (cj >> 16) - (ci >> 16);
     if (WRAP) {
@@ -970,13 +999,15 @@ __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const 
     dy = (float)((double)iy * F.scly);
 }
 
-// Wrap mode of sep32 for the active lanes: 0 when every lane's build cell
+// Wrap mode of sep32 for the active lanes: 3 for box-relative records, 0
+// when every lane's build cell
 // is at least one cell from each side (the stencil of such a cell needs no
 // wrap, so every listed neighbour's cell offset is already -1, 0 or 1 and
 // the separations are the same as with the wrap), 2 for a box with a
 // single-cell axis, else 1.
 __device__ __forceinline__ int wave_wrap32(const float4 &ri, const BoxF &F)
 {
+    if (F.fixbox) return 3;
     const int c = __float_as_int(ri.w), cx = c & 0xffff, cy = c >> 16;
     const bool in = cx >= 1 && cx <= F.ncx - 2 && cy >= 1 && cy <= F.ncy - 2;
     if (__ballot(!in) == 0ull) return 0;
@@ -993,7 +1024,8 @@ struct Dens32 {
 
 // FP32 sums: the terms of one unrolled round (U32 list entries) are added in
 // FP32, and each round's partial sum goes into a Kahan-compensated total.
-template <int U32, bool COH, int WRAP>
+// UNIM (box-relative records only): all masses equal, none fetched.
+template <int U32, bool COH, int WRAP, bool UNIM>
 __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F, int s, Dens32 &d)
 {
     const float4 ri = A.rf[s];
@@ -1004,11 +1036,13 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
         int t[U32];
         float4 rj[U32];
         float2 vv[U32];
+        float mm[U32];
         short_round<U32>(nb, k0, c.x, s, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
-            if (COH) vv[u] = A.vf[t[u]];
+            if (COH && WRAP != 3) vv[u] = A.vf[t[u]];
+            if (WRAP == 3 && !UNIM) mm[u] = (float)A.m[t[u]];
         }
         float grho = 0.f, grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
 #pragma unroll
@@ -1016,22 +1050,24 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
             float dx, dy;
             sep32<WRAP>(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
+            const float mj = WRAP == 3 ? (UNIM ? F.munif : mm[u]) : rj[u].z;
+            const float vjx = WRAP == 3 ? rj[u].z : vv[u].x, vjy = WRAP == 3 ? rj[u].w : vv[u].y;
             if (k0 + u < c.x && r2 < F.cut2) {
                 const float rr = __builtin_amdgcn_sqrtf(r2);
                 if (r2 < F.cutl) {
                     const float q = rr * F.ihl, w = 1.f - q;
-                    grhol += rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w));
+                    grhol += mj * (F.qhl * (1.f + 3.f * q) * (w * w * w));
                     if (COH) {
                         const float Fl = F.fhl * w * w;
                         const float fx = Fl * dx, fy = Fl * dy;
-                        ggx += rj[u].z * fx;
-                        ggy += rj[u].z * fy;
-                        gev += rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                        ggx += mj * fx;
+                        ggy += mj * fy;
+                        gev += mj * ((vjx - vi.x) * fx + (vjy - vi.y) * fy);
                     }
                 }
                 if (r2 < F.h2) {
                     const float q = rr * F.ih, w = 1.f - q;
-                    grho += rj[u].z * (F.qh * (1.f + 3.f * q) * (w * w * w));
+                    grho += mj * (F.qh * (1.f + 3.f * q) * (w * w * w));
                 }
             }
         }
@@ -1049,11 +1085,13 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
         int t[U32];
         float4 rj[U32];
         float2 vv[U32];
+        float mm[U32];
         long_round<U32>(nb, A.lcap, k0, c.y, s, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
-            if (COH) vv[u] = A.vf[t[u]];
+            if (COH && WRAP != 3) vv[u] = A.vf[t[u]];
+            if (WRAP == 3 && !UNIM) mm[u] = (float)A.m[t[u]];
         }
         float grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
 #pragma unroll
@@ -1061,16 +1099,18 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
             float dx, dy;
             sep32<WRAP>(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
+            const float mj = WRAP == 3 ? (UNIM ? F.munif : mm[u]) : rj[u].z;
+            const float vjx = WRAP == 3 ? rj[u].z : vv[u].x, vjy = WRAP == 3 ? rj[u].w : vv[u].y;
             if (k0 + u < c.y && r2 < F.cutl) {
                 const float rr = __builtin_amdgcn_sqrtf(r2);
                 const float q = rr * F.ihl, w = 1.f - q;
-                grhol += rj[u].z * (F.qhl * (1.f + 3.f * q) * (w * w * w));
+                grhol += mj * (F.qhl * (1.f + 3.f * q) * (w * w * w));
                 if (COH) {
                     const float Fl = F.fhl * w * w;
                     const float fx = Fl * dx, fy = Fl * dy;
-                    ggx += rj[u].z * fx;
-                    ggy += rj[u].z * fy;
-                    gev += rj[u].z * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                    ggx += mj * fx;
+                    ggy += mj * fy;
+                    gev += mj * ((vjx - vi.x) * fx + (vjy - vi.y) * fy);
                 }
             }
         }
@@ -1102,14 +1142,21 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
         d.rho = Kahan(F.qh);
         d.rhol = Kahan(F.qhl);
         const int wrap = wave_wrap32(A.rf[s], F);
+        const bool unim = F.munif > 0.f;
         if (F.terms & 2) {
-            if (wrap == 0) density32_walk<U32, true, 0>(A, F, s, d);
-            else if (wrap == 1) density32_walk<U32, true, 1>(A, F, s, d);
-            else density32_walk<U32, true, 2>(A, F, s, d);
+            if (wrap == 3) {
+                if (unim) density32_walk<U32, true, 3, true>(A, F, s, d);
+                else density32_walk<U32, true, 3, false>(A, F, s, d);
+            } else if (wrap == 0) density32_walk<U32, true, 0, false>(A, F, s, d);
+            else if (wrap == 1) density32_walk<U32, true, 1, false>(A, F, s, d);
+            else density32_walk<U32, true, 2, false>(A, F, s, d);
         } else {
-            if (wrap == 0) density32_walk<U32, false, 0>(A, F, s, d);
-            else if (wrap == 1) density32_walk<U32, false, 1>(A, F, s, d);
-            else density32_walk<U32, false, 2>(A, F, s, d);
+            if (wrap == 3) {
+                if (unim) density32_walk<U32, false, 3, true>(A, F, s, d);
+                else density32_walk<U32, false, 3, false>(A, F, s, d);
+            } else if (wrap == 0) density32_walk<U32, false, 0, false>(A, F, s, d);
+            else if (wrap == 1) density32_walk<U32, false, 1, false>(A, F, s, d);
+            else density32_walk<U32, false, 2, false>(A, F, s, d);
         }
     });
     const float2 vi = A.vf[s];
@@ -1128,7 +1175,7 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     A.T[s] = T;
     const double ps = (B.terms & 1u) ? p / (rd * rd) : 0.0;
     A.S[s] = make_double4(ps, 0.0, ps, mi);
-    A.S32[s] = make_float4((float)ps, 0.f, (float)ps, 0.f);
+    A.S32[s] = make_float4((float)ps, 0.f, (float)ps, (float)mi);
     A.Q[s] = make_double2(0.0, 0.0);
     A.vq32[s] = make_float4(vi.x, vi.y, 0.f, 0.f);
     A.vt32[s] = make_float4(vi.x, vi.y, (float)(mi / rd), (float)T);
@@ -1209,7 +1256,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
         const int wrap = wave_wrap32(A.rf[s], F);
-        if (wrap == 0) grad32_walk<U32, 0>(A, F, s, g);
+        if (wrap == 3) grad32_walk<U32, 3>(A, F, s, g);
+        else if (wrap == 0) grad32_walk<U32, 0>(A, F, s, g);
         else if (wrap == 1) grad32_walk<U32, 1>(A, F, s, g);
         else grad32_walk<U32, 2>(A, F, s, g);
     });
@@ -1226,7 +1274,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
         const double4 S0 = A.S[s];
         const double4 Sn = make_double4((pr + pxx) / r2i, pxy / r2i, (pr + pyy) / r2i, S0.w);
         A.S[s] = Sn;
-        A.S32[s] = make_float4((float)Sn.x, (float)Sn.y, (float)Sn.z, 0.f);
+        A.S32[s] = make_float4((float)Sn.x, (float)Sn.y, (float)Sn.z, (float)Sn.w);
     }
     if (B.terms & 8u) {
         qx = -B.kcond * g.gt0.value();
@@ -1276,7 +1324,7 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
                 const float fx = axx * wx + axy * wy;
                 const float fy = axy * wx + ayy * wy;
                 const float cc = COND ? (vqi.z + vq[u].z) * wx + (vqi.w + vq[u].w) * wy : 0.f;
-                const float mj = rj[u].z;
+                const float mj = Sj[u].w;
                 const float e = 0.5f * ((vq[u].x - vqi.x) * fx + (vq[u].y - vqi.y) * fy);
                 px += mj * fx;
                 py += mj * fy;
@@ -1289,8 +1337,8 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
     }
 }
 
-// P3 in FP32 over the short entries; neighbour record: rf (m in .z),
-// (v, q/rho^2), S.  Adds the cohesive part from P1.
+// P3 in FP32 over the short entries; neighbour record: rf, (v, q/rho^2),
+// (S, m).  Adds the cohesive part from P1.
 template <int U32>
 __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
 {
@@ -1306,11 +1354,13 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
         const BoxF F = boxf(A.boxes[bu], true);
         const int wrap = wave_wrap32(A.rf[s], F);
         if (F.terms & 8) {
-            if (wrap == 0) rates32_walk<U32, true, 0>(A, F, s, ns, o);
+            if (wrap == 3) rates32_walk<U32, true, 3>(A, F, s, ns, o);
+            else if (wrap == 0) rates32_walk<U32, true, 0>(A, F, s, ns, o);
             else if (wrap == 1) rates32_walk<U32, true, 1>(A, F, s, ns, o);
             else rates32_walk<U32, true, 2>(A, F, s, ns, o);
         } else {
-            if (wrap == 0) rates32_walk<U32, false, 0>(A, F, s, ns, o);
+            if (wrap == 3) rates32_walk<U32, false, 3>(A, F, s, ns, o);
+            else if (wrap == 0) rates32_walk<U32, false, 0>(A, F, s, ns, o);
             else if (wrap == 1) rates32_walk<U32, false, 1>(A, F, s, ns, o);
             else rates32_walk<U32, false, 2>(A, F, s, ns, o);
         }
@@ -1353,7 +1403,10 @@ __global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
         A.e[s] = m * (0.5 * (vx * vx + vy * vy) + un);
         A.vp[s] = make_double2(vx, vy);
         A.up[s] = un;
-        if (fp32) A.vf[s] = make_float2((float)vx, (float)vy);
+        if (fp32) {
+            A.vf[s] = make_float2((float)vx, (float)vy);
+            if (B.fixbox) A.rf[s] = fix_rec(A.r[s], make_double2(vx, vy), A.pcell[s], m, B);
+        }
         return;
     }
     const double2 r = A.r[s];
@@ -1370,7 +1423,7 @@ __global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
     min_image(ddx, ddy, B);
     if (ddx * ddx + ddy * ddy > B.hskin2) flag_health(A.dyn, b, HEALTH_SKIN);
     if (fp32) {
-        A.rf[s] = fix_pos(rn, A.pcell[s], m, B);
+        A.rf[s] = fix_rec(rn, vp, A.pcell[s], m, B);
         A.vf[s] = make_float2((float)vp.x, (float)vp.y);
     }
 }

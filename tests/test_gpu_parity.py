@@ -2,6 +2,7 @@
 the same seeded inputs.  Pair sets and separations must be bit-exact; FP64
 fields within 1e-12 (normwise where sums cancel); FP32-compensated fields
 within 1e-6."""
+import ctypes
 import math
 
 import numpy as np
@@ -222,7 +223,19 @@ def test_positions_outside_box_rejected():
 
 
 # ------------------------------------------------------------------ FP32-compensated form
-def test_fp32_compensated_error_vs_fp64():
+@pytest.fixture(params=["box_relative", "cell_relative"])
+def fp32_mirror(request):
+    """Both FP32C position mirrors: box-relative fixed point (boxes up to the
+    process-wide side limit, the default) and build-cell-relative (limit 0)."""
+    old = ctypes.c_double()
+    _lib.check(_lib.lib.sph_get_fp32_box_relative_max(ctypes.byref(old)))
+    if request.param == "cell_relative":
+        _lib.check(_lib.lib.sph_set_fp32_box_relative_max(0.0))
+    yield request.param
+    _lib.check(_lib.lib.sph_set_fp32_box_relative_max(old.value))
+
+
+def test_fp32_compensated_error_vs_fp64(fp32_mirror):
     r, v, m, prm = _system(64, seed=12)
     prm32 = dict(prm, precision=_lib.FP32C)
     u = np.full(r.shape[0], -1.1)
@@ -234,7 +247,7 @@ def test_fp32_compensated_error_vs_fp64():
     assert rel_norm(got["rho_lr"], ref["rho_lr"]) < 1e-6
     assert rel_norm(got["acc"], ref["acc"]) < 1e-6
     assert rel_norm(got["udot"], ref["udot"]) < 1e-6
-    print("fp32c errors rho %.2e rho_lr %.2e acc %.2e udot %.2e" % (
+    print("fp32c (%s) errors rho %.2e rho_lr %.2e acc %.2e udot %.2e" % (fp32_mirror,
         rel_norm(got["rho"], ref["rho"]), rel_norm(got["rho_lr"], ref["rho_lr"]),
         rel_norm(got["acc"], ref["acc"]), rel_norm(got["udot"], ref["udot"])))
 
@@ -256,7 +269,7 @@ def _rect_system(nx, ny, Lx, Ly, seed):
 
 @pytest.mark.parametrize("nx,ny,Lx,Ly", [(40, 13, 40.0, 13.0), (64, 8, 64.0, 8.0), (6, 50, 6.0, 50.0),
                                          (30, 30, 30.0, 30.5)])
-def test_rectangular_boxes(nx, ny, Lx, Ly):
+def test_rectangular_boxes(nx, ny, Lx, Ly, fp32_mirror):
     # elongated boxes, and boxes with a single cell along one axis (minimum
     # image instead of cell offsets, in the FP64 build and the FP32 mirrors)
     r, v, m, u = _rect_system(nx, ny, Lx, Ly, seed=nx * ny)
