@@ -77,7 +77,7 @@ const char *sph_version(void);
 /* Per-pass GPU timing with HIP events (off by default; on disables graph
  * replay so every pass can be bracketed). */
 int sph_ctx_set_timing(sph_ctx *ctx, int on);
-/* FP32C position mirror: boxes whose sides are all <= L (default 1024) keep
+/* FP32C position mirror: boxes whose sides are all <= L (default 512) keep
  * box-relative 32-bit fixed-point positions with the velocity in the same
  * 16-byte neighbour record (resolution L / 2^32); larger boxes keep
  * build-cell-relative positions (2^29 units per cell).  Applies to boxes

@@ -577,8 +577,8 @@ int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
     // listed pairs are closer than r_cut + skin at the build and each end
     // moves at most skin / 2 before the next, so within r_cut + 2 skin
     B.rin = 2.0 * (rcut + p->skin);
-    // FP32C boxes up to FIXBOX_MAX_L per side mirror positions as box-relative
-    // fixed point (2^32 units per side)
+    // FP32C boxes up to the box-relative limit per side mirror positions as
+    // box-relative fixed point (2^32 units per side)
     B.fixbox = (p->precision == SPH_FP32C && p->Lx <= g_fixbox_max_l && p->Ly <= g_fixbox_max_l) ? 1 : 0;
     // cells cover both the pair-export radius and the list radius
     const double Rc = std::max(std::sqrt(B.R2list), rcut + p->skin) * (1.0 + 1e-9);
