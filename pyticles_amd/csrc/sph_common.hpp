@@ -9,19 +9,25 @@
 //                    integer time from the last rates; every pass reads these
 //   last rates       acc (double2), udot
 //   sorted scratch   *_s copies written by the permutation of a build
-//   neighbour list   tiles of 64 slots, entry k of slot s at
-//                    ((s/64)*lcap + k)*64 + s%64 (uint32 slot), cnt[s] = (short, long):
-//                    short entries (r < h + skin at build) at k = 0.., long
-//                    entries (h + skin <= r < r_cut + skin) at k = lcap-1 down
+//   neighbour list   tiles of 64 slots in groups of 4 entries: position P of
+//                    slot s at ((s/64)*(lcap/4) + P/4)*256 + (s%64)*4 + P%4
+//                    (uint32 slot), cnt[s] = (short, long): short entries
+//                    (r < h + skin at build) at P = 0.., long entries
+//                    (h + skin <= r < r_cut + skin) at P = lcap-1 down
 //   derived          rho (double2: rho, rho_lr), T, S (double4: Sxx, Sxy,
 //                    Syy = (p I + Pi)/rho^2, and m), Q (double2: q/rho^2),
 //                    vt (double2: m/rho, T), aco / uco (cohesive rates)
-//   FP32 mirrors     rf (int x, y fixed point relative to the build cell
-//                    corner, float m, cell column|row), vf (float2 v),
-//                    S32 (float4), vq32 (float4: v, q/rho^2), vt32 (float4: v, m/rho, T)
+//   FP32 mirrors     rf: box-relative fixed-point x, y (2^32 units per side)
+//                    and float v, or (boxes wider than the box-relative
+//                    limit) x, y fixed point relative to the build cell
+//                    corner, float m, cell column|row; vf (float2 v),
+//                    S32 (float4: S, m), vq32 (float4: v, q/rho^2),
+//                    vt32 (float4: v, m/rho, T)
 // A box is a contiguous slot range [part_off, part_off + n) plus a key range
-// [key_base, key_base + ncx*ncy] of cells; free slot ranges own one key each
-// so a stable sort over the whole arena never moves a particle across boxes.
+// [key_base, key_base + ncx*ncy) of cells (key_base + ncx*ncy, the next
+// segment's first key, ends its cell_start range); free slot ranges own one
+// key each.  Key ranges increase with the slot offset, so a stable sort of
+// the building slots' keys never moves a particle across boxes.
 #pragma once
 #include <stdint.h>
 #include <hip/hip_runtime.h>
@@ -98,7 +104,7 @@ struct ArenaView {
     int *pcell;          // cell of the slot at the last build
     double2 *r_ref;      // position at the last build (skin check)
     int *slot_box;       // box of the slot (-1 = free)
-    unsigned *nbr;       // neighbour lists in tiles of 64 slots (list_of)
+    unsigned *nbr;       // neighbour lists: 64-slot tiles, 4-entry groups (list_base)
     int2 *cnt;           // (short, long) entries per slot
     // derived fields
     double2 *rho;
