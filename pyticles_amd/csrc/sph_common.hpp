@@ -82,6 +82,9 @@ struct BoxDyn {
     int64_t next_rebuild;   // step index of the next neighbour-list build
     unsigned long long vmax2, amax2;  // max |v|^2, |a|^2 at the last build (double bits)
     int64_t builds;         // list builds so far
+    double vlast, alast;    // max |v|, |a| measured at the last build
+    int64_t built_at;       // step index of the last build
+    int64_t extends;        // scheduled builds postponed on the measured displacement
 };
 
 // Segment of the slot space: a box or a free range, sorted by offset.
@@ -132,6 +135,8 @@ struct ArenaView {
     // position of building box b's first slot (its slots numbered on)
     int *plan_n;
     int *box_coff;
+    double *wdisp;       // per 64-slot wave: largest squared displacement since the
+                         // build, written by k_integrate on a box's review step
     // boxes
     const BoxDesc *boxes;
     BoxDyn *dyn;

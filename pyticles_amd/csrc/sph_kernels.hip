@@ -1381,14 +1381,12 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
 // half-step values; vp / up get the prediction to the next integer time.
 // The energy sum m (v^2/2 + u) of each slot is captured at the first and
 // last synchronised state.
-__global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
+// Leapfrog step of slot s of running box b; returns the squared displacement
+// since the box's last build (-1 at the closing half kick, which moves
+// nothing).
+__device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, int s, int b)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
-    if (s >= A.n) return;
-    const int b = A.slot_box[s];
-    if (b < 0) return;
     const BoxDyn &D = A.dyn[b];
-    if (D.state != BOX_RUN) return;
     const BoxDesc &B = A.boxes[b];
     const int64_t k = D.k, K = D.nsteps;
     const double kick = (k == 0 || k == K) ? 0.5 * B.dt : B.dt;
@@ -1407,7 +1405,7 @@ __global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
             A.vf[s] = make_float2((float)vx, (float)vy);
             if (B.fixbox) A.rf[s] = fix_rec(A.r[s], make_double2(vx, vy), A.pcell[s], m, B);
         }
-        return;
+        return -1.0;
     }
     const double2 r = A.r[s];
     double2 rn;
@@ -1421,11 +1419,43 @@ __global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
     const double2 r0 = A.r_ref[s];
     double ddx = rn.x - r0.x, ddy = rn.y - r0.y;
     min_image(ddx, ddy, B);
-    if (ddx * ddx + ddy * ddy > B.hskin2) flag_health(A.dyn, b, HEALTH_SKIN);
+    const double d2 = ddx * ddx + ddy * ddy;
+    if (d2 > B.hskin2) flag_health(A.dyn, b, HEALTH_SKIN);
     if (fp32) {
         A.rf[s] = fix_rec(rn, vp, A.pcell[s], m, B);
         A.vf[s] = make_float2((float)vp.x, (float)vp.y);
     }
+    return d2;
+}
+
+// A running box re-examines its list at the step before a scheduled build
+// (k_box_step, extend_steps); on that step k_integrate leaves the largest
+// squared displacement of every 64-slot wave in wdisp.
+__device__ __forceinline__ bool list_review(const BoxDyn &D)
+{
+    return D.state == BOX_RUN && !building(D, 0) && D.k + 1 < D.nsteps && D.k + 1 >= D.next_rebuild;
+}
+
+// Integrate every running slot; on a box's review step also store the wave's
+// largest squared displacement (a shuffle maximum over all 64 lanes, lanes of
+// other boxes included, which only over-estimates; no early return, so every
+// lane takes part).
+__global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    double d2 = 0.0;
+    bool review = false;
+    if (s < A.n) {
+        const int b = A.slot_box[s];
+        if (b >= 0 && A.dyn[b].state == BOX_RUN) {
+            d2 = fmax(integrate_slot(A, fp32, s, b), 0.0);
+            review = list_review(A.dyn[b]);
+        }
+    }
+    if (__ballot(review) == 0ull) return;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off));
+    if ((threadIdx.x & 63) == 0) A.wdisp[s >> 6] = d2;
 }
 
 // ===========================================================================
@@ -1449,6 +1479,29 @@ __device__ __forceinline__ int64_t reuse_steps(const BoxDesc &B, const BoxDyn &D
     return K < 1 ? 1 : (K > 64 ? 64 : K);
 }
 
+// Steps a box may still reuse its list at a scheduled build step: the
+// largest c with  c dt v + (c dt)^2 a <= 0.8 (skin/2 - d)  for the speed and
+// acceleration measured at the last build and the largest current
+// displacement d from the build positions (reuse_steps' bound applied to the
+// margin that is left),
+// capped so that a list lives at most 64 steps; 0 = build now.  Exceeding
+// half the skin anyway raises HEALTH_SKIN in k_integrate, as before.
+__device__ __forceinline__ int64_t extend_steps(const BoxDesc &B, const BoxDyn &D, double dmax2)
+{
+    // a list built before any rates existed (acceleration 0, as reuse_steps
+    // notes) is never extended: its bound has no acceleration term
+    if (!(D.alast > 0.0)) return 0;
+    const double d = sqrt(dmax2);
+    const double C = 0.8 * (0.5 * B.skin - d);
+    if (!(C > 0.0)) return 0;
+    const double Bq = B.dt * D.vlast, Aq = B.dt * B.dt * D.alast;
+    const double c = (-Bq + sqrt(Bq * Bq + 4.0 * Aq * C)) / (2.0 * Aq);
+    const int64_t room = 64 - (D.k + 1 - D.built_at);  // list age at the next step
+    int64_t e = (int64_t)floor(c);
+    e = e > room ? room : e;
+    return e < 0 ? 0 : e;
+}
+
 // Per-box step bookkeeping, one block per box.  A running box at k = 0 or
 // k = nsteps first sums its slots' energies m (v^2/2 + u) into E0 / E1
 // (fixed order: four strided partial sums per thread, then a tree).  Then
@@ -1465,7 +1518,23 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
     const int state = D.state;
     const int64_t k = D.k, K = D.nsteps;
     const bool capture = state == BOX_RUN && (k == 0 || k == K);
+    const bool review = list_review(D);
     __shared__ double red[NT_BOX];
+    double dmax2 = 0.0;
+    if (review) {  // largest squared displacement over the box's waves
+        const BoxDesc &B = A.boxes[b];
+        const int w0 = B.part_off >> 6, w1 = (B.part_off + B.n - 1) >> 6;
+        double m = 0.0;
+        for (int w = w0 + (int)threadIdx.x; w <= w1; w += NT_BOX) m = fmax(m, A.wdisp[w]);
+        red[threadIdx.x] = m;
+        __syncthreads();
+        for (int off = NT_BOX / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+            __syncthreads();
+        }
+        dmax2 = red[0];
+        __syncthreads();  // red is reused below
+    }
     if (capture) {
         const BoxDesc &B = A.boxes[b];
         const double *e = A.e + B.part_off;
@@ -1495,8 +1564,19 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
         const int64_t nK = reuse_steps(A.boxes[b], D);
         D.next_rebuild = (state == BOX_INIT ? 0 : k) + nK;
         D.builds += 1;
+        D.built_at = state == BOX_INIT ? 0 : k;
+        D.vlast = sqrt(__longlong_as_double((long long)D.vmax2));
+        D.alast = sqrt(__longlong_as_double((long long)D.amax2));
         D.vmax2 = 0ull;
         D.amax2 = 0ull;
+    } else if (review) {
+        // the next step would build: postpone while the measured displacement
+        // leaves room, by the same bound as reuse_steps on what is left
+        const int64_t c = extend_steps(A.boxes[b], D, dmax2);
+        if (c > 0) {
+            D.next_rebuild = k + 1 + c;
+            D.extends += 1;
+        }
     }
     if (state == BOX_INIT) {
         D.state = K > 0 ? BOX_RUN : BOX_DONE;
