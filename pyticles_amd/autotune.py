@@ -139,8 +139,8 @@ def tune(systems, precision, cls=None, steps=None, warm=5, save=True, path=CACHE
         systems = [systems]
     if cls is None:
         cls = "box" if len(systems) == 1 else "batched"
-    if steps is None:  # several rebuild cycles, so a skin's list cost and rebuild savings both show
-        steps = 120 if cls == "box" else 100
+    if steps is None:  # many rebuild cycles, so a skin's list cost and rebuild savings both show
+        steps = 300 if cls == "box" else 200
     table = []
     for c in candidates():
         if cls == "box":
@@ -161,7 +161,7 @@ def tune(systems, precision, cls=None, steps=None, warm=5, save=True, path=CACHE
     return best, table
 
 
-def pick(table, within=0.02):
+def pick(table, within=0.03):
     """The candidate with the smallest skin among those within `within` of
     the fastest rate (then the fastest of those): skins whose sampled rates
     differ by timing noise are not worth longer lists."""

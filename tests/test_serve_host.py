@@ -143,8 +143,8 @@ def test_committed_schedule_cache_is_valid():
 
 def test_autotune_pick_prefers_smaller_skin_on_near_ties():
     t = [dict(unroll=4, skin=0.5, rate=1.00e9), dict(unroll=4, skin=0.75, rate=1.01e9),
-         dict(unroll=4, skin=1.25, rate=1.015e9), dict(unroll=2, skin=0.5, rate=0.90e9)]
-    assert autotune.pick(t) == dict(unroll=4, skin=0.5)  # 1.25 and 0.75 are within 2 % of 0.5
+         dict(unroll=4, skin=1.25, rate=1.025e9), dict(unroll=2, skin=0.5, rate=0.90e9)]
+    assert autotune.pick(t) == dict(unroll=4, skin=0.5)  # 1.25 and 0.75 are within 3 % of 0.5
     t.append(dict(unroll=4, skin=1.0, rate=1.2e9))  # a clear winner is kept
     assert autotune.pick(t) == dict(unroll=4, skin=1.0)
     # equal skins: the faster unroll
