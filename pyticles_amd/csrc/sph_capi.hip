@@ -194,6 +194,7 @@ struct Arena {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int use_graph = 1;
     int always_build = 0;  // arenas: build stage every step (per-box schedule on device)
+    bool small_boxes = false;  // all live boxes fit the per-box sort (set by relayout)
     int unroll = 0;        // list entries in flight per lane in the pair passes (0 = default)
 
     bool fp32() const { return precision == SPH_FP32C; }
@@ -356,6 +357,10 @@ struct Arena {
             }
         }
         nkeys = key;
+        // every live box small enough for the per-box sort: no global sort
+        small_boxes = true;
+        for (size_t i = 0; i < boxes.size(); ++i)
+            if (box_live[i] && !box_sort_fits(boxes[i].ncx * boxes[i].ncy, boxes[i].n)) small_boxes = false;
         if (nkeys + 1 > cell_cap) {
             int nc = std::max(nkeys + 1, cell_cap * 2);
             dfree(cell_start);
@@ -441,10 +446,14 @@ struct Arena {
         // the sort runs over the M building slots (M read on the device);
         // the other kernels run over the slots and skip non-building boxes
         launch_build_plan(A, force_mask, st);
-        launch_keys(A, force_mask, st);
-        radix_sort_pairs(keys_in, k_out, v_out, k_alt, v_alt, hwm, bits_for(nkeys), sort_tmp, st, plan_n + 1);
+        if (small_boxes) {
+            launch_box_sort(A, force_mask, (int)boxes.size(), st);
+        } else {
+            launch_keys(A, force_mask, st);
+            radix_sort_pairs(keys_in, k_out, v_out, k_alt, v_alt, hwm, bits_for(nkeys), sort_tmp, st, plan_n + 1);
+            launch_cell_start(A, force_mask, st);
+        }
         launch_permute(A, force_mask, st);
-        launch_cell_start(A, force_mask, st);
         launch_build_lists(A, force_mask, fp32() ? 1 : 0, st);
     }
 

@@ -365,6 +365,116 @@ __global__ __launch_bounds__(NT) void k_cell_start(ArenaView A, int force_mask)
     }
 }
 
+// Per-box counting sort for arenas of small boxes (every box with at most
+// BS_MAX_CELLS cells and BS_MAX_N slots): one block per building box does
+// what k_keys, the radix sort and k_cell_start do for it, in LDS.  The order
+// is the same stable one (by cell, then by compact position), so the two
+// pipelines give identical lists.  Ranks come from wave ballots and per-wave
+// counters (each wave owns one counter row and walks its slice of the box in
+// order), so there are no atomics and the result is deterministic.
+static constexpr int BS_THREADS = 512;
+static constexpr int BS_WAVES = BS_THREADS / 64;
+static constexpr int BS_CELL_BITS = 11;
+static constexpr int BS_MAX_CELLS = 1 << BS_CELL_BITS;
+
+__device__ __forceinline__ uint64_t lanes_below()
+{
+    const unsigned lane = threadIdx.x & 63u;
+    return lane == 0 ? 0ull : (~0ull >> (64u - lane));
+}
+
+// Lanes of the wave holding the same cell (valid lanes only); every lane of
+// the wave calls it.
+__device__ __forceinline__ uint64_t match_cell(int c, bool valid)
+{
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < BS_CELL_BITS; ++bit) {
+        const bool x = (c >> bit) & 1;
+        const uint64_t bb = __ballot(x);
+        m &= x ? bb : ~bb;
+    }
+    return m;
+}
+
+__device__ __forceinline__ int cell_of(double2 r, const BoxDesc &B)
+{
+    int cx = (int)(r.x * B.icwx), cy = (int)(r.y * B.icwy);
+    cx = cx < 0 ? 0 : (cx >= B.ncx ? B.ncx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= B.ncy ? B.ncy - 1 : cy);
+    return cy * B.ncx + cx;
+}
+
+__global__ __launch_bounds__(BS_THREADS) void k_box_sort(ArenaView A, int force_mask, int nbox)
+{
+    __shared__ unsigned short cnt[BS_WAVES][BS_MAX_CELLS];
+    __shared__ int start[BS_MAX_CELLS + 1];
+    __shared__ int part[BS_THREADS];
+    const int b = blockIdx.x;
+    if (b >= nbox || !building(A.dyn[b], force_mask)) return;
+    const BoxDesc &B = A.boxes[b];
+    const int n = B.n, C = B.ncx * B.ncy;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    for (int c = tid; c < C; c += BS_THREADS)
+        for (int v = 0; v < BS_WAVES; ++v) cnt[v][c] = 0;
+    __syncthreads();
+    // wave w walks slots [s0, s1) of the box, 64 at a time, in order
+    const int S = ((n + BS_WAVES - 1) / BS_WAVES + 63) & ~63;
+    const int s0 = min(w * S, n), s1 = min(s0 + S, n);
+    for (int g = s0; g < s1; g += 64) {
+        const int i = g + lane;
+        const bool valid = i < s1;
+        const int c = valid ? cell_of(A.r[B.part_off + i], B) : 0;
+        const uint64_t m = match_cell(c, valid);
+        if (valid && (m & lanes_below()) == 0) cnt[w][c] += (unsigned short)__popcll(m);
+    }
+    __syncthreads();
+    // start[c] = slots in cells below c; cnt[w][c] becomes the count of cell
+    // c in the slices of waves below w
+    const int per = (C + BS_THREADS - 1) / BS_THREADS;
+    const int c0 = min(tid * per, C), c1 = min(c0 + per, C);
+    int sum = 0;
+    for (int c = c0; c < c1; ++c)
+        for (int v = 0; v < BS_WAVES; ++v) sum += cnt[v][c];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < BS_THREADS; off <<= 1) {  // inclusive scan
+        const int x = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += x;
+        __syncthreads();
+    }
+    int run = part[tid] - sum;
+    for (int c = c0; c < c1; ++c) {
+        start[c] = run;
+        int acc = 0;
+        for (int v = 0; v < BS_WAVES; ++v) {
+            const int x = cnt[v][c];
+            cnt[v][c] = (unsigned short)acc;
+            acc += x;
+        }
+        run += acc;
+    }
+    if (tid == 0) start[C] = n;
+    __syncthreads();
+    // stable scatter: compact positions of the sorted order
+    const int coff = A.box_coff[b];
+    for (int g = s0; g < s1; g += 64) {
+        const int i = g + lane;
+        const bool valid = i < s1;
+        const int c = valid ? cell_of(A.r[B.part_off + i], B) : 0;
+        const uint64_t m = match_cell(c, valid);
+        const int rank = __popcll(m & lanes_below());
+        if (valid) {
+            const int pos = start[c] + cnt[w][c] + rank;
+            A.vals[coff + pos] = (unsigned)(coff + i);
+            A.keys[coff + pos] = (unsigned)(B.key_base + c);
+        }
+        if (valid && rank == 0) cnt[w][c] += (unsigned short)__popcll(m);
+    }
+    for (int c = tid; c <= C; c += BS_THREADS) A.cell_start[B.key_base + c] = B.part_off + start[c];
+}
+
 // Candidate scan of one building slot over the stencil cells of its build
 // cell, in stencil order and slot order within a cell (the list order).
 // Four candidate positions are loaded before any is tested; SINGLE: a box
@@ -1953,6 +2063,11 @@ void launch_permute(const ArenaView &A, int force_mask, hipStream_t st)
 {
     hipLaunchKernelGGL(k_permute, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
 }
+void launch_box_sort(const ArenaView &A, int force_mask, int nbox, hipStream_t st)
+{
+    if (nbox > 0) hipLaunchKernelGGL(k_box_sort, dim3(nbox), dim3(BS_THREADS), 0, st, A, force_mask, nbox);
+}
+bool box_sort_fits(int ncells, int n) { return ncells <= BS_MAX_CELLS && n <= 65535; }
 void launch_cell_start(const ArenaView &A, int force_mask, hipStream_t st)
 {
     hipLaunchKernelGGL(k_cell_start, grid_for(A.n), dim3(NT), 0, st, A, force_mask);

@@ -18,6 +18,10 @@ void launch_build_plan(const ArenaView &A, int force_mask, hipStream_t st);
 void launch_keys(const ArenaView &A, int force_mask, hipStream_t st);
 void launch_permute(const ArenaView &A, int force_mask, hipStream_t st);
 void launch_cell_start(const ArenaView &A, int force_mask, hipStream_t st);
+// arenas whose boxes all fit box_sort_fits: per-box counting sort in place of
+// keys + radix sort + cell starts (same order)
+void launch_box_sort(const ArenaView &A, int force_mask, int nbox, hipStream_t st);
+bool box_sort_fits(int ncells, int n);
 void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_t st);
 void launch_density(const ArenaView &A, int fp32, int state_mask, int unroll, hipStream_t st);
 void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, int unroll, hipStream_t st);

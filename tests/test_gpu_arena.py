@@ -53,6 +53,25 @@ def test_batched_equals_alone(precision):
             assert got["E1"] == pytest.approx(rf["E1"], rel=1e-12)
 
 
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32C])
+def test_box_sort_and_radix_sort_agree(precision):
+    # Alone, a small box is ordered by the per-box counting sort; beside a
+    # box with more than 2048 cells the arena uses the global radix sort for
+    # all boxes.  Both are stable by (cell, slot), so results agree bitwise.
+    boxes = small_boxes()
+    ref = [alone(s, precision) for s in boxes]
+    big = configs.lognormal_box(110, 44000, 44001, 0.6, 0.7, steps=45)
+    assert (int(big.prm["Lx"] / 4.5) ** 2) > 2048
+    with Arena(sum(s.n for s in boxes) + big.n + 64, precision=precision) as ar:
+        ids = [ar.add(s) for s in boxes]
+        ib = ar.add(big)
+        run_to_completion(ar, ids + [ib])
+        for i, rf in zip(ids, ref):
+            got = ar.result(i)
+            for k in ("r", "v", "u", "rho"):
+                assert np.array_equal(got[k], rf[k]), (precision, i, k)
+
+
 def test_served_admission_mid_run_and_slot_reuse():
     prec = _lib.FP32C
     a, b, c = small_boxes()

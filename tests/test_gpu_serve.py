@@ -34,7 +34,9 @@ def rel(a, b):
 def test_served_stream_overlaps_and_equals_alone():
     prec = _lib.FP32C
     jobs = [configs.lognormal_box(300 + j, 256, 1500, 0.3, 1.2, slo=8, shi=40) for j in range(10)]
-    arr = np.linspace(0.0, 0.03, len(jobs))
+    # arrivals 0.3 ms apart: each job runs 8-40 steps, so later jobs arrive
+    # while earlier ones are on the device
+    arr = np.linspace(0.0, 0.003, len(jobs))
     sched = dict(unroll=2, skin=0.75)
     srv = Server(2e9, precision=prec, max_resident=4000, max_job=1500, chunk=3, keep_results=True,
                  schedule=sched)
@@ -42,8 +44,8 @@ def test_served_stream_overlaps_and_equals_alone():
         recs = srv.run(jobs, arr)
     finally:
         srv.close()
+    assert all(r.finished is not None and r.health == 0 for r in recs), [(r.job_id, r.health) for r in recs]
     assert srv.max_running >= 2, "jobs never overlapped on the device"
-    assert all(r.finished is not None and r.health == 0 for r in recs)
     for rec in recs:
         job = jobs[rec.job_id]
         ref = alone(job, prec, unroll=sched["unroll"], skin=sched["skin"])
