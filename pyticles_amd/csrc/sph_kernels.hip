@@ -2067,7 +2067,9 @@ void launch_box_sort(const ArenaView &A, int force_mask, int nbox, hipStream_t s
 {
     if (nbox > 0) hipLaunchKernelGGL(k_box_sort, dim3(nbox), dim3(BS_THREADS), 0, st, A, force_mask, nbox);
 }
-bool box_sort_fits(int ncells, int n) { return ncells <= BS_MAX_CELLS && n <= 65535; }
+// one block sorts a box: beyond ~16k slots its sequential walk outlasts the
+// radix sort (measured on the served workload, boxes up to 40k slots)
+bool box_sort_fits(int ncells, int n) { return ncells <= BS_MAX_CELLS && n <= 16384; }
 void launch_cell_start(const ArenaView &A, int force_mask, hipStream_t st)
 {
     hipLaunchKernelGGL(k_cell_start, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
