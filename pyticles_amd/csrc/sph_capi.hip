@@ -164,6 +164,8 @@ struct Arena {
     // staging
     double *d_stage = nullptr;
     size_t stage_bytes = 0;
+    double *h_stage = nullptr;  // pinned
+    size_t h_stage_bytes = 0;
 
     // asynchronous snapshots: state copied on the device in stream order,
     // then to the host on a copy stream
@@ -292,6 +294,9 @@ struct Arena {
         dfree(d_snap);
         snap_cap = 0;
         dfree(cell_start); dfree(d_boxes); dfree(d_dyn); dfree(d_segs); dfree(d_stage); dfree(d_e3);
+        if (h_stage) (void)hipHostFree(h_stage);
+        h_stage = nullptr;
+        h_stage_bytes = 0;
         dfree(d_inv); dfree(d_cnt); dfree(d_off); dfree(d_grand); dfree(d_jbuf); dfree(d_iap);
         if (scan_tmp) (void)hipFree(scan_tmp);
         scan_tmp = nullptr;
@@ -332,6 +337,8 @@ struct Arena {
 
     void invalidate_graph()
     {
+        if (!graph[0] && !graph[1]) return;
+        (void)hipStreamSynchronize(ctx->stream);  // no replay of them is in flight
         for (auto &g : graph) {
             if (g) (void)hipGraphExecDestroy(g);
             g = nullptr;
@@ -382,17 +389,31 @@ struct Arena {
         HIPCHK(hipMemcpyAsync(d_segs, segs.data(), sizeof(SegDesc) * segs.size(), hipMemcpyHostToDevice,
                               ctx->stream));
         launch_slot_box(view(), ctx->stream);
-        // host-side vectors must outlive the async copies
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        // copies from pageable host memory are staged before hipMemcpyAsync
+        // returns, so the tables may change right away; no wait
         // the slot count, key count and segment count are baked into
         // captured launches
         invalidate_graph();
         return SPH_OK;
     }
 
+    // pinned host staging for combined downloads
+    int host_stage(size_t bytes)
+    {
+        if (bytes <= h_stage_bytes) return SPH_OK;
+        (void)hipStreamSynchronize(ctx->stream);
+        if (h_stage) (void)hipHostFree(h_stage);
+        h_stage = nullptr;
+        h_stage_bytes = 0;
+        if (hipHostMalloc((void **)&h_stage, bytes, hipHostMallocDefault) != hipSuccess) return SPH_ERR_NOMEM;
+        h_stage_bytes = bytes;
+        return SPH_OK;
+    }
+
     int stage(size_t bytes)
     {
         if (bytes <= stage_bytes) return SPH_OK;
+        (void)hipStreamSynchronize(ctx->stream);  // uploads do not wait: let them finish with it
         dfree(d_stage);
         stage_bytes = 0;
         if (dalloc((char **)&d_stage, bytes)) return SPH_ERR_NOMEM;
@@ -552,11 +573,12 @@ struct Arena {
         return SPH_OK;
     }
 
+    // h_dyn is pinned and copies run in stream order (a later read_dyn lands
+    // after this copy), so no wait is needed here
     int write_dyn(int b, const BoxDyn &D)
     {
         h_dyn[b] = D;
         HIPCHK(hipMemcpyAsync(d_dyn + b, h_dyn + b, sizeof(BoxDyn), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
         return SPH_OK;
     }
 
@@ -706,7 +728,8 @@ int Arena::upload(int b, int field, const double *host, int ncomp)
         boxes[b].munif = uni ? host[0] : 0.0;
         HIPCHK(hipMemcpyAsync(d_boxes + b, &boxes[b], sizeof(BoxDesc), hipMemcpyHostToDevice, ctx->stream));
     }
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // the pageable source is staged before the copy call returns and the
+    // staging buffer is reused in stream order: no wait
     return SPH_OK;
 }
 
@@ -1213,7 +1236,7 @@ int sph_arena_add(sph_arena *ar, int64_t n, const sph_params *prm, const double 
     int b = -1;
     if (int rc = A.add_box(n, prm, &b)) return rc;
     A.boxes[b].T0 = T0;
-    if (int rc = A.relayout()) return rc;
+    HIPCHK(hipMemcpyAsync(A.d_boxes + b, &A.boxes[b], sizeof(BoxDesc), hipMemcpyHostToDevice, A.ctx->stream));
     if (int rc = A.upload(b, 0, r, 2)) return rc;
     if (int rc = A.upload(b, 1, v, 2)) return rc;
     if (int rc = A.upload(b, 2, m, 1)) return rc;
@@ -1268,15 +1291,28 @@ int sph_arena_result(sph_arena *ar, int box_id, double *r, double *v, double *u,
     Arena &A = ar->ar;
     if (box_id < 0 || box_id >= (int)A.boxes.size() || !A.box_live[box_id]) return fail(SPH_ERR_ARG, "no such box");
     HIPCHK(hipSetDevice(A.ctx->device));
-    if (r) if (int rc = A.download(box_id, 0, r, 2)) return rc;
-    if (v) if (int rc = A.download(box_id, 1, v, 2)) return rc;
-    if (u) if (int rc = A.download(box_id, 3, u, 1)) return rc;
-    if (rho) if (int rc = A.download(box_id, 4, rho, 1)) return rc;
-    if (e0 || e1) {
-        if (int rc = A.read_dyn()) return rc;
-        if (e0) *e0 = A.h_dyn[box_id].E0;
-        if (e1) *e1 = A.h_dyn[box_id].E1;
-    }
+    // r, v, u, rho packed into the device staging buffer, one copy to pinned
+    // host memory and one wait (with the box states) for the four fields
+    const int n = A.boxes[box_id].n;
+    const size_t nd = (size_t)n;
+    if (A.stage(sizeof(double) * 6 * nd)) return SPH_ERR_NOMEM;
+    if (A.host_stage(sizeof(double) * 6 * nd)) return SPH_ERR_NOMEM;
+    double *d = A.d_stage;
+    const ArenaView V = A.view();
+    launch_download(V, box_id, n, 0, d, A.ctx->stream);
+    launch_download(V, box_id, n, 1, d + 2 * nd, A.ctx->stream);
+    launch_download(V, box_id, n, 3, d + 4 * nd, A.ctx->stream);
+    launch_download(V, box_id, n, 4, d + 5 * nd, A.ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(A.h_stage, d, sizeof(double) * 6 * nd, hipMemcpyDeviceToHost, A.ctx->stream));
+    if (int rc = A.read_dyn()) return rc;  // waits for the copy above too
+    const double *h = A.h_stage;
+    if (r) memcpy(r, h, sizeof(double) * 2 * nd);
+    if (v) memcpy(v, h + 2 * nd, sizeof(double) * 2 * nd);
+    if (u) memcpy(u, h + 4 * nd, sizeof(double) * nd);
+    if (rho) memcpy(rho, h + 5 * nd, sizeof(double) * nd);
+    if (e0) *e0 = A.h_dyn[box_id].E0;
+    if (e1) *e1 = A.h_dyn[box_id].E1;
     return SPH_OK;
 }
 
