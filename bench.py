@@ -134,21 +134,61 @@ def run_gpu(s, steps, warmup, timing=True, unroll=0, kernel_steps=200):
     return b, t1 - t0, tm, (E0, E1)
 
 
-def cpu_baseline(s, target_s=12.0):
-    """The FP64 oracle (single-threaded C restatement of the reference's
-    pair-list loops) on a bounded sample of the same workload."""
+def host_cpu():
+    """The host CPU's model name and the cores this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count()
+    return model, ncpu
+
+
+def _oracle_rate(O, s, u0, nsteps, threaded, runs=5):
+    """Median particle-steps/s of `runs` oracle runs of nsteps steps each,
+    after a one-step warm-up."""
+    O.run(s.r, s.v, s.m, u0, s.prm, 1, terms=s.prm["terms"], threaded=threaded)
+    rates = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        O.run(s.r, s.v, s.m, u0, s.prm, nsteps, terms=s.prm["terms"], threaded=threaded)
+        rates.append(s.n * nsteps / (time.perf_counter() - t))
+    return float(np.median(rates)), rates
+
+
+def cpu_baseline(s, full=False, max_run_s=3.0):
+    """The FP64 oracle (C restatement of the reference's pair-list loops) on
+    the same workload: single-threaded (as the reference runs) and with
+    OpenMP on every host core the process may use.  Each figure is the
+    median of 5 runs after a one-step warm-up; a run is the whole job when
+    full=True (C1), else 10 steps, fewer when a step takes over a tenth of
+    max_run_s (the 1M-particle box)."""
     from oracle import oracle as O
     u0 = O.init_u(s.r, s.m, s.T, s.prm)
     t = time.perf_counter()
     O.run(s.r, s.v, s.m, u0, s.prm, 1, terms=s.prm["terms"])
-    one = time.perf_counter() - t  # 1 step = 2 evaluations
-    nsteps = max(1, int(target_s / max(one / 2, 1e-6)) - 1)
-    t = time.perf_counter()
-    O.run(s.r, s.v, s.m, u0, s.prm, nsteps, terms=s.prm["terms"])
-    el = time.perf_counter() - t
-    return dict(value=s.n * nsteps / el, unit="particle-steps/s", cores=1, kind="port",
-                sample="oracle/oracle.c FP64 on the same initial state, %d steps of N=%d (%d rate evaluations, "
-                       "%.1f s); single thread, pair list rebuilt every evaluation" % (nsteps, s.n, nsteps + 1, el))
+    one = time.perf_counter() - t  # 1 step = 2 rate evaluations
+    nsteps = int(s.steps) if full else max(1, min(10, int(max_run_s / max(one / 2, 1e-6))))
+    v1, runs1 = _oracle_rate(O, s, u0, nsteps, threaded=False)
+    vt, runst = _oracle_rate(O, s, u0, nsteps, threaded=True)
+    model, ncpu = host_cpu()
+    nth = O.num_threads(True)
+    return dict(value=v1, unit="particle-steps/s", cores=1, kind="port",
+                sample="oracle/oracle.c FP64 on the same initial state, %d-step runs of N=%d (%d rate "
+                       "evaluations each), median of 5 after a 1-step warm-up; pair list rebuilt every "
+                       "evaluation" % (nsteps, s.n, nsteps + 1),
+                host=dict(cpu=model, cores_available=ncpu),
+                runs=runs1,
+                all_cores=dict(value=vt, threads=nth, runs=runst,
+                               note="OpenMP build of the same source (per-thread pair sums)"))
 
 
 def error_vs_oracle(s, r=None, v=None, u=None, unroll=0):
@@ -454,8 +494,10 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         s_cpu, _ = workload(args.workload, "fp64")
-        line["cpu_baseline"] = cpu_baseline(s_cpu)
-        line["cpu_baseline"]["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+        cb = cpu_baseline(s_cpu, full=args.workload == "c1")
+        cb["gpu_over_cpu"] = value / cb["value"]
+        cb["all_cores"]["gpu_over_cpu"] = value / cb["all_cores"]["value"]
+        line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
 

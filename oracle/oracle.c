@@ -46,7 +46,16 @@
 
 #if defined(_OPENMP)
 #include <omp.h>
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_FOR
 #endif
+
+/* Two builds of this file: liboracle.so (no OpenMP) is the checker and sums */
+/* every pair loop serially in pair order; liboracle_omp.so (-fopenmp) is    */
+/* only the all-cores CPU baseline: pair loops split over threads into      */
+/* private partial sums added in thread order, so its results differ from   */
+/* the checker's by rounding only.                                          */
 
 typedef struct {
     double Lx, Ly;      /* periodic box sides (box.xmax, box.ymax)                 */
@@ -116,6 +125,41 @@ static int cmp_i64(const void *a, const void *b)
     return (x > y) - (x < y);
 }
 
+typedef struct {
+    const double *r;
+    double Lx, Ly, R2;
+    int ncx, ncy;
+    const int64_t *head, *order, *cellof;
+} cell_grid;
+
+/* Partners j > i of particle i with r^2 < R2 from the 3x3 cell stencil,    */
+/* ascending in j; returns their count.                                      */
+static int64_t cell_partners(const cell_grid *g, int64_t i, int64_t *js)
+{
+    const double *r = g->r;
+    int ncx = g->ncx, ncy = g->ncy;
+    int cx = (int)(g->cellof[i] % ncx), cy = (int)(g->cellof[i] / ncx);
+    int64_t nj = 0;
+    int dxs = ncx >= 3 ? 1 : 0, dys = ncy >= 3 ? 1 : 0;
+    for (int oy = -dys; oy <= dys; ++oy) {
+        int yy = (cy + oy + ncy) % ncy;
+        for (int ox = -dxs; ox <= dxs; ++ox) {
+            int xx = (cx + ox + ncx) % ncx;
+            int64_t c = (int64_t)yy * ncx + xx;
+            for (int64_t t = g->head[c]; t < g->head[c + 1]; ++t) {
+                int64_t j = g->order[t];
+                if (j <= i) continue;
+                double dx = r[2 * j] - r[2 * i];
+                double dy = r[2 * j + 1] - r[2 * i + 1];
+                orc_min_image(&dx, &dy, g->Lx, g->Ly);
+                if (pair_rsq(dx, dy) < g->R2) js[nj++] = j;
+            }
+        }
+    }
+    qsort(js, (size_t)nj, sizeof(int64_t), cmp_i64);
+    return nj;
+}
+
 /* Same pair set via an O(N) periodic cell list; identical output order.     */
 /* The pair test is the same expression as orc_pairs_brute; only the         */
 /* candidate enumeration differs.                                            */
@@ -147,28 +191,38 @@ int64_t orc_pairs_cell(int64_t n, const double *r, double Lx, double Ly,
         for (int64_t i = 0; i < n; ++i) order[fill[cellof[i]]++] = i;
         free(fill);
     }
+    cell_grid g = {r, Lx, Ly, R2, ncx, ncy, head, order, cellof};
     int64_t nip = 0;
-    int64_t *js = (int64_t *)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
-    for (int64_t i = 0; i < n; ++i) {
-        int cx = (int)(cellof[i] % ncx), cy = (int)(cellof[i] / ncx);
-        int64_t nj = 0;
-        int dxs = ncx >= 3 ? 1 : 0, dys = ncy >= 3 ? 1 : 0;
-        for (int oy = -dys; oy <= dys; ++oy) {
-            int yy = (cy + oy + ncy) % ncy;
-            for (int ox = -dxs; ox <= dxs; ++ox) {
-                int xx = (cx + ox + ncx) % ncx;
-                int64_t c = (int64_t)yy * ncx + xx;
-                for (int64_t t = head[c]; t < head[c + 1]; ++t) {
-                    int64_t j = order[t];
-                    if (j <= i) continue;
-                    double dx = r[2 * j] - r[2 * i];
-                    double dy = r[2 * j + 1] - r[2 * i + 1];
-                    orc_min_image(&dx, &dy, Lx, Ly);
-                    if (pair_rsq(dx, dy) < R2) js[nj++] = j;
-                }
+#if defined(_OPENMP)
+    /* count pass, exclusive scan, then each particle writes its own rows    */
+    int64_t *at = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+#pragma omp parallel
+    {
+        int64_t *js = (int64_t *)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t i = 0; i < n; ++i) at[i + 1] = cell_partners(&g, i, js);
+#pragma omp single
+        {
+            at[0] = 0;
+            for (int64_t i = 0; i < n; ++i) at[i + 1] += at[i];
+        }
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t i = 0; i < n; ++i) {
+            if (at[i] >= cap) continue;
+            int64_t nj = cell_partners(&g, i, js);
+            for (int64_t k = 0; k < nj && at[i] + k < cap; ++k) {
+                iap[2 * (at[i] + k)] = i;
+                iap[2 * (at[i] + k) + 1] = js[k];
             }
         }
-        qsort(js, (size_t)nj, sizeof(int64_t), cmp_i64);
+        free(js);
+    }
+    nip = at[n];
+    free(at);
+#else
+    int64_t *js = (int64_t *)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t nj = cell_partners(&g, i, js);
         for (int64_t k = 0; k < nj; ++k) {
             if (nip < cap) {
                 iap[2 * nip] = i;
@@ -178,6 +232,7 @@ int64_t orc_pairs_cell(int64_t n, const double *r, double Lx, double Ly,
         }
     }
     free(js);
+#endif
     free(head);
     free(order);
     free(cellof);
@@ -190,6 +245,7 @@ void orc_separations(const double *r, const double *v, double Lx, double Ly,
                      int64_t nip, const int64_t *iap,
                      double *drij, double *rij, double *rsq, double *dv)
 {
+    OMP_FOR
     for (int64_t k = 0; k < nip; ++k) {
         int64_t i = iap[2 * k], j = iap[2 * k + 1];
         double dx = r[2 * j] - r[2 * i];
@@ -256,9 +312,78 @@ static inline void lucy2(double r, double h, double *W, double *F)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Pair scatter loops.  body(ctx, k0, k1, out) adds pairs [k0, k1) into the  */
+/* nf arrays out[f] (len[f] doubles each).  Serially that is one call over  */
+/* all pairs on the caller's arrays; with OpenMP each thread sums its block */
+/* of pairs into zeroed private arrays, which are then added to out[] in    */
+/* thread order.                                                             */
+typedef void (*pair_body)(const void *ctx, int64_t k0, int64_t k1, double *const *out);
+
+static void scatter_pairs(pair_body body, const void *ctx, int64_t nip, int nf,
+                          const int64_t *len, double *const *out)
+{
+#if defined(_OPENMP)
+    int nt = omp_get_max_threads();
+    int64_t width = 0;
+    for (int f = 0; f < nf; ++f) width += len[f];
+    double *priv = nt > 1 ? (double *)calloc((size_t)nt * (size_t)width, sizeof(double)) : NULL;
+    if (priv) {
+#pragma omp parallel num_threads(nt)
+        {
+            int t = omp_get_thread_num(), nth = omp_get_num_threads();
+            double *mine[4] = {NULL, NULL, NULL, NULL};
+            int64_t off = (int64_t)t * width;
+            for (int f = 0; f < nf; ++f) {
+                mine[f] = priv + off;
+                off += len[f];
+            }
+            body(ctx, nip * t / nth, nip * (t + 1) / nth, mine);
+#pragma omp barrier
+            int64_t base = 0;
+            for (int f = 0; f < nf; ++f) {
+#pragma omp for schedule(static)
+                for (int64_t x = 0; x < len[f]; ++x) {
+                    double s = 0.;
+                    for (int q = 0; q < nth; ++q) s += priv[(int64_t)q * width + base + x];
+                    out[f][x] += s;
+                }
+                base += len[f];
+            }
+        }
+        free(priv);
+        return;
+    }
+#endif
+    body(ctx, 0, nip, out);
+}
+
+/* ------------------------------------------------------------------------ */
 /* b3  density: properties.py:77-91 / spdensity.py:40-56, long range as      */
 /* spam_properties_ls properties.py:138-161.  Self term W(0,h) is not        */
 /* multiplied by m_i (properties.py:77-78).                                  */
+typedef struct {
+    const double *m, *rij;
+    const int64_t *iap;
+    double h, hl;
+} density_ctx;
+
+static void density_body(const void *cv, int64_t k0, int64_t k1, double *const *out)
+{
+    const density_ctx *c = (const density_ctx *)cv;
+    const double *m = c->m;
+    double *rho = out[0], *rho_lr = out[1];
+    for (int64_t k = k0; k < k1; ++k) {
+        int64_t i = c->iap[2 * k], j = c->iap[2 * k + 1];
+        double W, Wl, f;
+        lucy2(c->rij[k], c->h, &W, &f);
+        lucy2(c->rij[k], c->hl, &Wl, &f);
+        rho[i] += W * m[j];
+        rho[j] += W * m[i];
+        rho_lr[i] += Wl * m[j];
+        rho_lr[j] += Wl * m[i];
+    }
+}
+
 void orc_density(int64_t n, const double *m, int64_t nip, const int64_t *iap,
                  const double *rij, double h, double hl,
                  double *rho, double *rho_lr)
@@ -267,16 +392,10 @@ void orc_density(int64_t n, const double *m, int64_t nip, const int64_t *iap,
     lucy2(0., h, &w0, &f);
     lucy2(0., hl, &w0l, &f);
     for (int64_t i = 0; i < n; ++i) { rho[i] = w0; rho_lr[i] = w0l; }
-    for (int64_t k = 0; k < nip; ++k) {
-        int64_t i = iap[2 * k], j = iap[2 * k + 1];
-        double W, Wl;
-        lucy2(rij[k], h, &W, &f);
-        lucy2(rij[k], hl, &Wl, &f);
-        rho[i] += W * m[j];
-        rho[j] += W * m[i];
-        rho_lr[i] += Wl * m[j];
-        rho_lr[j] += Wl * m[i];
-    }
+    density_ctx c = {m, rij, iap, h, hl};
+    int64_t len[2] = {n, n};
+    double *out[2] = {rho, rho_lr};
+    scatter_pairs(density_body, &c, nip, 2, len, out);
 }
 
 /* b4  EOS: vdw properties.py:38-41 (p from short rho, pco = -a rho_lr^2 as */
@@ -285,6 +404,7 @@ void orc_density(int64_t n, const double *m, int64_t nip, const int64_t *iap,
 void orc_eos(int64_t n, const double *rho, const double *rho_lr, const double *u,
              double a, double b, double kB, double *T, double *p, double *pco)
 {
+    OMP_FOR
     for (int64_t i = 0; i < n; ++i) {
         double t = (u[i] + a * rho[i]) / kB;
         if (t < 0.) t = 0.;
@@ -299,17 +419,21 @@ void orc_eos(int64_t n, const double *rho, const double *rho_lr, const double *u
 /*   gradT_i[b]    = sum_j (m_j/rho_j) (T_j - T_i)   (grad_i W_ij)_b          */
 /* Parity unpinned: the reference's splib.calc_grad_v / calc_heat_flux_3d    */
 /* (f_properties.py:111,130) are absent.  Layout gradv[n][2][2], gradT[n][2]. */
-void orc_gradients(int64_t n, const double *m, const double *rho, const double *v,
-                   const double *T, int64_t nip, const int64_t *iap,
-                   const double *rij, const double *drij, double h,
-                   double *gradv, double *gradT)
+typedef struct {
+    const double *m, *rho, *v, *T, *rij, *drij;
+    const int64_t *iap;
+    double h;
+} gradients_ctx;
+
+static void gradients_body(const void *cv, int64_t k0, int64_t k1, double *const *out)
 {
-    memset(gradv, 0, sizeof(double) * 4 * n);
-    memset(gradT, 0, sizeof(double) * 2 * n);
-    for (int64_t k = 0; k < nip; ++k) {
-        int64_t i = iap[2 * k], j = iap[2 * k + 1];
+    const gradients_ctx *c = (const gradients_ctx *)cv;
+    const double *m = c->m, *rho = c->rho, *v = c->v, *T = c->T, *drij = c->drij;
+    double *gradv = out[0], *gradT = out[1];
+    for (int64_t k = k0; k < k1; ++k) {
+        int64_t i = c->iap[2 * k], j = c->iap[2 * k + 1];
         double W, F;
-        lucy2(rij[k], h, &W, &F);
+        lucy2(c->rij[k], c->h, &W, &F);
         if (F == 0.) continue;
         double gx = -F * drij[2 * k], gy = -F * drij[2 * k + 1];  /* grad_i W */
         double dvx = v[2 * j] - v[2 * i], dvy = v[2 * j + 1] - v[2 * i + 1];
@@ -330,6 +454,19 @@ void orc_gradients(int64_t n, const double *m, const double *rho, const double *
     }
 }
 
+void orc_gradients(int64_t n, const double *m, const double *rho, const double *v,
+                   const double *T, int64_t nip, const int64_t *iap,
+                   const double *rij, const double *drij, double h,
+                   double *gradv, double *gradT)
+{
+    memset(gradv, 0, sizeof(double) * 4 * n);
+    memset(gradT, 0, sizeof(double) * 2 * n);
+    gradients_ctx c = {m, rho, v, T, rij, drij, iap, h};
+    int64_t len[2] = {4 * n, 2 * n};
+    double *out[2] = {gradv, gradT};
+    scatter_pairs(gradients_body, &c, nip, 2, len, out);
+}
+
 /* c4 per-particle tensors.  Short-range pressure tensor                      */
 /*   P_i = p_i I + Pi_i,  Pi_i = -eta (G + G^T - tr G I) - zeta tr G I  (d=2)  */
 /* stored divided by rho_i^2 as S = (Sxx, Sxy, Syy); cohesive pco_i/rho_lr^2; */
@@ -340,6 +477,7 @@ void orc_tensors(int64_t n, const double *rho, const double *rho_lr, const doubl
                  double eta, double zeta, double kcond, unsigned terms,
                  double *S, double *Sco, double *Q, double *P, double *jq)
 {
+    OMP_FOR
     for (int64_t i = 0; i < n; ++i) {
         double pi_xx = 0., pi_xy = 0., pi_yy = 0.;
         if ((terms & ORC_VISCOSITY) && gradv) {
@@ -382,18 +520,22 @@ void orc_tensors(int64_t n, const double *rho, const double *rho_lr, const doubl
 /*   u_i += m_j (e + c),  u_j += m_i (e - c)                                 */
 /* The reference's SpamForce omits m_j in vdot (forces.py:357-362); with    */
 /* m = 1 (every configuration) the two are bitwise identical.               */
-void orc_rates(int64_t n, const double *m, const double *v, int64_t nip,
-               const int64_t *iap, const double *rij, const double *drij,
-               double h, double hl, const double *S, const double *Sco,
-               const double *Q, double *acc, double *udot)
+typedef struct {
+    const double *m, *v, *rij, *drij, *S, *Sco, *Q;
+    const int64_t *iap;
+    double h, hl;
+} rates_ctx;
+
+static void rates_body(const void *cv, int64_t k0, int64_t k1, double *const *out)
 {
-    memset(acc, 0, sizeof(double) * 2 * n);
-    memset(udot, 0, sizeof(double) * n);
-    for (int64_t k = 0; k < nip; ++k) {
-        int64_t i = iap[2 * k], j = iap[2 * k + 1];
+    const rates_ctx *cx = (const rates_ctx *)cv;
+    const double *m = cx->m, *v = cx->v, *drij = cx->drij, *S = cx->S, *Sco = cx->Sco, *Q = cx->Q;
+    double *acc = out[0], *udot = out[1];
+    for (int64_t k = k0; k < k1; ++k) {
+        int64_t i = cx->iap[2 * k], j = cx->iap[2 * k + 1];
         double W, F, Wl, Fl;
-        lucy2(rij[k], h, &W, &F);
-        lucy2(rij[k], hl, &Wl, &Fl);
+        lucy2(cx->rij[k], cx->h, &W, &F);
+        lucy2(cx->rij[k], cx->hl, &Wl, &Fl);
         double dx = drij[2 * k], dy = drij[2 * k + 1];
         double wx = F * dx, wy = F * dy, wlx = Fl * dx, wly = Fl * dy;
         double axx = S[3 * i] + S[3 * j], axy = S[3 * i + 1] + S[3 * j + 1];
@@ -411,6 +553,19 @@ void orc_rates(int64_t n, const double *m, const double *v, int64_t nip,
         udot[i] += m[j] * (e + c);
         udot[j] += m[i] * (e - c);
     }
+}
+
+void orc_rates(int64_t n, const double *m, const double *v, int64_t nip,
+               const int64_t *iap, const double *rij, const double *drij,
+               double h, double hl, const double *S, const double *Sco,
+               const double *Q, double *acc, double *udot)
+{
+    memset(acc, 0, sizeof(double) * 2 * n);
+    memset(udot, 0, sizeof(double) * n);
+    rates_ctx c = {m, v, rij, drij, S, Sco, Q, iap, h, hl};
+    int64_t len[2] = {2 * n, n};
+    double *out[2] = {acc, udot};
+    scatter_pairs(rates_body, &c, nip, 2, len, out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -551,6 +706,7 @@ int orc_run(int64_t n, double *r, double *v, const double *m, double *u,
     if (!acc || !udot || !vp || !up) return -1;
     const double hdt = 0.5 * P->dt;
     for (int64_t k = 0; k <= nsteps; ++k) {
+        OMP_FOR
         for (int64_t i = 0; i < n; ++i) {
             if (k == 0) {
                 vp[2 * i] = v[2 * i];
@@ -578,6 +734,7 @@ int orc_run(int64_t n, double *r, double *v, const double *m, double *u,
             }
             energy[k] = e;
         }
+        OMP_FOR
         for (int64_t i = 0; i < n; ++i) {
             v[2 * i] += kick * acc[2 * i];
             v[2 * i + 1] += kick * acc[2 * i + 1];

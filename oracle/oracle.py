@@ -10,6 +10,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# Same source built with OpenMP: pair loops summed per thread, so results
+# differ from the serial checker's by rounding.  Used only for the all-cores
+# CPU baseline; every check uses the serial library.
+LIB_OMP_PATH = os.path.join(HERE, "_build", "liboracle_omp.so")
 
 PRESSURE, COHESION, VISCOSITY, CONDUCTION = 1, 2, 4, 8
 ALL_TERMS = 15
@@ -25,15 +29,15 @@ def build():
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(threaded=False):
+    path = LIB_OMP_PATH if threaded else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         D = ctypes.POINTER(ctypes.c_double)
         I = ctypes.POINTER(ctypes.c_int64)
         P = ctypes.POINTER(Params)
@@ -58,8 +62,8 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _d(a):
@@ -81,10 +85,10 @@ def params(prm):
     return p
 
 
-def pairs(r, Lx, Ly, R2, method="cell"):
+def pairs(r, Lx, Ly, R2, method="cell", threaded=False):
     r = _c(r)
     n = r.shape[0]
-    f = lib().orc_pairs_cell if method == "cell" else lib().orc_pairs_brute
+    f = lib(threaded).orc_pairs_cell if method == "cell" else lib(threaded).orc_pairs_brute
     cap = max(16, 16 * n)
     while True:
         iap = np.empty((cap, 2), dtype=np.int64)
@@ -176,16 +180,22 @@ def init_u(r, m, T, prm):
     return u
 
 
-def run(r, v, m, u, prm, nsteps, terms=ALL_TERMS, energy=False):
-    """nsteps leapfrog steps from a synchronised state; returns (r, v, u[, E])."""
+def run(r, v, m, u, prm, nsteps, terms=ALL_TERMS, energy=False, threaded=False):
+    """nsteps leapfrog steps from a synchronised state; returns (r, v, u[, E]).
+    threaded=True runs the OpenMP build (CPU baseline only)."""
     r, v, m, u = _c(r).copy(), _c(v).copy(), _c(m), _c(u).copy()
     n = r.shape[0]
     E = np.empty(nsteps + 1) if energy else None
     p = params(prm)
-    rc = lib().orc_run(n, _d(r), _d(v), _d(m), _d(u), ctypes.byref(p), terms, nsteps, _d(E))
+    rc = lib(threaded).orc_run(n, _d(r), _d(v), _d(m), _d(u), ctypes.byref(p), terms, nsteps, _d(E))
     if rc != 0:
         raise MemoryError("oracle allocation failed")
     return (r, v, u, E) if energy else (r, v, u)
+
+
+def num_threads(threaded=False):
+    """Threads the oracle's pair loops use (1 for the serial checker)."""
+    return int(lib(threaded).orc_num_threads())
 
 
 def splash_map_2d(x, y, r, m, rho, a, h, bounds, cutoff):

@@ -377,6 +377,21 @@ def test_leapfrog_free_particles_exact():
     np.testing.assert_allclose(r2, np.mod(r + 1.5 * v, 60.0), rtol=1e-12)
 
 
+def test_threaded_baseline_build_matches_checker():
+    # the OpenMP build (all-cores CPU baseline) lists the same pairs in the
+    # same order and steps to the serial checker's state up to rounding
+    s = configs.square_system(48, 0.9, 2, 3)
+    R2 = s.prm["rc"] ** 2 + s.prm["tol"] ** 2
+    L = s.prm["Lx"]
+    np.testing.assert_array_equal(O.pairs(s.r, L, L, R2, threaded=True), O.pairs(s.r, L, L, R2))
+    u = O.init_u(s.r, s.m, s.T, s.prm)
+    serial = O.run(s.r, s.v, s.m, u, s.prm, 5)
+    threaded = O.run(s.r, s.v, s.m, u, s.prm, 5, threaded=True)
+    for a, b in zip(threaded, serial):
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+    assert O.num_threads(False) == 1
+
+
 # ---------------------------------------------------------------- splash rendering
 def test_splash_lattice_sites_render_one():
     # at a lattice site sum_j m_j/rho_j W = (sum_j W)/rho = 1 (self term included)
