@@ -79,27 +79,41 @@ __device__ __forceinline__ void load_entries(const unsigned *base, int P0, unsig
     }
 }
 
-// Slots of the round of short entries k0 .. k0 + U - 1 (k0 a multiple of U);
-// entries past the count n read as s (a valid record whose terms the
-// callers mask off), so the gathers need no branch.
-template <int U>
-__device__ __forceinline__ void short_round(const unsigned *base, int k0, int n, int s, int (&t)[U])
-{
+// A slot's short (LONG = false) or long entries, U per round (k0 a multiple
+// of U; long entry k sits at position lcap - 1 - k).  take(k0, t) gives the
+// slots of round k0.  With AHEAD, next(k0) issues the load of the next
+// round's group, so that list fetch is in flight while round k0's neighbour
+// records are gathered and summed; at the last round it re-reads the current
+// group, so every load stays inside the slot's list rows.  Without AHEAD,
+// take() loads its own round's group.  Entries past the count read as s (a
+// valid record whose terms the callers mask off), so the gathers need no
+// branch.  AHEAD pays on the density walks (~15 rounds per slot); on the
+// short walks of the gradient and rate passes (3-4 rounds) its extra load
+// and registers cost more than the overlap gains.
+template <int U, bool LONG, bool AHEAD>
+struct ListRounds {
+    const unsigned *base;
+    int lcap, n, s;
     unsigned w[U];
-    load_entries<U>(base, k0, w);
+    __device__ __forceinline__ ListRounds(const unsigned *b, int lcap_, int n_, int s_)
+        : base(b), lcap(lcap_), n(n_), s(s_)
+    {
+        if (AHEAD) load_entries<U>(base, pos(0), w);
+    }
+    __device__ __forceinline__ int pos(int k0) const { return LONG ? lcap - k0 - U : k0; }
+    __device__ __forceinline__ void take(int k0, int (&t)[U])
+    {
+        if (!AHEAD) load_entries<U>(base, pos(k0), w);
 #pragma unroll
-    for (int u = 0; u < U; ++u) t[u] = k0 + u < n ? (int)w[u] : s;
-}
-
-// The same for long entries k0 .. k0 + U - 1, at positions lcap - 1 - k.
-template <int U>
-__device__ __forceinline__ void long_round(const unsigned *base, int lcap, int k0, int n, int s, int (&t)[U])
-{
-    unsigned w[U];
-    load_entries<U>(base, lcap - k0 - U, w);
-#pragma unroll
-    for (int u = 0; u < U; ++u) t[u] = k0 + u < n ? (int)w[U - 1 - u] : s;
-}
+        for (int u = 0; u < U; ++u) t[u] = k0 + u < n ? (int)w[LONG ? U - 1 - u : u] : s;
+    }
+    // issued after round k0's gathers, so waiting for them does not wait for
+    // this load too (loads retire in order)
+    __device__ __forceinline__ void next(int k0)
+    {
+        if (AHEAD) load_entries<U>(base, pos(k0 + U < n ? k0 + U : k0), w);
+    }
+};
 
 // Minimum image exactly as neighbour_list.py:110-123 (all '>' tests, then '<').
 template <class Box>
@@ -718,17 +732,19 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
     // U64 entries per round: all their loads are issued before any use so
     // several neighbour fetches are in flight per wavefront; one load brings
     // a round's list entries.
+    ListRounds<U64, false, true> rounds(nb, A.lcap, c.x, s);
     for (int k0 = 0; k0 < c.x; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64];
         double mj[U64];
-        short_round<U64>(nb, k0, c.x, s, t);
+        rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
             if (!UNIM) mj[u] = A.m[t[u]];
             if (COH) vv[u] = A.vp[t[u]];
         }
+        rounds.next(k0);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
@@ -757,17 +773,19 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
     }
     // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
     // read the same row at step k (coalesced), whatever their counts
+    ListRounds<U64, true, true> lrounds(nb, A.lcap, c.y, s);
     for (int k0 = 0; k0 < c.y; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64];
         double mj[U64];
-        long_round<U64>(nb, A.lcap, k0, c.y, s, t);
+        lrounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
             if (!UNIM) mj[u] = A.m[t[u]];
             if (COH) vv[u] = A.vp[t[u]];
         }
+        lrounds.next(k0);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
@@ -860,16 +878,18 @@ __device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K,
     const unsigned *nb = list_base(A, s);
     Grad64 g;
     g.g00 = g.g01 = g.g10 = g.g11 = g.gt0 = g.gt1 = 0.0;
+    ListRounds<U64, false, false> rounds(nb, A.lcap, ns, s);
     for (int k0 = 0; k0 < ns; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64], vt[U64];
-        short_round<U64>(nb, k0, ns, s, t);
+        rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
             vv[u] = A.vp[t[u]];
             vt[u] = A.vt[t[u]];
         }
+        rounds.next(k0);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
@@ -951,11 +971,12 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
     const unsigned *nb = list_base(A, s);
     Rate64 o;
     o.ax = o.ay = o.ud = 0.0;
+    ListRounds<U64, false, false> rounds(nb, A.lcap, ns, s);
     for (int k0 = 0; k0 < ns; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64], Qj[U64];
         double4 Sj[U64];
-        short_round<U64>(nb, k0, ns, s, t);
+        rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             rj[u] = A.r[t[u]];
@@ -963,6 +984,7 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
             Sj[u] = A.S[t[u]];
             if (COND) Qj[u] = A.Q[t[u]];
         }
+        rounds.next(k0);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
             double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y;
@@ -1040,6 +1062,7 @@ struct BoxF {
     double sclx, scly;  // length of one fixed-point unit
     float cut2, cutl, h2, cuts;
     float ih, ihl, qh, qhl, fh, fhl, munif;
+    float ih3, ihl3;  // 3/h, 3/h_l: 1 + 3q = fma(r, 3/h, 1)
     int ncx, ncy, sx, sy, terms, fixbox;
 };
 
@@ -1059,12 +1082,14 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
     f.cuts = (float)(B.R2list < B.h2 ? B.R2list : B.h2);
     f.ih = (float)B.ih; f.ihl = (float)B.ihl; f.qh = (float)B.qh; f.qhl = (float)B.qhl;
     f.fh = (float)B.fh; f.fhl = (float)B.fhl;
+    f.ih3 = (float)(3.0 * B.ih); f.ihl3 = (float)(3.0 * B.ihl);
     f.ncx = B.ncx; f.ncy = B.ncy; f.sx = B.sx; f.sy = B.sy; f.terms = B.terms;
     if (uniform) {
         f.sclx = wave_uniform(f.sclx); f.scly = wave_uniform(f.scly);
         f.cut2 = wave_uniform(f.cut2); f.cutl = wave_uniform(f.cutl); f.h2 = wave_uniform(f.h2);
         f.cuts = wave_uniform(f.cuts); f.ih = wave_uniform(f.ih); f.ihl = wave_uniform(f.ihl);
         f.qh = wave_uniform(f.qh); f.qhl = wave_uniform(f.qhl); f.fh = wave_uniform(f.fh); f.fhl = wave_uniform(f.fhl);
+        f.ih3 = wave_uniform(f.ih3); f.ihl3 = wave_uniform(f.ihl3);
         f.ncx = wave_uniform(f.ncx); f.ncy = wave_uniform(f.ncy); f.sx = wave_uniform(f.sx); f.sy = wave_uniform(f.sy);
         f.terms = wave_uniform(f.terms);
         f.munif = wave_uniform(f.munif); f.fixbox = wave_uniform(f.fixbox);
@@ -1133,102 +1158,109 @@ struct Dens32 {
 // instructions per entry and moved no error against the FP64 oracle).
 
 // FP32 sums: the terms of one unrolled round (U32 list entries) are added in
-// FP32, and each round's partial sum goes into a Kahan-compensated total.
+// FP32, and each round's partial sum, times the kernel constant it leaves
+// out, goes into a Kahan-compensated total.
 // UNIM (box-relative records only): all masses equal, none fetched.
 template <int U32, bool COH, int WRAP, bool UNIM>
 __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F, int s, Dens32 &d)
 {
+    // the kernel constants (and, with UNIM, the mass) scale each round's
+    // partial sums instead of every term
+    const float cqh = UNIM ? F.qh * F.munif : F.qh, cqhl = UNIM ? F.qhl * F.munif : F.qhl;
+    const float cfhl = UNIM ? F.fhl * F.munif : F.fhl;
     const float4 ri = A.rf[s];
     const float2 vi = A.vf[s];
     const int2 c = A.cnt[s];
     const unsigned *nb = list_base(A, s);
+    ListRounds<U32, false, true> rounds(nb, A.lcap, c.x, s);
     for (int k0 = 0; k0 < c.x; k0 += U32) {
         int t[U32];
         float4 rj[U32];
         float2 vv[U32];
         float mm[U32];
-        short_round<U32>(nb, k0, c.x, s, t);
+        rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
             if (COH && WRAP != 3) vv[u] = A.vf[t[u]];
             if (WRAP == 3 && !UNIM) mm[u] = (float)A.m[t[u]];
         }
+        rounds.next(k0);
         float grho = 0.f, grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
             sep32<WRAP>(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
-            const float mj = WRAP == 3 ? (UNIM ? F.munif : mm[u]) : rj[u].z;
+            const float mj = WRAP == 3 ? (UNIM ? 1.f : mm[u]) : rj[u].z;
             const float vjx = WRAP == 3 ? rj[u].z : vv[u].x, vjy = WRAP == 3 ? rj[u].w : vv[u].y;
             if (k0 + u < c.x && r2 < F.cut2) {
                 const float rr = __builtin_amdgcn_sqrtf(r2);
                 if (r2 < F.cutl) {
-                    const float q = rr * F.ihl, w = 1.f - q;
-                    grhol += mj * (F.qhl * (1.f + 3.f * q) * (w * w * w));
+                    const float w = fmaf(-rr, F.ihl, 1.f), w2 = w * w;
+                    grhol = fmaf(mj * fmaf(rr, F.ihl3, 1.f), w2 * w, grhol);
                     if (COH) {
-                        const float Fl = F.fhl * w * w;
-                        const float fx = Fl * dx, fy = Fl * dy;
-                        ggx += mj * fx;
-                        ggy += mj * fy;
-                        gev += mj * ((vjx - vi.x) * fx + (vjy - vi.y) * fy);
+                        const float g = mj * w2;
+                        ggx = fmaf(g, dx, ggx);
+                        ggy = fmaf(g, dy, ggy);
+                        gev = fmaf(g, (vjx - vi.x) * dx + (vjy - vi.y) * dy, gev);
                     }
                 }
                 if (r2 < F.h2) {
-                    const float q = rr * F.ih, w = 1.f - q;
-                    grho += mj * (F.qh * (1.f + 3.f * q) * (w * w * w));
+                    const float w = fmaf(-rr, F.ih, 1.f);
+                    grho = fmaf(mj * fmaf(rr, F.ih3, 1.f), w * w * w, grho);
                 }
             }
         }
-        d.rho.add(grho);
-        d.rhol.add(grhol);
+        d.rho.add(cqh * grho);
+        d.rhol.add(cqhl * grhol);
         if (COH) {
-            d.gx.add(ggx);
-            d.gy.add(ggy);
-            d.ev.add(gev);
+            d.gx.add(cfhl * ggx);
+            d.gy.add(cfhl * ggy);
+            d.ev.add(cfhl * gev);
         }
     }
     // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
     // read the same row at step k (coalesced), whatever their counts
+    ListRounds<U32, true, true> lrounds(nb, A.lcap, c.y, s);
     for (int k0 = 0; k0 < c.y; k0 += U32) {
         int t[U32];
         float4 rj[U32];
         float2 vv[U32];
         float mm[U32];
-        long_round<U32>(nb, A.lcap, k0, c.y, s, t);
+        lrounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
             if (COH && WRAP != 3) vv[u] = A.vf[t[u]];
             if (WRAP == 3 && !UNIM) mm[u] = (float)A.m[t[u]];
         }
+        lrounds.next(k0);
         float grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             float dx, dy;
             sep32<WRAP>(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
-            const float mj = WRAP == 3 ? (UNIM ? F.munif : mm[u]) : rj[u].z;
+            const float mj = WRAP == 3 ? (UNIM ? 1.f : mm[u]) : rj[u].z;
             const float vjx = WRAP == 3 ? rj[u].z : vv[u].x, vjy = WRAP == 3 ? rj[u].w : vv[u].y;
             if (k0 + u < c.y && r2 < F.cutl) {
                 const float rr = __builtin_amdgcn_sqrtf(r2);
-                const float q = rr * F.ihl, w = 1.f - q;
-                grhol += mj * (F.qhl * (1.f + 3.f * q) * (w * w * w));
+                const float w = fmaf(-rr, F.ihl, 1.f), w2 = w * w;
+                grhol = fmaf(mj * fmaf(rr, F.ihl3, 1.f), w2 * w, grhol);
                 if (COH) {
-                    const float Fl = F.fhl * w * w;
-                    const float fx = Fl * dx, fy = Fl * dy;
-                    ggx += mj * fx;
-                    ggy += mj * fy;
-                    gev += mj * ((vjx - vi.x) * fx + (vjy - vi.y) * fy);
+                    const float g = mj * w2;
+                    ggx = fmaf(g, dx, ggx);
+                    ggy = fmaf(g, dy, ggy);
+                    gev = fmaf(g, (vjx - vi.x) * dx + (vjy - vi.y) * dy, gev);
                 }
             }
         }
-        d.rhol.add(grhol);
+        d.rhol.add(cqhl * grhol);
         if (COH) {
-            d.gx.add(ggx);
-            d.gy.add(ggy);
-            d.ev.add(gev);
+            d.gx.add(cfhl * ggx);
+            d.gy.add(cfhl * ggy);
+            d.ev.add(cfhl * gev);
         }
     }
 }
@@ -1308,15 +1340,17 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
     const float Ti = vti.w;
     const int ns = A.cnt[s].x;
     const unsigned *nb = list_base(A, s);
+    ListRounds<U32, false, false> rounds(nb, A.lcap, ns, s);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32], vt[U32];
-        short_round<U32>(nb, k0, ns, s, t);
+        rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
             vt[u] = A.vt32[t[u]];
         }
+        rounds.next(k0);
         float a00 = 0.f, a01 = 0.f, a10 = 0.f, a11 = 0.f, at0 = 0.f, at1 = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
@@ -1409,16 +1443,18 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
     const float4 vqi = A.vq32[s];
     const float4 Si = A.S32[s];
     const unsigned *nb = list_base(A, s);
+    ListRounds<U32, false, false> rounds(nb, A.lcap, ns, s);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32], Sj[U32], vq[U32];
-        short_round<U32>(nb, k0, ns, s, t);
+        rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
             Sj[u] = A.S32[t[u]];
             vq[u] = A.vq32[t[u]];
         }
+        rounds.next(k0);
         float px = 0.f, py = 0.f, pu = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {

@@ -166,12 +166,15 @@ def test_gradients_api():
 
 
 # ------------------------------------------------------------------ stepping
-@pytest.mark.parametrize("cfg,nsteps", [("c1", 20), ("full", 10)])
+@pytest.mark.parametrize("cfg,nsteps", [("c1", 20), ("full", 10), ("full_mass", 10)])
 def test_step_parity(cfg, nsteps):
     if cfg == "c1":
         s = configs.c1()
     else:
         s = configs.square_system(48, 0.9, 2, nsteps)
+    if cfg == "full_mass":  # non-uniform masses: the per-neighbour mass loads and m-weighted energy
+        m = np.random.Generator(np.random.PCG64(9)).uniform(0.8, 1.2, size=s.n)
+        s = configs.System(s.r, s.v, m, s.T, s.prm, s.steps, s.seed)
     terms = s.prm["terms"]
     u0 = O.init_u(s.r, s.m, s.T, s.prm)
     r1, v1, u1, E = O.run(s.r, s.v, s.m, u0, s.prm, nsteps, terms=terms, energy=True)
