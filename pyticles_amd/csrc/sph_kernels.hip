@@ -639,6 +639,7 @@ struct BoxK {
     double Lx, Ly, hLx, hLy;
     double cut2, h2, hl2, cutl, cuth;  // cutl = min(cut2, hl2), cuth = min(cut2, h2)
     double ih, ihl, qh, qhl, fh, fhl, munif, rin, iLx, iLy;
+    double ih3, ihl3;  // 3/h, 3/h_l: 1 + 3q = fma(r, 3/h, 1)
     int terms;
 };
 
@@ -651,6 +652,7 @@ __device__ __forceinline__ BoxK box_k(const BoxDesc &B, bool uniform)
     k.cuth = B.cut2 < B.h2 ? B.cut2 : B.h2;
     k.ih = B.ih; k.ihl = B.ihl; k.qh = B.qh; k.qhl = B.qhl; k.fh = B.fh; k.fhl = B.fhl;
     k.munif = B.munif;
+    k.ih3 = 3.0 * B.ih; k.ihl3 = 3.0 * B.ihl;
     k.rin = B.rin;
     k.iLx = 1.0 / B.Lx;
     k.iLy = 1.0 / B.Ly;
@@ -662,6 +664,7 @@ __device__ __forceinline__ BoxK box_k(const BoxDesc &B, bool uniform)
         k.ih = wave_uniform(k.ih); k.ihl = wave_uniform(k.ihl); k.qh = wave_uniform(k.qh); k.qhl = wave_uniform(k.qhl);
         k.fh = wave_uniform(k.fh); k.fhl = wave_uniform(k.fhl); k.munif = wave_uniform(k.munif);
         k.rin = wave_uniform(k.rin); k.iLx = wave_uniform(k.iLx); k.iLy = wave_uniform(k.iLy);
+        k.ih3 = wave_uniform(k.ih3); k.ihl3 = wave_uniform(k.ihl3);
         k.terms = wave_uniform(k.terms);
     }
     return k;
@@ -725,10 +728,10 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
     const double2 ri = A.r[s], vi = A.vp[s];
     const int2 c = A.cnt[s];
     const unsigned *nb = list_base(A, s);
+    // the sums leave out the kernel constants (and, with UNIM, the mass);
+    // they scale the totals at the end
     Dens64 d;
-    d.rho = K.qh;
-    d.rhol = K.qhl;
-    d.gx = d.gy = d.ev = 0.0;
+    d.rho = d.rhol = d.gx = d.gy = d.ev = 0.0;
     // U64 entries per round: all their loads are issued before any use so
     // several neighbour fetches are in flight per wavefront; one load brings
     // a round's list entries.
@@ -751,22 +754,21 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
             if (WRAP) min_image_listed(dx, dy, K);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < c.x && r2 < K.cut2) {
-                const double m = UNIM ? K.munif : mj[u];
+                const double m = UNIM ? 1.0 : mj[u];
                 const double rr = sqrt_pair(r2);
                 if (r2 < K.hl2) {
-                    const double q = rr * K.ihl, w = 1.0 - q;
-                    d.rhol += m * (K.qhl * (1.0 + 3.0 * q) * (w * w * w));
+                    const double w = fma(-rr, K.ihl, 1.0), w2 = w * w;
+                    d.rhol = fma(m * fma(rr, K.ihl3, 1.0), w2 * w, d.rhol);
                     if (COH) {
-                        const double Fl = K.fhl * w * w;
-                        const double fx = Fl * dx, fy = Fl * dy;
-                        d.gx += m * fx;
-                        d.gy += m * fy;
-                        d.ev += m * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                        const double g = m * w2;
+                        d.gx = fma(g, dx, d.gx);
+                        d.gy = fma(g, dy, d.gy);
+                        d.ev = fma(g, (vv[u].x - vi.x) * dx + (vv[u].y - vi.y) * dy, d.ev);
                     }
                 }
                 if (r2 < K.h2) {
-                    const double q = rr * K.ih, w = 1.0 - q;
-                    d.rho += m * (K.qh * (1.0 + 3.0 * q) * (w * w * w));
+                    const double w = fma(-rr, K.ih, 1.0);
+                    d.rho = fma(m * fma(rr, K.ih3, 1.0), w * w * w, d.rho);
                 }
             }
         }
@@ -792,20 +794,25 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
             if (WRAP) min_image_listed(dx, dy, K);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < c.y && r2 < K.cutl) {
-                const double m = UNIM ? K.munif : mj[u];
+                const double m = UNIM ? 1.0 : mj[u];
                 const double rr = sqrt_pair(r2);
-                const double q = rr * K.ihl, w = 1.0 - q;
-                d.rhol += m * (K.qhl * (1.0 + 3.0 * q) * (w * w * w));
+                const double w = fma(-rr, K.ihl, 1.0), w2 = w * w;
+                d.rhol = fma(m * fma(rr, K.ihl3, 1.0), w2 * w, d.rhol);
                 if (COH) {
-                    const double Fl = K.fhl * w * w;
-                    const double fx = Fl * dx, fy = Fl * dy;
-                    d.gx += m * fx;
-                    d.gy += m * fy;
-                    d.ev += m * ((vv[u].x - vi.x) * fx + (vv[u].y - vi.y) * fy);
+                    const double g = m * w2;
+                    d.gx = fma(g, dx, d.gx);
+                    d.gy = fma(g, dy, d.gy);
+                    d.ev = fma(g, (vv[u].x - vi.x) * dx + (vv[u].y - vi.y) * dy, d.ev);
                 }
             }
         }
     }
+    const double mu = UNIM ? K.munif : 1.0, cf = K.fhl * mu;
+    d.rho = fma(K.qh * mu, d.rho, K.qh);
+    d.rhol = fma(K.qhl * mu, d.rhol, K.qhl);
+    d.gx *= cf;
+    d.gy *= cf;
+    d.ev *= cf;
     return d;
 }
 
