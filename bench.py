@@ -401,7 +401,7 @@ def main():
     ap.add_argument("--c5-jobs", type=int, default=2200)
     ap.add_argument("--c5-warmup", type=int, default=200)
     ap.add_argument("--c5-rate", type=float, default=200.0)
-    ap.add_argument("--c5-resident", type=int, default=2_000_000)
+    ap.add_argument("--c5-resident", type=int, default=4_000_000)
     ap.add_argument("--c5-budget", type=float, default=200e9)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip every extra leg")

@@ -49,20 +49,27 @@ class Server:
                    count whose footprint fits, capped by what max_resident
                    and the largest job can ever use)
     max_resident   resident particles beyond which jobs wait in the queue
+    slack_jobs     slot headroom beyond max_resident, in largest jobs: a job
+                   needs one contiguous slot range, and with a single job of
+                   headroom, freed ranges too short for the queue's head
+                   stalled admission (first-fit, FIFO) while the resident
+                   count sat well under max_resident.  Launches cover slots
+                   up to the last live box only, so headroom costs memory,
+                   not work.
     chunk          arena steps launched between polls (admission granularity)
     keep_results   copy each finished job's state out (r, v, u, rho, E0, E1)
     schedule       dict(unroll=..., skin=...) for every job (autotune output)
     """
 
     def __init__(self, budget_bytes, precision=_lib.FP32C, max_resident=None, max_job=65536, chunk=4,
-                 keep_results=False, schedule=None, arena=None):
+                 keep_results=False, schedule=None, arena=None, slack_jobs=16):
         self.precision = int(precision)
         self.chunk = int(chunk)
         self.keep_results = keep_results
         self.schedule = dict(schedule or {})
         cap = capacity_for(budget_bytes, self.precision) if arena is None else arena.capacity
         if max_resident is not None:
-            cap = min(cap, int(max_resident) + int(max_job))
+            cap = min(cap, int(max_resident) + max(1, int(slack_jobs)) * int(max_job))
         self.max_resident = int(max_resident) if max_resident is not None else cap
         self.budget_bytes = budget_bytes
         if arena is None:
