@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
                     [--precision fp64|fp32c] [--legs c2x,c3,c4,c5] [--no-cpu]
-                    [--no-extra] [--tune PATH] [--calib-copy]
+                    [--no-extra] [--tune PATH] [--calib-copy] [--c4-sample STEPS]
 
 Headline: one periodic box 512 x 512, N = 262,144 jittered lattice, T = 0.9,
 full physics (pressure, cohesion, viscosity eta = 1, conduction k = 5),
@@ -408,7 +408,27 @@ def main():
     ap.add_argument("--tune", metavar="PATH", help="run the autotuner for every workload class, write PATH, exit")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB (HBM counter calibration)")
+    ap.add_argument("--c4-sample", type=int, default=0, metavar="STEPS",
+                    help="only step the batched FP32C arena of the 512 C4 boxes STEPS times (for profilers)")
     args = ap.parse_args()
+
+    if args.c4_sample:
+        from pyticles_amd import _lib, autotune, configs
+        from pyticles_amd.arena import Arena
+        sc = autotune.schedule_for(_lib.FP32C, "batched")
+        boxes = [configs.c4_box(b) for b in range(512)]
+        ar = Arena(sum(b.n for b in boxes), precision=_lib.FP32C)
+        ar.set_schedule(sc["unroll"])
+        for b in boxes:
+            ar.add(autotune._Resched(b, autotune.with_schedule(b, sc, _lib.FP32C)))
+        t = time.perf_counter()
+        ar.step(args.c4_sample)
+        ar.poll_all()
+        el = time.perf_counter() - t
+        print(json.dumps({"c4_sample_steps": args.c4_sample, "particles": sum(b.n for b in boxes),
+                          "ms_per_step": 1e3 * el / args.c4_sample}))
+        ar.close()
+        return
 
     if args.calib_copy:
         # reference kernel of known traffic for the FETCH_SIZE / WRITE_SIZE
