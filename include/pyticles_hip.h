@@ -157,6 +157,15 @@ int sph_energy(sph_box *box, double *ke, double *ue, double *etot);
  * clears them.  Recorded only while sph_ctx_set_timing is on.
  * Mirrors the p.timing dictionary (particles.py:164-169,359). */
 int sph_box_timing(sph_box *box, double *out, int n, int reset);
+/* Average duration (microseconds) of one launch of a pair pass (0 density
+ * + EOS, 1 gradients, 2 rates) at the box's current state: HIP events on
+ * the context stream around reps back-to-back launches.  The passes write
+ * only derived fields (densities, EOS, gradients, rates), never r, v, u or
+ * m; the next sph_step re-derives them at its first evaluation, so timing
+ * does not change later steps.  The box must have been stepped at least
+ * once (its lists and derived fields exist).  No counterpart in the
+ * reference: the measurement behind the dominant kernel's roofline figure. */
+int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch);
 
 /* ---- rendering ------------------------------------------------------------ */
 /* SPH interpolation of a particle field onto grid points (x[i], y[j]):

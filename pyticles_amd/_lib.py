@@ -76,6 +76,7 @@ _SIGS = {
     "sph_step": [_P, _I64],
     "sph_energy": [_P, _D, _D, _D],
     "sph_box_timing": [_P, _D, ctypes.c_int, ctypes.c_int],
+    "sph_box_time_pass": [_P, ctypes.c_int, ctypes.c_int, _D],
     "sph_box_set_graph": [_P, ctypes.c_int],
     "sph_arena_create": [_P, _I64, ctypes.c_int, ctypes.POINTER(_P)],
     "sph_arena_destroy": [_P],

@@ -191,6 +191,7 @@ struct Arena {
     std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_marks;
     double t_acc[T_N] = {0, 0, 0, 0, 0, 0};
     double steps_done = 0;
+    bool stepped = false;  // a box (single-box use) has taken a sph_step
 
     // captured graphs of one arena step without / with a list build
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -1142,6 +1143,7 @@ int sph_step(sph_box *box, int64_t nsteps)
     // nsteps leapfrog steps take nsteps + 1 rate evaluations
     if (int rc = A.step_single(box->b, nsteps + 1)) return rc;
     A.steps_done -= 1.0;
+    A.stepped = true;
     return check_health(A, box->b);
 }
 
@@ -1178,6 +1180,43 @@ int sph_box_timing(sph_box *box, double *out, int n, int reset)
 {
     if (!box) return fail(SPH_ERR_ARG, "null box");
     return read_timing(box->ar, out, n, reset);
+}
+
+int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch)
+{
+    if (!box || !us_per_launch || pass < 0 || pass > 2 || reps < 1 || reps > 100000)
+        return fail(SPH_ERR_ARG, "invalid argument (pass 0..2, reps 1..100000)");
+    Arena &A = box->ar;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    if (!A.stepped) return fail(SPH_ERR_ARG, "box has not been stepped (step it once before timing a pass)");
+    // between sph_step calls the box rests in BOX_DONE, which the stepping
+    // masks skip: the passes run under MASK_ALL, as the single evaluations
+    // do.  They write only derived fields (never r, v, u, m), which the next
+    // sph_step re-derives at its first evaluation.
+    const ArenaView V = A.view();
+    const int f = A.fp32() ? 1 : 0;
+    hipStream_t st = A.ctx->stream;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    if (e == hipSuccess) e = hipEventRecord(ev[0], st);
+    if (e == hipSuccess) {
+        for (int i = 0; i < reps; ++i) {
+            if (pass == 0) launch_density(V, f, MASK_ALL, A.unroll, st);
+            else if (pass == 1) launch_grad(V, f, MASK_ALL, 0, A.unroll, st);
+            else launch_rates(V, f, MASK_ALL, A.unroll, st);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e != hipSuccess) return fail(SPH_ERR_HIP, std::string("sph_box_time_pass: ") + hipGetErrorString(e));
+    *us_per_launch = 1000.0 * (double)ms / reps;
+    return check_health(A, box->b);
 }
 
 // ---- batching and serving ------------------------------------------------------

@@ -125,6 +125,17 @@ class DeviceBox:
         return dict(zip(("build_ms", "density_ms", "gradients_ms", "rates_ms", "pairs_ms", "steps",
                          "integrate_ms"), out))
 
+    PASSES = ("density", "gradients", "rates")
+
+    def time_pass(self, name, reps=50):
+        """Average microseconds of one launch of a pair pass ('density',
+        'gradients' or 'rates') at the current state: events around reps
+        back-to-back launches.  Writes only derived fields (the next step
+        re-derives them); the box must have been stepped."""
+        us = ctypes.c_double()
+        _lib.check(_lib.lib.sph_box_time_pass(self._h, self.PASSES.index(name), int(reps), ctypes.byref(us)))
+        return us.value
+
     # -- (a) neighbour list ----------------------------------------------
     def pairs(self, cap=None):
         """Every unordered pair (i<j) with rsq < rc^2 + tol^2, lexicographic."""

@@ -206,6 +206,33 @@ def test_step_graph_equals_direct_launches():
         assert np.array_equal(outs[0][k], outs[1][k]), k
 
 
+def test_time_pass_leaves_state_unchanged():
+    # the roofline timing relaunches the pair passes after a step: the
+    # particle state and the densities must be bit-identical afterwards (the
+    # rates are re-derived at the synchronised velocities, which the next
+    # step re-derives anyway), and stepping on must give what the same
+    # calls give untimed
+    s = configs.square_system(24, 0.9, 6, 5)
+    with from_system(s) as b:
+        with pytest.raises(_lib.SphError):
+            b.time_pass("density", reps=2)  # not stepped yet
+        b.step(5)
+        before = b.download("r", "v", "u", "rho", "rho_lr")
+        for p in b.PASSES:
+            assert b.time_pass(p, reps=3) > 0.0
+        after = b.download("r", "v", "u", "rho", "rho_lr")
+        for k in before:
+            assert np.array_equal(before[k], after[k]), k
+        b.step(2)  # and stepping on gives what the same calls give untimed
+        got = b.download("r", "v", "u")
+    with from_system(s) as c:
+        c.step(5)
+        c.step(2)
+        ref = c.download("r", "v", "u")
+    for k in ref:
+        assert np.array_equal(got[k], ref[k]), k
+
+
 def test_eos_pole_is_reported():
     # lattice spacing 0.6: rho*b > 1, the vdW pole (properties.py:41)
     ns = 16
