@@ -132,9 +132,12 @@ struct ArenaView {
     unsigned *vals;      // source slot of each sorted slot
     unsigned *keys_in;   // unsorted keys written by the key pass
     // build plan: plan_n[1] = building slots M; box_coff[b] = compact
-    // position of building box b's first slot (its slots numbered on)
+    // position of building box b's first slot (its slots numbered on);
+    // tile_list[0 .. plan_n[2]) = the 64-slot tiles holding building slots,
+    // in slot order, each once
     int *plan_n;
     int *box_coff;
+    int *tile_list;
     double *wdisp;       // per 64-slot wave: largest squared displacement since the
                          // build, written by k_integrate on a box's review step
     // boxes

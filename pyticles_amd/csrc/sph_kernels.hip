@@ -264,8 +264,30 @@ __device__ __forceinline__ void per_box(int b, F &&f)
 // skip slots of boxes that do not build.
 
 static constexpr int PLAN_THREADS = 1024;
+static constexpr int PLAN_SEGS = 4096;  // segment tables up to this size are cached in LDS
 
 // plan_n[1] = M (building slots); box_coff[b] for each building box b.
+// Building tiles of segment g (64-slot tiles holding its slots, if its box
+// builds): count, first tile in t0.  A tile is listed by the segment holding
+// its first building slot, so a tile shared with an earlier building
+// segment is left to that one.
+__device__ __forceinline__ int seg_tiles(const ArenaView &A, int g, int force_mask, int &t0)
+{
+    const SegDesc sg = A.segs[g];
+    if (sg.box < 0 || sg.len <= 0 || !building(A.dyn[sg.box], force_mask)) return 0;
+    t0 = sg.off >> 6;
+    const int t1 = (sg.off + sg.len - 1) >> 6;
+    for (int k = g - 1; k >= 0; --k) {
+        const SegDesc pv = A.segs[k];
+        if (pv.off + pv.len <= t0 * 64) break;  // ends before tile t0
+        if (pv.box >= 0 && pv.len > 0 && building(A.dyn[pv.box], force_mask)) {
+            ++t0;
+            break;
+        }
+    }
+    return t1 >= t0 ? t1 - t0 + 1 : 0;
+}
+
 __global__ __launch_bounds__(PLAN_THREADS) void k_build_plan(ArenaView A, int force_mask)
 {
     __shared__ int ss[PLAN_THREADS];
@@ -294,6 +316,74 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_build_plan(ArenaView A, int fo
         }
     }
     if (tid == PLAN_THREADS - 1) A.plan_n[1] = ss[tid];
+    // the building tiles, compacted in slot order
+    __shared__ int seg_t0[PLAN_SEGS], seg_end[PLAN_SEGS];  // per segment: first tile, inclusive count
+    const bool cached = A.nseg <= PLAN_SEGS;
+    __syncthreads();
+    int nt = 0;
+    for (int g = g0; g < g1; ++g) {
+        int t0 = 0;
+        const int c = seg_tiles(A, g, force_mask, t0);
+        if (cached) {
+            seg_t0[g] = t0;
+            seg_end[g] = c;  // the range's running count, offset below
+        }
+        nt += c;
+    }
+    ss[tid] = nt;
+    __syncthreads();
+    for (int off = 1; off < PLAN_THREADS; off <<= 1) {
+        const int c = tid >= off ? ss[tid - off] : 0;
+        __syncthreads();
+        ss[tid] += c;
+        __syncthreads();
+    }
+    const int T = ss[PLAN_THREADS - 1];
+    if (cached) {
+        int run = ss[tid] - nt;
+        for (int g = g0; g < g1; ++g) {
+            run += seg_end[g];
+            seg_end[g] = run;
+        }
+        __syncthreads();
+        // entry i: the segment whose inclusive count first passes i
+        for (int i = tid; i < T; i += PLAN_THREADS) {
+            int lo = 0, hi = A.nseg - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (seg_end[mid] > i) hi = mid;
+                else lo = mid + 1;
+            }
+            const int before = lo > 0 ? seg_end[lo - 1] : 0;
+            A.tile_list[i] = seg_t0[lo] + (i - before);
+        }
+        if (tid == PLAN_THREADS - 1) A.plan_n[2] = T;
+        return;
+    }
+    // more segments than the cache holds: entry i of the list, written by
+    // thread i % PLAN_THREADS (one large box's tiles spread over the block):
+    // the thread range holding it by a binary search over the inclusive
+    // counts, then a walk over that range's segments
+    for (int i = tid; i < T; i += PLAN_THREADS) {
+        int lo = 0, hi = PLAN_THREADS - 1;  // first range whose count passes i
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ss[mid] > i) hi = mid;
+            else lo = mid + 1;
+        }
+        int base = lo > 0 ? ss[lo - 1] : 0;
+        const int ga = lo * per, gb = min(ga + per, A.nseg);
+        for (int g = ga; g < gb; ++g) {
+            int t0 = 0;
+            const int c = seg_tiles(A, g, force_mask, t0);
+            if (i < base + c) {
+                A.tile_list[i] = t0 + (i - base);
+                break;
+            }
+            base += c;
+        }
+    }
+    if (tid == PLAN_THREADS - 1) A.plan_n[2] = T;
 }
 
 // Box of slot s if it builds in this launch, else -1.
@@ -586,9 +676,8 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
 // (Over compact positions, which are not tile-aligned, two waves, often on
 // two XCDs with their own L2, wrote most tiles and the kernel ran 2.4x
 // slower.)
-__global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask, int fp32)
+__device__ __forceinline__ void build_slot(const ArenaView &A, int s, int force_mask, int fp32)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
     const int b = building_box(A, s, force_mask);
     if (b < 0) return;
@@ -627,6 +716,26 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
     A.cnt[s] = make_int2(ns, nl);
     max_bits(&D.vmax2, vpi.x * vpi.x + vpi.y * vpi.y);
     max_bits(&D.amax2, ai.x * ai.x + ai.y * ai.y);
+}
+
+// Over the building tiles of the plan (tile_list[0 .. plan_n[2])), one wave
+// per tile.  Blocks are dealt to the XCDs in turn (block i on XCD i % 8), so
+// XCD x's blocks walk the x-th eighth of the list: the work of a few
+// building boxes spreads over the whole chip, while neighbouring tiles,
+// which read much the same positions, share an XCD's L2.  (Over all slots,
+// with the slot blocks of each XCD contiguous, a step in which a few small
+// boxes built left most XCDs idle and launched a wave per 64 slots of the
+// arena.)  The grid is at least 8 blocks, so every XCD share has a block.
+__global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask, int fp32)
+{
+    constexpr int WPB = NT / 64;  // waves per block
+    const int x = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int nbx = ((int)gridDim.x - x + 7) >> 3;  // blocks on this XCD
+    const int T = A.plan_n[2];
+    const int k0 = (int)(((long long)T * x) >> 3), k1 = (int)(((long long)T * (x + 1)) >> 3);
+    const int lane = threadIdx.x & 63;
+    for (int k = k0 + local * WPB + (int)(threadIdx.x >> 6); k < k1; k += nbx * WPB)
+        build_slot(A, A.tile_list[k] * 64 + lane, force_mask, fp32);
 }
 
 // ---------------------------------------------------------------------------
@@ -2119,7 +2228,12 @@ void launch_cell_start(const ArenaView &A, int force_mask, hipStream_t st)
 }
 void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_build_lists, grid_for(A.n), dim3(NT), 0, st, A, force_mask, fp32);
+    // one wave per building tile at most: enough blocks for every tile of
+    // the slots in use, a multiple of 8 (an equal share per XCD), 8 to 2048
+    long long nb = ((long long)A.n + NT - 1) / NT;
+    nb = (nb + 7) / 8 * 8;
+    nb = nb < 8 ? 8 : (nb > 2048 ? 2048 : nb);
+    hipLaunchKernelGGL(k_build_lists, dim3((unsigned)nb), dim3(NT), 0, st, A, force_mask, fp32);
 }
 // unroll: entries in flight per lane (1, 2 or 4; anything else = default)
 #define SPH_DISPATCH(K64, K32, ...)                                                              \
