@@ -110,8 +110,8 @@ static size_t arena_bytes(size_t cap, int prec, int lcap)
     b += cap * (2 * d2 + d1);                             // vt, aco, uco
     if (prec == SPH_FP32C) b += cap * (4 * sizeof(float4) + sizeof(float2));  // rf S32 vq32 vt32, vf
     b += cap * 5 * sizeof(unsigned);             // sort keys / values
-    b += (cap + 63) / 64 * sizeof(double);       // per-wave displacement
     b += ((cap + 63) / 64 + 1) * i1;             // building tiles
+    b += (cap + 63) / 64 * sizeof(double);       // per-wave displacement
     b += radix_sort_temp_bytes((int)cap);
     return b;
 }
@@ -145,8 +145,8 @@ struct Arena {
     double *uco = nullptr;
     unsigned *keys_in = nullptr, *k_out = nullptr, *v_out = nullptr, *k_alt = nullptr, *v_alt = nullptr;
     int *plan_n = nullptr, *box_coff = nullptr;  // box_coff sized box_cap
-    double *wdisp = nullptr;                      // one per 64 slots
     int *tile_list = nullptr;                     // building tiles, one per 64 slots
+    double *wdisp = nullptr;                      // one per 64 slots
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int *cell_start = nullptr;
@@ -217,7 +217,7 @@ struct Arena {
         A.vt = vt; A.aco = aco; A.uco = uco;
         A.cell_start = cell_start;
         A.keys = k_out; A.vals = v_out; A.keys_in = keys_in;
-        A.plan_n = plan_n; A.box_coff = box_coff; A.wdisp = wdisp; A.tile_list = tile_list;
+        A.plan_n = plan_n; A.box_coff = box_coff; A.tile_list = tile_list; A.wdisp = wdisp;
         A.boxes = d_boxes; A.dyn = d_dyn; A.segs = d_segs;
         A.nseg = (int)segs.size();
         A.n = hwm;
@@ -256,8 +256,8 @@ struct Arena {
         rc |= dalloc((char **)&sort_tmp, sort_tmp_bytes);
         rc |= dalloc(&d_e3, 3);
         rc |= dalloc(&plan_n, 4);
-        rc |= dalloc(&wdisp, (size_t)(cap + 63) / 64);
         rc |= dalloc(&tile_list, (size_t)(cap + 63) / 64 + 1);
+        rc |= dalloc(&wdisp, (size_t)(cap + 63) / 64);
         if (rc) return SPH_ERR_NOMEM;
         // zero-fill the state so that unused slots hold finite values
         size_t z2 = sizeof(double2) * cap, z1 = sizeof(double) * cap;
@@ -285,7 +285,7 @@ struct Arena {
         dfree(rho); dfree(Q); dfree(jq); dfree(T); dfree(e); dfree(S); dfree(gradv);
         dfree(rf); dfree(S32); dfree(vf); dfree(vq32); dfree(vt32); dfree(vt); dfree(aco); dfree(uco);
         dfree(keys_in); dfree(k_out); dfree(v_out); dfree(k_alt); dfree(v_alt);
-        dfree(plan_n); dfree(box_coff); dfree(wdisp); dfree(tile_list);
+        dfree(plan_n); dfree(box_coff); dfree(tile_list); dfree(wdisp);
         if (sort_tmp) (void)hipFree(sort_tmp);
         sort_tmp = nullptr;
         if (snap_pending) (void)hipEventSynchronize(ev_snap_done);

@@ -85,6 +85,9 @@ struct BoxDyn {
     double vlast, alast;    // max |v|, |a| measured at the last build
     int64_t built_at;       // step index of the last build
     int64_t extends;        // scheduled builds postponed on the measured displacement
+    unsigned long long dmax2;  // largest squared displacement of the box's slots in
+                               // waves shared with other boxes, gathered on its review
+                               // step (double bits; cleared when read)
 };
 
 // Segment of the slot space: a box or a free range, sorted by offset.
@@ -138,8 +141,8 @@ struct ArenaView {
     int *plan_n;
     int *box_coff;
     int *tile_list;
-    double *wdisp;       // per 64-slot wave: largest squared displacement since the
-                         // build, written by k_integrate on a box's review step
+    double *wdisp;       // per 64-slot wave wholly inside one box: its largest squared
+                         // displacement since the build, stored on the box's review step
     // boxes
     const BoxDesc *boxes;
     BoxDyn *dyn;

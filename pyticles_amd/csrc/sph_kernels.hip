@@ -1691,33 +1691,57 @@ __device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, i
 }
 
 // A running box re-examines its list at the step before a scheduled build
-// (k_box_step, extend_steps); on that step k_integrate leaves the largest
-// squared displacement of every 64-slot wave in wdisp.
+// (k_box_step, extend_steps); on that step k_integrate gathers the largest
+// squared displacement of the box's slots: per wave in wdisp for the waves
+// wholly inside the box, in its dyn.dmax2 for the waves it shares.
 __device__ __forceinline__ bool list_review(const BoxDyn &D)
 {
     return D.state == BOX_RUN && !building(D, 0) && D.k + 1 < D.nsteps && D.k + 1 >= D.next_rebuild;
 }
 
-// Integrate every running slot; on a box's review step also store the wave's
-// largest squared displacement (a shuffle maximum over all 64 lanes, lanes of
-// other boxes included, which only over-estimates; no early return, so every
-// lane takes part).
+// Integrate every running slot; on a box's review step also record the
+// largest squared displacement of its lanes.  A wave holds 64 consecutive
+// slots and a box a contiguous slot range, so a wave whose first and last
+// lanes share a box lies wholly inside it: it stores its maximum in wdisp.
+// A wave shared by several boxes (at most two per box) folds each reviewing
+// box's lanes into that box's dyn.dmax2 with one atomic.  Each box thus sees
+// only its own slots, so its schedule, and with it its results, do not depend
+// on the boxes beside it.  (One atomic per wave into a box field made every
+// review step of a large box slow, the waves queueing on one address.)  No
+// early return before the shuffles, so every lane takes part.
 __global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
 {
     const int s = xcd_block() * NT + threadIdx.x;
     double d2 = 0.0;
     bool review = false;
+    int b = -1;
     if (s < A.n) {
-        const int b = A.slot_box[s];
+        b = A.slot_box[s];
         if (b >= 0 && A.dyn[b].state == BOX_RUN) {
             d2 = fmax(integrate_slot(A, fp32, s, b), 0.0);
             review = list_review(A.dyn[b]);
         }
     }
-    if (__ballot(review) == 0ull) return;
+    uint64_t pending = __ballot(review);
+    if (pending == 0ull) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int bfirst = __shfl(b, 0);
+    if (bfirst >= 0 && bfirst == __shfl(b, 63)) {  // wholly inside one box
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off));
-    if ((threadIdx.x & 63) == 0) A.wdisp[s >> 6] = d2;
+        for (int off = 32; off >= 1; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off));
+        if (lane == 0) A.wdisp[s >> 6] = d2;
+        return;
+    }
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int bu = __shfl(b, leader);
+        const bool mine = review && b == bu;
+        double m = mine ? d2 : 0.0;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+        if (lane == leader) max_bits(&A.dyn[bu].dmax2, m);
+        pending &= ~__ballot(mine);
+    }
 }
 
 // ===========================================================================
@@ -1783,9 +1807,9 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
     const bool review = list_review(D);
     __shared__ double red[NT_BOX];
     double dmax2 = 0.0;
-    if (review) {  // largest squared displacement over the box's waves
+    if (review) {  // largest squared displacement: the box's whole waves, then its shared ones
         const BoxDesc &B = A.boxes[b];
-        const int w0 = B.part_off >> 6, w1 = (B.part_off + B.n - 1) >> 6;
+        const int w0 = (B.part_off + 63) >> 6, w1 = ((B.part_off + B.n) >> 6) - 1;
         double m = 0.0;
         for (int w = w0 + (int)threadIdx.x; w <= w1; w += NT_BOX) m = fmax(m, A.wdisp[w]);
         red[threadIdx.x] = m;
@@ -1794,7 +1818,7 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
             if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
             __syncthreads();
         }
-        dmax2 = red[0];
+        dmax2 = fmax(red[0], __longlong_as_double((long long)D.dmax2));
         __syncthreads();  // red is reused below
     }
     if (capture) {
@@ -1833,7 +1857,9 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
         D.amax2 = 0ull;
     } else if (review) {
         // the next step would build: postpone while the measured displacement
-        // leaves room, by the same bound as reuse_steps on what is left
+        // (gathered by k_integrate this step) leaves room, by the same bound
+        // as reuse_steps on what is left
+        D.dmax2 = 0ull;
         const int64_t c = extend_steps(A.boxes[b], D, dmax2);
         if (c > 0) {
             D.next_rebuild = k + 1 + c;

@@ -54,6 +54,29 @@ def test_batched_equals_alone(precision):
 
 
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32C])
+def test_boxes_smaller_than_a_tile(precision):
+    # Boxes of 16-60 slots with a freed gap among them: one 64-slot tile
+    # holds parts of three or more boxes and free slots, and each box
+    # rebuilds its lists on its own schedule, so the list build's tile plan
+    # meets tiles shared by building and resting boxes.
+    boxes = [configs.lognormal_box(seed, 16, 60, 0.3, 1.0, steps=st) for seed, st in
+             ((201, 23), (202, 31), (203, 17), (204, 40), (205, 29), (206, 35))]
+    boxes.append(configs.lognormal_box(207, 300, 301, 0.8, 0.9, steps=26))
+    ref = [alone(s, precision) for s in boxes]
+    gap = configs.lognormal_box(208, 20, 21, 0.5, 0.6, steps=5)
+    with Arena(sum(s.n for s in boxes) + gap.n + 200, precision=precision) as ar:
+        ids = [ar.add(s) for s in boxes[:3]]
+        g = ar.add(gap)
+        ids += [ar.add(s) for s in boxes[3:]]
+        ar.remove(g)
+        run_to_completion(ar, ids)
+        for i, rf in zip(ids, ref):
+            got = ar.result(i)
+            for k in ("r", "v", "u", "rho"):
+                assert np.array_equal(got[k], rf[k]), (precision, i, k)
+
+
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32C])
 def test_box_sort_and_radix_sort_agree(precision):
     # Alone, a small box is ordered by the per-box counting sort; beside a
     # box with more than 2048 cells the arena uses the global radix sort for
