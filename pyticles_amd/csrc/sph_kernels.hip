@@ -586,10 +586,15 @@ __global__ __launch_bounds__(BS_THREADS) void k_box_sort(ArenaView A, int force_
 // Entries collect in two four-entry buffers (short from position 0 up, long
 // from lcap - 1 down) stored a group at a time; the partial groups are
 // stored at the end, merged when they share a group.  Entries that do not
-// fit (ns + nl = lcap) set lost.
+// fit (ns + nl = lcap) set lost.  The buffers are the lane's column of an
+// LDS array (buf[row * NT]), so an append is one indexed LDS write on one
+// path for short and long entries; in registers each append took a select
+// chain per buffer, and the wave ran both chains whenever its lanes split.
+// Positions past a count hold stale words, which the walks never read
+// (ListRounds maps them to the slot itself).
 template <bool SINGLE>
 __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu, int s, double2 ri, int cell,
-                                          unsigned *lb, int &ns, int &nl, bool &lost)
+                                          unsigned *lb, unsigned *buf, int &ns, int &nl, bool &lost)
 {
     const int lcap = A.lcap;
     const double rnl2 = wave_uniform(Bu.rnl2), hsk2 = wave_uniform(Bu.hsk2);
@@ -599,7 +604,6 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
     const int sx = wave_uniform(Bu.sx), sy = wave_uniform(Bu.sy);
     const int *cs = A.cell_start + wave_uniform(Bu.key_base);
     const int cx = cell % ncx, cy = cell / ncx;
-    unsigned s0 = 0, s1 = 0, s2 = 0, s3 = 0, l0 = 0, l1 = 0, l2 = 0, l3 = 0;
     for (int oy = -sy; oy <= sy; ++oy) {
         const int yr = cy + oy;
         const int yy = yr < 0 ? yr + ncy : (yr >= ncy ? yr - ncy : yr);
@@ -627,25 +631,24 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
                     }
                     const double r2 = rsq_exact(dx, dy);
                     if (t + u < t1 && r2 < rnl2 && t + u != s) {
-                        const unsigned e = (unsigned)(t + u);
                         if (ns + nl >= lcap) {
                             lost = true;
-                        } else if (r2 < hsk2) {  // short: offset ns % 4 of group ns / 4
-                            const int o = ns & 3;
-                            s0 = o == 0 ? e : s0;
-                            s1 = o == 1 ? e : s1;
-                            s2 = o == 2 ? e : s2;
-                            s3 = o == 3 ? e : s3;
-                            ++ns;
-                            if (o == 3) *(uint4 *)(lb + (size_t)((ns - 4) >> 2) * LIST_ROW) = make_uint4(s0, s1, s2, s3);
-                        } else {  // long entry nl at position lcap - 1 - nl: offset 3 - nl % 4
-                            const int o = 3 - (nl & 3);
-                            l0 = o == 0 ? e : l0;
-                            l1 = o == 1 ? e : l1;
-                            l2 = o == 2 ? e : l2;
-                            l3 = o == 3 ? e : l3;
-                            ++nl;
-                            if (o == 0) *(uint4 *)(lb + (size_t)((lcap - nl) >> 2) * LIST_ROW) = make_uint4(l0, l1, l2, l3);
+                        } else {
+                            // short entry ns at offset ns % 4 of buffer row 0-3, long
+                            // entry nl (position lcap - 1 - nl) at offset 3 - nl % 4
+                            // of rows 4-7; one path for both, the group stored
+                            // when its last offset fills
+                            const bool sh = r2 < hsk2;
+                            const int row = sh ? (ns & 3) : 7 - (nl & 3);
+                            buf[row * NT] = (unsigned)(t + u);
+                            ns += sh ? 1 : 0;
+                            nl += sh ? 0 : 1;
+                            if (row == 3 || row == 4) {
+                                const int g0 = sh ? 0 : 4;
+                                const int grp = sh ? ((ns - 4) >> 2) : ((lcap - nl) >> 2);
+                                *(uint4 *)(lb + (size_t)grp * LIST_ROW) =
+                                    make_uint4(buf[g0 * NT], buf[(g0 + 1) * NT], buf[(g0 + 2) * NT], buf[(g0 + 3) * NT]);
+                            }
                         }
                     }
                 }
@@ -657,6 +660,8 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
     // they never overlap, and a shared group is stored once
     const int ps = ns & 3, pl = nl & 3;
     const int gs = ns >> 2, gl = (lcap - nl) >> 2;
+    const unsigned s0 = buf[0], s1 = buf[NT], s2 = buf[2 * NT], s3 = buf[3 * NT];
+    const unsigned l0 = buf[4 * NT], l1 = buf[5 * NT], l2 = buf[6 * NT], l3 = buf[7 * NT];
     if (ps && pl && gs == gl) {
         *(uint4 *)(lb + (size_t)gs * LIST_ROW) =
             make_uint4(0 < ps ? s0 : l0, 1 < ps ? s1 : l1, 2 < ps ? s2 : l2, 3 < ps ? s3 : l3);
@@ -676,7 +681,7 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
 // (Over compact positions, which are not tile-aligned, two waves, often on
 // two XCDs with their own L2, wrote most tiles and the kernel ran 2.4x
 // slower.)
-__device__ __forceinline__ void build_slot(const ArenaView &A, int s, int force_mask, int fp32)
+__device__ __forceinline__ void build_slot(const ArenaView &A, int s, int force_mask, int fp32, unsigned *buf)
 {
     if (s >= A.n) return;
     const int b = building_box(A, s, force_mask);
@@ -709,8 +714,8 @@ __device__ __forceinline__ void build_slot(const ArenaView &A, int s, int force_
     // needs no minimum image (axes with a single cell keep it).
     per_box(b, [&](int bu) {
         const BoxDesc &Bu = A.boxes[bu];
-        if (wave_uniform(Bu.sx) == 0 || wave_uniform(Bu.sy) == 0) list_walk<true>(A, Bu, s, ri, cell, lb, ns, nl, lost);
-        else list_walk<false>(A, Bu, s, ri, cell, lb, ns, nl, lost);
+        if (wave_uniform(Bu.sx) == 0 || wave_uniform(Bu.sy) == 0) list_walk<true>(A, Bu, s, ri, cell, lb, buf, ns, nl, lost);
+        else list_walk<false>(A, Bu, s, ri, cell, lb, buf, ns, nl, lost);
     });
     if (lost) flag_health(A.dyn, b, HEALTH_LISTCAP);
     A.cnt[s] = make_int2(ns, nl);
@@ -734,8 +739,11 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
     const int T = A.plan_n[2];
     const int k0 = (int)(((long long)T * x) >> 3), k1 = (int)(((long long)T * (x + 1)) >> 3);
     const int lane = threadIdx.x & 63;
+    // each lane's two four-entry groups being filled (short, long), one
+    // column per lane: row r of lane t at lbuf[r * NT + t], bank-conflict free
+    __shared__ unsigned lbuf[8 * NT];
     for (int k = k0 + local * WPB + (int)(threadIdx.x >> 6); k < k1; k += nbx * WPB)
-        build_slot(A, A.tile_list[k] * 64 + lane, force_mask, fp32);
+        build_slot(A, A.tile_list[k] * 64 + lane, force_mask, fp32, lbuf + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
