@@ -475,10 +475,13 @@ __global__ __launch_bounds__(NT) void k_cell_start(ArenaView A, int force_mask)
 // is the same stable one (by cell, then by compact position), so the two
 // pipelines give identical lists.  Ranks come from wave ballots and per-wave
 // counters (each wave owns one counter row and walks its slice of the box in
-// order), so there are no atomics and the result is deterministic.
-static constexpr int BS_THREADS = 512;
+// order), so there are no atomics and the result is deterministic.  1024
+// threads and 1024 cells (40 KB of LDS): the block of the largest box sets
+// the kernel's time, and with 512 threads and 2048 cells it took twice as
+// long per slot; boxes of up to 16,384 slots have at most 784 cells.
+static constexpr int BS_THREADS = 1024;
 static constexpr int BS_WAVES = BS_THREADS / 64;
-static constexpr int BS_CELL_BITS = 11;
+static constexpr int BS_CELL_BITS = 10;
 static constexpr int BS_MAX_CELLS = 1 << BS_CELL_BITS;
 
 __device__ __forceinline__ uint64_t lanes_below()

@@ -79,12 +79,12 @@ def test_boxes_smaller_than_a_tile(precision):
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32C])
 def test_box_sort_and_radix_sort_agree(precision):
     # Alone, a small box is ordered by the per-box counting sort; beside a
-    # box with more than 2048 cells the arena uses the global radix sort for
-    # all boxes.  Both are stable by (cell, slot), so results agree bitwise.
+    # box beyond its limits (more than 16,384 slots and 1,024 cells) the
+    # arena uses the global radix sort for all boxes.  Both are stable by (cell, slot), so results agree bitwise.
     boxes = small_boxes()
     ref = [alone(s, precision) for s in boxes]
     big = configs.lognormal_box(110, 44000, 44001, 0.6, 0.7, steps=45)
-    assert (int(big.prm["Lx"] / 4.5) ** 2) > 2048
+    assert big.n > 16384 and (int(big.prm["Lx"] / 4.5) ** 2) > 1024
     with Arena(sum(s.n for s in boxes) + big.n + 64, precision=precision) as ar:
         ids = [ar.add(s) for s in boxes]
         ib = ar.add(big)
