@@ -129,6 +129,10 @@ struct ArenaView {
     float4 *S32;
     float4 *vq32;        // (v, q / rho^2): neighbour record of the rate pass
     float4 *vt32;        // (v, m / rho, T): the gradient pass's neighbour record
+    // boxes with box-relative records (rf = x, y, v): the velocity comes with
+    // the position, so the other records drop it
+    float2 *mt32;        // (m / rho, T) for the gradient pass
+    float2 *q32;         // q / rho^2 for the rate pass
     // cells and sort
     int *cell_start;
     unsigned *keys;      // sorted keys (final buffer of the radix sort)

@@ -1446,8 +1446,13 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     A.S[s] = make_double4(ps, 0.0, ps, mi);
     A.S32[s] = make_float4((float)ps, 0.f, (float)ps, (float)mi);
     A.Q[s] = make_double2(0.0, 0.0);
-    A.vq32[s] = make_float4(vi.x, vi.y, 0.f, 0.f);
-    A.vt32[s] = make_float4(vi.x, vi.y, (float)(mi / rd), (float)T);
+    if (B.fixbox) {  // rf carries v
+        A.q32[s] = make_float2(0.f, 0.f);
+        A.mt32[s] = make_float2((float)(mi / rd), (float)T);
+    } else {
+        A.vq32[s] = make_float4(vi.x, vi.y, 0.f, 0.f);
+        A.vt32[s] = make_float4(vi.x, vi.y, (float)(mi / rd), (float)T);
+    }
     A.aco[s] = coh ? make_double2(-2.0 * B.a * d.gx.value(), -2.0 * B.a * d.gy.value()) : make_double2(0.0, 0.0);
     A.uco[s] = coh ? -B.a * d.ev.value() : 0.0;
     if (!(rd * B.b < 1.0)) flag_health(A.dyn, b, HEALTH_EOS_POLE);
@@ -1461,10 +1466,19 @@ struct Grad32 {
 template <int U32, int WRAP>
 __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, int s, Grad32 &g)
 {
+    // WRAP 3 (box-relative records): v from rf, (m/rho, T) from the 8-byte
+    // mt32, so a neighbour costs 24 bytes of gathers instead of 32
     const float4 ri = A.rf[s];
-    const float4 vti = A.vt32[s];
-    const float2 vi = make_float2(vti.x, vti.y);
-    const float Ti = vti.w;
+    float2 vi;
+    float Ti;
+    if (WRAP == 3) {
+        vi = make_float2(ri.z, ri.w);
+        Ti = A.mt32[s].y;
+    } else {
+        const float4 vti = A.vt32[s];
+        vi = make_float2(vti.x, vti.y);
+        Ti = vti.w;
+    }
     const int ns = A.cnt[s].x;
     const unsigned *nb = list_base(A, s);
     ListRounds<U32, false, false> rounds(nb, A.lcap, ns, s);
@@ -1475,7 +1489,12 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
-            vt[u] = A.vt32[t[u]];
+            if (WRAP == 3) {
+                const float2 mt = A.mt32[t[u]];
+                vt[u] = make_float4(0.f, 0.f, mt.x, mt.y);
+            } else {
+                vt[u] = A.vt32[t[u]];
+            }
         }
         rounds.next(k0);
         float a00 = 0.f, a01 = 0.f, a10 = 0.f, a11 = 0.f, at0 = 0.f, at1 = 0.f;
@@ -1490,7 +1509,8 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
                 const float Fk = F.fh * w * w;
                 const float gx = -Fk * dx, gy = -Fk * dy;
                 const float vj = vt[u].z;
-                const float dvx = vt[u].x - vi.x, dvy = vt[u].y - vi.y;
+                const float vjx = WRAP == 3 ? rj[u].z : vt[u].x, vjy = WRAP == 3 ? rj[u].w : vt[u].y;
+                const float dvx = vjx - vi.x, dvy = vjy - vi.y;
                 const float dT = vt[u].w - Ti;
                 a00 += vj * dvx * gx;
                 a01 += vj * dvx * gy;
@@ -1509,8 +1529,9 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
     }
 }
 
-// P2 in FP32; neighbour record: rf and (v, m/rho, T) from P1.  Writes the rate
-// pass's (v, q/rho^2) record.
+// P2 in FP32; neighbour record: rf and (v, m/rho, T) from P1, or rf and
+// (m/rho, T) for box-relative records.  Writes the rate pass's (v, q/rho^2)
+// record (q/rho^2 alone for box-relative records).
 // Five waves per SIMD (94 VGPRs, no spill) instead of the compiler's four:
 // 4% faster at C4.  The other pass kernels gained nothing or spilled.
 template <int U32>
@@ -1552,7 +1573,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
         qy = -B.kcond * g.gt1.value();
         A.Q[s] = make_double2(qx / r2i, qy / r2i);
     }
-    A.vq32[s] = make_float4(vi.x, vi.y, (float)(qx / r2i), (float)(qy / r2i));
+    if (B.fixbox) A.q32[s] = make_float2((float)(qx / r2i), (float)(qy / r2i));
+    else A.vq32[s] = make_float4(vi.x, vi.y, (float)(qx / r2i), (float)(qy / r2i));
     if (diag) {
         A.gradv[s] = make_double4(G00, G01, G10, G11);
         A.jq[s] = make_double2(qx, qy);
@@ -1566,8 +1588,16 @@ struct Rate32 {
 template <int U32, bool COND, int WRAP>
 __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, int s, int ns, Rate32 &o)
 {
+    // WRAP 3 (box-relative records): v from rf and q/rho^2 from the 8-byte
+    // q32, fetched only with conduction
     const float4 ri = A.rf[s];
-    const float4 vqi = A.vq32[s];
+    float4 vqi;
+    if (WRAP == 3) {
+        const float2 qi = COND ? A.q32[s] : make_float2(0.f, 0.f);
+        vqi = make_float4(ri.z, ri.w, qi.x, qi.y);
+    } else {
+        vqi = A.vq32[s];
+    }
     const float4 Si = A.S32[s];
     const unsigned *nb = list_base(A, s);
     ListRounds<U32, false, false> rounds(nb, A.lcap, ns, s);
@@ -1579,7 +1609,12 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
             Sj[u] = A.S32[t[u]];
-            vq[u] = A.vq32[t[u]];
+            if (WRAP == 3) {
+                const float2 qj = COND ? A.q32[t[u]] : make_float2(0.f, 0.f);
+                vq[u] = make_float4(rj[u].z, rj[u].w, qj.x, qj.y);
+            } else {
+                vq[u] = A.vq32[t[u]];
+            }
         }
         rounds.next(k0);
         float px = 0.f, py = 0.f, pu = 0.f;
@@ -1611,7 +1646,7 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
 }
 
 // P3 in FP32 over the short entries; neighbour record: rf, (v, q/rho^2),
-// (S, m).  Adds the cohesive part from P1.
+// (S, m); for box-relative records rf, (S, m) and, with conduction, q/rho^2.  Adds the cohesive part from P1.
 template <int U32>
 __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
 {
