@@ -70,11 +70,12 @@ def workload(name, precision):
 
 def schedule(s, cls):
     """Apply the autotuned (or default) schedule of a workload class to a
-    configs.System: sets its skin and returns (unroll, schedule)."""
+    configs.System: sets its skin and returns (launch schedule, schedule);
+    the first goes to set_schedule (pass unrolls)."""
     from pyticles_amd import autotune
     sc = autotune.schedule_for(s.prm["precision"], cls)
     s.prm["skin"] = sc["skin"]
-    return sc["unroll"], sc
+    return sc, sc
 
 
 def device_box(s, unroll):
@@ -280,7 +281,7 @@ def run_arena(boxes, precision, chunk=25):
     sc = autotune.schedule_for(precision, "batched")
     total = sum(b.n for b in boxes)
     ar = Arena(total, precision=precision)
-    ar.set_schedule(sc["unroll"])
+    ar.set_schedule(sc)
     ids = [ar.add(autotune._Resched(b, autotune.with_schedule(b, sc, precision))) for b in boxes]
     ar.poll_all()
     t = time.perf_counter()
@@ -424,7 +425,7 @@ def main():
         sc = autotune.schedule_for(_lib.FP32C, "batched")
         boxes = [configs.c4_box(b) for b in range(512)]
         ar = Arena(sum(b.n for b in boxes), precision=_lib.FP32C)
-        ar.set_schedule(sc["unroll"])
+        ar.set_schedule(sc)
         for b in boxes:
             ar.add(autotune._Resched(b, autotune.with_schedule(b, sc, _lib.FP32C)))
         t = time.perf_counter()

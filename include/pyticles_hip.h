@@ -236,6 +236,12 @@ int sph_box_set_graph(sph_box *box, int on);
  * to summation order.  No counterpart in the reference. */
 int sph_arena_set_schedule(sph_arena *ar, int unroll);
 int sph_box_set_schedule(sph_box *box, int unroll);
+/* The same for the rate pass alone (0 = as the other passes): its walk keeps
+ * five records per entry in flight, so its best depth differs (two at one
+ * large FP64 box, where four cost occupancy; four for batched boxes).
+ * Tuned per workload class.  No counterpart in the reference. */
+int sph_arena_set_rates_unroll(sph_arena *ar, int unroll);
+int sph_box_set_rates_unroll(sph_box *box, int unroll);
 /* Device bytes an arena of particle_capacity slots allocates (the serving
  * loop's HBM-budget admission).  No counterpart in the reference. */
 int sph_arena_footprint(int64_t particle_capacity, int precision, int64_t *bytes);

@@ -1,12 +1,16 @@
 """Launch-schedule autotuner for the pair passes.
 
-Two knobs set how the device walks a box's neighbours, and neither changes
+Three knobs set how the device walks a box's neighbours, and none changes
 which pairs interact (only the summation order, so results agree to
 rounding):
 
     unroll  list entries whose loads are in flight per lane in the density,
             gradient and rate passes (1, 2 or 4): more hides more gather
             latency, fewer keeps more waves resident
+    rates_unroll
+            the same for the rate pass alone, whose walk keeps five records
+            per entry in flight (two was best for one large FP64 box, four
+            for batched boxes); tuned after the other two
     skin    neighbour-list skin: lists hold pairs within r_cut + skin and are
             reused while no particle can have crossed half of it; a larger
             skin rebuilds less often and walks longer lists
@@ -67,7 +71,7 @@ def save_cache(data, path=CACHE):
 
 def valid(s):
     return (isinstance(s, dict) and s.get("unroll") in UNROLLS and isinstance(s.get("skin"), (int, float))
-            and 0.0 < float(s["skin"]) <= 2.0)
+            and 0.0 < float(s["skin"]) <= 2.0 and s.get("rates_unroll", 0) in (0,) + UNROLLS)
 
 
 def schedule_for(precision, cls, path=CACHE):
@@ -76,9 +80,10 @@ def schedule_for(precision, cls, path=CACHE):
     entry = load_cache(path).get(key(precision, cls))
     if entry and valid(entry.get("best")):
         b = entry["best"]
-        return dict(unroll=int(b["unroll"]), skin=float(b["skin"]), source="tuned")
+        return dict(unroll=int(b["unroll"]), skin=float(b["skin"]), rates_unroll=int(b.get("rates_unroll", 0)),
+                    source="tuned")
     d = DEFAULT[int(precision)]
-    return dict(d, source="default")
+    return dict(d, rates_unroll=0, source="default")
 
 
 def candidates():
@@ -94,7 +99,7 @@ def _time_box(sysm, sched, precision, steps, warm):
     from .device import from_system
     b = from_system(_Resched(sysm, with_schedule(sysm, sched, precision)))
     try:
-        b.set_schedule(sched["unroll"])
+        b.set_schedule(sched)
         b.step(warm)
         t = time.perf_counter()
         b.step(steps)
@@ -109,7 +114,7 @@ def _time_batched(systems, sched, precision, steps, warm):
     total = sum(s.n for s in systems)
     ar = Arena(total, precision=precision)
     try:
-        ar.set_schedule(sched["unroll"])
+        ar.set_schedule(sched)
         for s in systems:
             ar.add(_Resched(s, with_schedule(s, sched, precision)), nsteps=steps + warm + 10)
         ar.step(warm)
@@ -158,6 +163,20 @@ def tune(systems, precision, cls=None, steps=None, warm=100, save=True, path=CAC
             log("tune %s unroll=%d skin=%.2f  %.3e particle-steps/s" % (key(precision, cls), c["unroll"],
                                                                         c["skin"], rate))
     best = pick(table)
+    # then the rate pass's own depth at that schedule
+    rtable = []
+    for ru in UNROLLS:
+        c = dict(best, rates_unroll=ru)
+        if cls == "box":
+            rate = _time_box(systems[0], c, precision, steps, warm)
+        else:
+            rate = _time_batched(systems, c, precision, steps, warm)
+        rtable.append(dict(c, rate=rate))
+        if log:
+            log("tune %s unroll=%d skin=%.2f rates_unroll=%d  %.3e particle-steps/s" % (
+                key(precision, cls), c["unroll"], c["skin"], ru, rate))
+    best["rates_unroll"] = pick_rates(rtable, best["unroll"])
+    table += rtable
     if save:
         data = load_cache(path)
         data[key(precision, cls)] = dict(
@@ -175,6 +194,16 @@ def pick(table, within=0.03):
     near = [e for e in table if e["rate"] >= (1.0 - within) * top]
     best = min(near, key=lambda e: (e["skin"], -e["rate"]))
     return dict(unroll=best["unroll"], skin=best["skin"])
+
+
+def pick_rates(rtable, unroll, within=0.01):
+    """The rate pass's depth: the fastest, unless the overall unroll is
+    within `within` of it (then 0, meaning the same as the other passes)."""
+    top = max(rtable, key=lambda e: e["rate"])
+    same = [e for e in rtable if e["rates_unroll"] == unroll]
+    if same and same[0]["rate"] >= (1.0 - within) * top["rate"]:
+        return 0
+    return int(top["rates_unroll"])
 
 
 def rate_spread(table):

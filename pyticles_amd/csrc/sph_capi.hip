@@ -202,6 +202,8 @@ struct Arena {
     int always_build = 0;  // arenas: build stage every step (per-box schedule on device)
     bool small_boxes = false;  // all live boxes fit the per-box sort (set by relayout)
     int unroll = 0;        // list entries in flight per lane in the pair passes (0 = default)
+    int unroll_rates = 0;  // the same for the rate pass alone (0 = unroll)
+    int rates_unroll() const { return unroll_rates ? unroll_rates : unroll; }
 
     bool fp32() const { return precision == SPH_FP32C; }
 
@@ -505,7 +507,7 @@ struct Arena {
         launch_grad(A, f, MASK_RUN, 0, unroll, st);
         mark(T_GRAD, false);
         mark(T_RATES, true);
-        launch_rates(A, f, MASK_RUN, unroll, st);
+        launch_rates(A, f, MASK_RUN, rates_unroll(), st);
         mark(T_RATES, false);
         mark(T_INT, true);
         launch_integrate(A, f, st);
@@ -1073,7 +1075,7 @@ static int eval_box(sph_box *box, int grad, int rates)
     }
     if (rates) {
         A.mark(T_RATES, true);
-        launch_rates(V, f, MASK_ALL, A.unroll, st);
+        launch_rates(V, f, MASK_ALL, A.rates_unroll(), st);
         A.mark(T_RATES, false);
     }
     HIPCHK(hipGetLastError());
@@ -1210,7 +1212,7 @@ int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch)
         for (int i = 0; i < reps; ++i) {
             if (pass == 0) launch_density(V, f, MASK_ALL, A.unroll, st);
             else if (pass == 1) launch_grad(V, f, MASK_ALL, 0, A.unroll, st);
-            else launch_rates(V, f, MASK_ALL, A.unroll, st);
+            else launch_rates(V, f, MASK_ALL, A.rates_unroll(), st);
         }
         e = hipGetLastError();
     }
@@ -1563,6 +1565,27 @@ int sph_box_set_schedule(sph_box *box, int unroll)
 {
     if (!box) return fail(SPH_ERR_ARG, "null box");
     return set_schedule(box->ar, unroll);
+}
+
+static int set_rates_unroll(Arena &A, int unroll)
+{
+    if (unroll != 0 && unroll != 1 && unroll != 2 && unroll != 4)
+        return fail(SPH_ERR_ARG, "rate-pass unroll must be 0 (as the other passes), 1, 2 or 4");
+    A.unroll_rates = unroll;
+    A.invalidate_graph();
+    return SPH_OK;
+}
+
+int sph_arena_set_rates_unroll(sph_arena *ar, int unroll)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    return set_rates_unroll(ar->ar, unroll);
+}
+
+int sph_box_set_rates_unroll(sph_box *box, int unroll)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    return set_rates_unroll(box->ar, unroll);
 }
 
 int sph_arena_footprint(int64_t particle_capacity, int precision, int64_t *bytes)

@@ -115,9 +115,14 @@ class DeviceBox:
     def snapshot_wait(self):
         _lib.check(_lib.lib.sph_box_snapshot_wait(self._h))
 
-    def set_schedule(self, unroll):
-        """List entries in flight per lane in the pair passes (1, 2, 4; 0 = default)."""
+    def set_schedule(self, unroll, rates_unroll=0):
+        """List entries in flight per lane in the pair passes (1, 2, 4; 0 =
+        default) and in the rate pass alone (0 = the same).  unroll may be a
+        schedule dict from autotune.schedule_for."""
+        if isinstance(unroll, dict):
+            unroll, rates_unroll = unroll["unroll"], unroll.get("rates_unroll", 0)
         _lib.check(_lib.lib.sph_box_set_schedule(self._h, int(unroll)))
+        _lib.check(_lib.lib.sph_box_set_rates_unroll(self._h, int(rates_unroll)))
 
     def timing(self, reset=False):
         out = np.zeros(7)

@@ -75,7 +75,7 @@ class Server:
         if arena is None:
             arena = Arena(cap, precision=self.precision)
             if "unroll" in self.schedule:
-                arena.set_schedule(self.schedule["unroll"])
+                arena.set_schedule(self.schedule)
         self.arena = arena
         self.capacity = cap
         self.resident = 0

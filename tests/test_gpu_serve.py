@@ -13,14 +13,14 @@ from pyticles_amd.serve import Server, latency_summary
 pytestmark = pytest.mark.gpu
 
 
-def alone(sysm, precision, unroll=0, skin=None):
+def alone(sysm, precision, unroll=0, skin=None, rates_unroll=0):
     prm = dict(sysm.prm, precision=precision)
     if skin is not None:
         prm["skin"] = skin
     # the schedule is set before the initial density pass (u = kB T - a rho):
     # the FP32 form's partial sums follow the unroll factor
     with DeviceBox(sysm.n, prm) as b:
-        b.set_schedule(unroll)
+        b.set_schedule(unroll, rates_unroll)
         b.upload(r=sysm.r, v=sysm.v, m=sysm.m, u=np.zeros(sysm.n))
         b.init_thermal(sysm.T)
         b.step(sysm.steps)
@@ -64,10 +64,10 @@ def test_unroll_schedules_agree(precision):
     tol = 1e-13 if precision == _lib.FP64 else 1e-7
     s = configs.square_system(24, 0.9, 7, 25)
     base = alone(s, precision, unroll=0)
-    for u in (1, 2, 4):
-        got = alone(s, precision, unroll=u)
+    for u, ru in ((1, 0), (2, 0), (4, 0), (4, 2), (2, 4), (1, 4)):
+        got = alone(s, precision, unroll=u, rates_unroll=ru)
         for k in ("r", "v", "u", "rho"):
-            assert rel(got[k], base[k]) <= tol, (u, k)
+            assert rel(got[k], base[k]) <= tol, (u, ru, k)
 
 
 def test_skin_changes_lists_not_physics():
@@ -84,13 +84,17 @@ def test_bad_schedule_rejected():
         with pytest.raises(_lib.SphError) as ei:
             b.set_schedule(3)
         assert ei.value.code == _lib.SPH_ERR_ARG
+        with pytest.raises(_lib.SphError) as ei:
+            b.set_schedule(2, 3)
+        assert ei.value.code == _lib.SPH_ERR_ARG
 
 
 def test_autotune_small_workload(tmp_path):
     p = str(tmp_path / "sched.json")
     s = configs.square_system(32, 0.9, 10, 10)
     best, table = autotune.tune(s, _lib.FP64, "box", steps=3, warm=1, path=p)
-    assert autotune.valid(best) and len(table) == len(autotune.candidates())
+    assert autotune.valid(best) and len(table) == len(autotune.candidates()) + len(autotune.UNROLLS)
+    assert best["rates_unroll"] in (0,) + autotune.UNROLLS
     assert all(e["rate"] > 0 for e in table)
     assert autotune.schedule_for(_lib.FP64, "box", path=p)["source"] == "tuned"
     boxes = [configs.lognormal_box(400 + j, 256, 600, 0.4, 0.9, steps=20) for j in range(4)]

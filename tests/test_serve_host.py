@@ -124,7 +124,7 @@ def test_autotune_cache_roundtrip(tmp_path):
     assert d["source"] == "default" and d["unroll"] == 2
     autotune.save_cache({autotune.key(_lib.FP32C, "batched"): {"best": {"unroll": 1, "skin": 0.3}}}, p)
     got = autotune.schedule_for(_lib.FP32C, "batched", path=p)
-    assert got == dict(unroll=1, skin=0.3, source="tuned")
+    assert got == dict(unroll=1, skin=0.3, rates_unroll=0, source="tuned")  # a cache without rates_unroll
     # an invalid entry falls back to the default
     autotune.save_cache({autotune.key(_lib.FP32C, "box"): {"best": {"unroll": 3, "skin": 0.3}}}, p)
     assert autotune.schedule_for(_lib.FP32C, "box", path=p)["source"] == "default"
@@ -150,3 +150,12 @@ def test_autotune_pick_prefers_smaller_skin_on_near_ties():
     # equal skins: the faster unroll
     assert autotune.pick([dict(unroll=2, skin=0.5, rate=1.0e9), dict(unroll=4, skin=0.5, rate=1.005e9)]) == \
         dict(unroll=4, skin=0.5)
+
+
+def test_autotune_pick_rates():
+    # the rate pass keeps the overall unroll unless another depth is clearly faster
+    t = [dict(rates_unroll=1, rate=0.90e9), dict(rates_unroll=2, rate=1.10e9), dict(rates_unroll=4, rate=1.00e9)]
+    assert autotune.pick_rates(t, 4) == 2
+    t[1]["rate"] = 1.005e9  # within 1 % of the overall unroll's own rate
+    assert autotune.pick_rates(t, 4) == 0
+    assert autotune.pick_rates(t, 2) == 0
