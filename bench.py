@@ -73,6 +73,8 @@ def schedule(s, cls):
     configs.System: sets its skin and returns (launch schedule, schedule);
     the first goes to set_schedule (pass unrolls)."""
     from pyticles_amd import autotune
+    if cls == "box":
+        cls = autotune.single_box_class(s.n)
     sc = autotune.schedule_for(s.prm["precision"], cls)
     s.prm["skin"] = sc["skin"]
     return sc, sc
@@ -377,6 +379,9 @@ def tune_all(path):
         s.prm["precision"] = prec
         best, table = autotune.tune(s, prec, "box", path=path, log=log)
         res[autotune.key(prec, "box")] = best
+        big = configs.c3(precision=prec)
+        best, table = autotune.tune(big, prec, "large", steps=400, path=path, log=log)
+        res[autotune.key(prec, "large")] = best
         boxes = [configs.c4_box(b, precision=prec) for b in range(0, 512, 8)]
         best, table = autotune.tune(boxes, prec, "batched", path=path, log=log)
         res[autotune.key(prec, "batched")] = best

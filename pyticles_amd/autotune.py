@@ -15,9 +15,10 @@ rounding):
             reused while no particle can have crossed half of it; a larger
             skin rebuilds less often and walks longer lists
 
-The best pair depends on precision (FP64 registers are twice as wide), on box
-size (many small boxes share launches with a sort over all of them) and on
-density.  tune() times every candidate on a sample of the caller's workload
+The best choice depends on precision (FP64 registers are twice as wide), on
+box size (many small boxes share launches with a sort over all of them; a
+box of a million particles fills the chip many times over where one of a
+quarter million fills it about once) and on density.  tune() times every candidate on a sample of the caller's workload
 through the same entry points the workload uses and keeps the fastest;
 results are cached per (precision, workload class) in schedules.json beside
 this file so later runs start from the measured choice.
@@ -43,12 +44,22 @@ def prec_name(precision):
     return "fp32c" if int(precision) == _lib.FP32C else "fp64"
 
 
+CLASSES = ("box", "large", "batched")
+LARGE_N = 500_000  # single boxes above this many particles form their own class
+
+
 def key(precision, cls):
-    """Cache key of a workload class: 'box' (one large box per launch) or
-    'batched' (many boxes sharing an arena's launches)."""
-    if cls not in ("box", "batched"):
-        raise ValueError("workload class must be 'box' or 'batched', got %r" % (cls,))
+    """Cache key of a workload class: 'box' (one box per launch), 'large'
+    (one box of more than LARGE_N particles) or 'batched' (many boxes
+    sharing an arena's launches)."""
+    if cls not in CLASSES:
+        raise ValueError("workload class must be one of %s, got %r" % (CLASSES, cls))
     return "%s:%s" % (prec_name(precision), cls)
+
+
+def single_box_class(n):
+    """Workload class of one box of n particles."""
+    return "large" if int(n) > LARGE_N else "box"
 
 
 def load_cache(path=CACHE):
@@ -149,12 +160,13 @@ def tune(systems, precision, cls=None, steps=None, warm=100, save=True, path=CAC
     if not isinstance(systems, (list, tuple)):
         systems = [systems]
     if cls is None:
-        cls = "box" if len(systems) == 1 else "batched"
+        cls = single_box_class(systems[0].n) if len(systems) == 1 else "batched"
+    single = cls != "batched"
     if steps is None:  # many rebuild cycles, so a skin's list cost and rebuild savings both show
-        steps = 600 if cls == "box" else 300
+        steps = 600 if single else 300
     table = []
     for c in candidates():
-        if cls == "box":
+        if single:
             rate = _time_box(systems[0], c, precision, steps, warm)
         else:
             rate = _time_batched(systems, c, precision, steps, warm)
@@ -167,7 +179,7 @@ def tune(systems, precision, cls=None, steps=None, warm=100, save=True, path=CAC
     rtable = []
     for ru in UNROLLS:
         c = dict(best, rates_unroll=ru)
-        if cls == "box":
+        if single:
             rate = _time_box(systems[0], c, precision, steps, warm)
         else:
             rate = _time_batched(systems, c, precision, steps, warm)
