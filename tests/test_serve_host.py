@@ -137,7 +137,7 @@ def test_autotune_cache_roundtrip(tmp_path):
 def test_committed_schedule_cache_is_valid():
     data = autotune.load_cache()
     for k, e in data.items():
-        assert k.split(":")[0] in ("fp64", "fp32c") and k.split(":")[1] in ("box", "batched")
+        assert k.split(":")[0] in ("fp64", "fp32c") and k.split(":")[1] in autotune.CLASSES
         assert autotune.valid(e["best"])
 
 
