@@ -409,9 +409,31 @@ __global__ __launch_bounds__(NT) void k_slot_box(ArenaView A)
 
 // key = box key base + current cell (row major) at the slot's compact
 // position.
-__global__ __launch_bounds__(NT) void k_keys(ArenaView A, int force_mask)
+// The build kernels run over the building tiles of the plan
+// (tile_list[0 .. plan_n[2])), one wave per tile.  Blocks are dealt to the
+// XCDs in turn (block i on XCD i % 8), so XCD x's blocks walk the x-th eighth
+// of the list: the work of a few building boxes spreads over the whole chip,
+// while neighbouring tiles, which read much the same data, share an XCD's
+// L2.  (Over all slots, with the slot blocks of each XCD contiguous, a step
+// in which a few small boxes built left most XCDs idle and launched a wave
+// per 64 slots of the arena.)  The grid is at least 8 blocks, so every XCD
+// share has a block.  f(s) is called for every slot of each tile; slots that
+// do not build return at once.
+template <class F>
+__device__ __forceinline__ void for_building_tiles(const ArenaView &A, F f)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
+    constexpr int WPB = NT / 64;  // waves per block
+    const int x = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int nbx = ((int)gridDim.x - x + 7) >> 3;  // blocks on this XCD
+    const int T = A.plan_n[2];
+    const int k0 = (int)(((long long)T * x) >> 3), k1 = (int)(((long long)T * (x + 1)) >> 3);
+    const int lane = threadIdx.x & 63;
+    for (int k = k0 + local * WPB + (int)(threadIdx.x >> 6); k < k1; k += nbx * WPB)
+        f(A.tile_list[k] * 64 + lane);
+}
+
+__device__ __forceinline__ void keys_slot(const ArenaView &A, int s, int force_mask)
+{
     if (s >= A.n) return;
     const int b = building_box(A, s, force_mask);
     if (b < 0) return;
@@ -423,12 +445,16 @@ __global__ __launch_bounds__(NT) void k_keys(ArenaView A, int force_mask)
     A.keys_in[A.box_coff[b] + (s - B.part_off)] = (unsigned)(B.key_base + cy * B.ncx + cx);
 }
 
+__global__ __launch_bounds__(NT) void k_keys(ArenaView A, int force_mask)
+{
+    for_building_tiles(A, [&](int s) { keys_slot(A, s, force_mask); });
+}
+
 // Sorted compact position j (slot s of box b) takes the state of the slot at
 // compact position vals[j] (same box: its key lies in the box's key range)
 // into sorted scratch and records its cell.
-__global__ __launch_bounds__(NT) void k_permute(ArenaView A, int force_mask)
+__device__ __forceinline__ void permute_slot(const ArenaView &A, int s, int force_mask)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
     const int b = building_box(A, s, force_mask);
     if (b < 0) return;
@@ -448,13 +474,17 @@ __global__ __launch_bounds__(NT) void k_permute(ArenaView A, int force_mask)
     A.pcell[s] = (cell < 0 || cell >= B.ncx * B.ncy) ? 0 : cell;
 }
 
+__global__ __launch_bounds__(NT) void k_permute(ArenaView A, int force_mask)
+{
+    for_building_tiles(A, [&](int s) { permute_slot(A, s, force_mask); });
+}
+
 // cell_start over the key range of each building box: cell_start[c] = first
 // slot whose key >= c, and the end marker key_base + ncx*ncy = the slot past
 // the box (the next segment's first slot: a building next box writes the same
 // value there).
-__global__ __launch_bounds__(NT) void k_cell_start(ArenaView A, int force_mask)
+__device__ __forceinline__ void cell_start_slot(const ArenaView &A, int s, int force_mask)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
     const int b = building_box(A, s, force_mask);
     if (b < 0) return;
@@ -467,6 +497,11 @@ __global__ __launch_bounds__(NT) void k_cell_start(ArenaView A, int force_mask)
         const int kend = B.key_base + B.ncx * B.ncy;
         for (int c = key + 1; c <= kend; ++c) A.cell_start[c] = s + 1;
     }
+}
+
+__global__ __launch_bounds__(NT) void k_cell_start(ArenaView A, int force_mask)
+{
+    for_building_tiles(A, [&](int s) { cell_start_slot(A, s, force_mask); });
 }
 
 // Per-box counting sort for arenas of small boxes (every box with at most
@@ -726,27 +761,13 @@ __device__ __forceinline__ void build_slot(const ArenaView &A, int s, int force_
     max_bits(&D.amax2, ai.x * ai.x + ai.y * ai.y);
 }
 
-// Over the building tiles of the plan (tile_list[0 .. plan_n[2])), one wave
-// per tile.  Blocks are dealt to the XCDs in turn (block i on XCD i % 8), so
-// XCD x's blocks walk the x-th eighth of the list: the work of a few
-// building boxes spreads over the whole chip, while neighbouring tiles,
-// which read much the same positions, share an XCD's L2.  (Over all slots,
-// with the slot blocks of each XCD contiguous, a step in which a few small
-// boxes built left most XCDs idle and launched a wave per 64 slots of the
-// arena.)  The grid is at least 8 blocks, so every XCD share has a block.
+// Over the building tiles (for_building_tiles).
 __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask, int fp32)
 {
-    constexpr int WPB = NT / 64;  // waves per block
-    const int x = blockIdx.x & 7, local = blockIdx.x >> 3;
-    const int nbx = ((int)gridDim.x - x + 7) >> 3;  // blocks on this XCD
-    const int T = A.plan_n[2];
-    const int k0 = (int)(((long long)T * x) >> 3), k1 = (int)(((long long)T * (x + 1)) >> 3);
-    const int lane = threadIdx.x & 63;
     // each lane's two four-entry groups being filled (short, long), one
     // column per lane: row r of lane t at lbuf[r * NT + t], bank-conflict free
     __shared__ unsigned lbuf[8 * NT];
-    for (int k = k0 + local * WPB + (int)(threadIdx.x >> 6); k < k1; k += nbx * WPB)
-        build_slot(A, A.tile_list[k] * 64 + lane, force_mask, fp32, lbuf + threadIdx.x);
+    for_building_tiles(A, [&](int s) { build_slot(A, s, force_mask, fp32, lbuf + threadIdx.x); });
 }
 
 // ---------------------------------------------------------------------------
@@ -2279,13 +2300,22 @@ void launch_build_plan(const ArenaView &A, int force_mask, hipStream_t st)
 {
     hipLaunchKernelGGL(k_build_plan, dim3(1), dim3(PLAN_THREADS), 0, st, A, force_mask);
 }
+// grid of the kernels over building tiles: at most one wave per tile of the
+// slots in use, a multiple of 8 (an equal share per XCD), 8 to 2048 blocks
+static inline dim3 tile_grid(const ArenaView &A)
+{
+    long long nb = ((long long)A.n + NT - 1) / NT;
+    nb = (nb + 7) / 8 * 8;
+    nb = nb < 8 ? 8 : (nb > 2048 ? 2048 : nb);
+    return dim3((unsigned)nb);
+}
 void launch_keys(const ArenaView &A, int force_mask, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_keys, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
+    hipLaunchKernelGGL(k_keys, tile_grid(A), dim3(NT), 0, st, A, force_mask);
 }
 void launch_permute(const ArenaView &A, int force_mask, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_permute, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
+    hipLaunchKernelGGL(k_permute, tile_grid(A), dim3(NT), 0, st, A, force_mask);
 }
 void launch_box_sort(const ArenaView &A, int force_mask, int nbox, hipStream_t st)
 {
@@ -2296,16 +2326,11 @@ void launch_box_sort(const ArenaView &A, int force_mask, int nbox, hipStream_t s
 bool box_sort_fits(int ncells, int n) { return ncells <= BS_MAX_CELLS && n <= 16384; }
 void launch_cell_start(const ArenaView &A, int force_mask, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cell_start, grid_for(A.n), dim3(NT), 0, st, A, force_mask);
+    hipLaunchKernelGGL(k_cell_start, tile_grid(A), dim3(NT), 0, st, A, force_mask);
 }
 void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_t st)
 {
-    // one wave per building tile at most: enough blocks for every tile of
-    // the slots in use, a multiple of 8 (an equal share per XCD), 8 to 2048
-    long long nb = ((long long)A.n + NT - 1) / NT;
-    nb = (nb + 7) / 8 * 8;
-    nb = nb < 8 ? 8 : (nb > 2048 ? 2048 : nb);
-    hipLaunchKernelGGL(k_build_lists, dim3((unsigned)nb), dim3(NT), 0, st, A, force_mask, fp32);
+    hipLaunchKernelGGL(k_build_lists, tile_grid(A), dim3(NT), 0, st, A, force_mask, fp32);
 }
 // unroll: entries in flight per lane (1, 2 or 4; anything else = default)
 #define SPH_DISPATCH(K64, K32, ...)                                                              \
