@@ -197,7 +197,12 @@ struct Arena {
     bool stepped = false;  // a box (single-box use) has taken a sph_step
 
     // captured graphs of one arena step without / with a list build
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // step graphs: [0] one step without the build stage, [1] one step with
+    // it, [2 + j] 2^(j+1) steps without it (a box's run between two builds
+    // replays the largest that fit: each graph launch costs ~1-2 us of idle
+    // GPU between graphs)
+    static constexpr int PLAIN_LOG2 = 3;
+    hipGraphExec_t graph[2 + PLAIN_LOG2] = {};
     int use_graph = 1;
     int always_build = 0;  // arenas: build stage every step (per-box schedule on device)
     bool small_boxes = false;  // all live boxes fit the per-box sort (set by relayout)
@@ -346,7 +351,9 @@ struct Arena {
 
     void invalidate_graph()
     {
-        if (!graph[0] && !graph[1]) return;
+        bool any = false;
+        for (auto g : graph) any = any || g;
+        if (!any) return;
         (void)hipStreamSynchronize(ctx->stream);  // no replay of them is in flight
         for (auto &g : graph) {
             if (g) (void)hipGraphExecDestroy(g);
@@ -518,13 +525,18 @@ struct Arena {
     int launch_steps(int64_t count, bool with_build)
     {
         const bool graphs = use_graph && !ctx->timing;
-        hipGraphExec_t &g = graph[with_build ? 1 : 0];
-        for (int64_t i = 0; i < count; ++i) {
+        for (int64_t i = 0; i < count;) {
+            int lg = 0;  // runs of plain steps: the largest chunk graph that fits
+            if (graphs && !with_build)
+                while (lg < PLAIN_LOG2 && (int64_t)2 << lg <= count - i) ++lg;
+            const int per = 1 << lg;
+            hipGraphExec_t &g = graph[lg > 0 ? 1 + lg : (with_build ? 1 : 0)];
+            i += per;
             if (graphs) {
                 if (!g) {
                     hipGraph_t gr;
                     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                    enqueue_step(with_build);
+                    for (int j = 0; j < per; ++j) enqueue_step(with_build);
                     HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
                     hipError_t e1 = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
                     (void)hipGraphDestroy(gr);
