@@ -194,13 +194,16 @@ def test_step_parity(cfg, nsteps):
     assert e[2] == pytest.approx(E[-1], rel=1e-11)
 
 
-def test_step_graph_equals_direct_launches():
+@pytest.mark.parametrize("nsteps", [8, 150])
+def test_step_graph_equals_direct_launches(nsteps):
+    # 150 steps: runs of plain steps between builds long enough to replay the
+    # 8-, 4- and 2-step graphs
     s = configs.square_system(24, 0.9, 5, 8)
     outs = []
     for graph in (True, False):
         with from_system(s) as b:
             b.set_graph(graph)
-            b.step(8)
+            b.step(nsteps)
             outs.append(b.download("r", "v", "u"))
     for k in ("r", "v", "u"):
         assert np.array_equal(outs[0][k], outs[1][k]), k
