@@ -154,6 +154,29 @@ def test_rates_parity(terms):
     assert np.all(np.abs(mom) < 1e-12 * np.sum(np.abs(m[:, None] * got["acc"])))
 
 
+def test_full_size_c2_pairs_and_rates():
+    # the 512 x 512 box of 262,144 particles at its real size: the pair set
+    # (6.8M pairs) bit-exact, FP64 density and rates within the parity
+    # tolerance, and the device accelerations conserving momentum
+    s = configs.c2()
+    prm = s.prm
+    ref_pairs = O.pairs(s.r, prm["Lx"], prm["Ly"], prm["rc"] ** 2 + prm["tol"] ** 2)
+    u0 = O.init_u(s.r, s.m, s.T, prm)
+    ref = O.derivatives(s.r, s.v, s.m, u0, prm, terms=prm["terms"])
+    with from_system(s) as b:
+        got_pairs = b.pairs()
+        assert got_pairs.shape == ref_pairs.shape
+        assert np.array_equal(got_pairs, ref_pairs)
+        b.rates(prm["terms"])
+        got = b.download("rho", "rho_lr", "acc", "udot")
+    for k in ("rho", "rho_lr"):
+        np.testing.assert_allclose(got[k], ref[k], rtol=1e-12, err_msg=k)
+    assert rel_norm(got["acc"], ref["acc"]) < TOL64
+    assert rel_norm(got["udot"], ref["udot"]) < TOL64
+    mom = (s.m[:, None] * got["acc"]).sum(axis=0)
+    assert np.all(np.abs(mom) < 1e-12 * np.sum(np.abs(s.m[:, None] * got["acc"])))
+
+
 def test_gradients_api():
     r, v, m, prm = _system(30, seed=4)
     u = np.full(r.shape[0], -1.1)
