@@ -177,6 +177,22 @@ def test_full_size_c2_pairs_and_rates():
     assert np.all(np.abs(mom) < 1e-12 * np.sum(np.abs(s.m[:, None] * got["acc"])))
 
 
+def test_full_size_c3_fp32c_within_1e6_of_fp64_oracle():
+    # the 1024 x 1024 quench box of 1,048,576 particles in the FP32
+    # compensated form (cell-relative mirror, the box being over 512 wide):
+    # one rate evaluation within 1e-6 of the FP64 oracle, normwise
+    s = configs.c3()
+    prm = s.prm
+    u0 = O.init_u(s.r, s.m, s.T, prm)
+    ref = O.derivatives(s.r, s.v, s.m, u0, prm, terms=prm["terms"])
+    with from_system(s) as b:
+        b.upload(u=u0)
+        b.rates(prm["terms"])
+        got = b.download("rho", "rho_lr", "acc", "udot")
+    for k in ("rho", "rho_lr", "acc", "udot"):
+        assert rel_norm(got[k], ref[k]) < 1e-6, k
+
+
 def test_gradients_api():
     r, v, m, prm = _system(30, seed=4)
     u = np.full(r.shape[0], -1.1)
