@@ -53,6 +53,28 @@ def test_batched_equals_alone(precision):
             assert got["E1"] == pytest.approx(rf["E1"], rel=1e-12)
 
 
+def test_full_c4_arena_equals_alone():
+    # All 512 boxes of the batched workload (1.97M particles, N from 256 to
+    # 16,384, T from 0.3 to 1.0) in one FP32C arena, 40 steps each; six of
+    # them, the smallest and the largest included, must be bitwise what they
+    # give alone.
+    boxes = []
+    for b in range(512):
+        s = configs.c4_box(b)
+        s.steps = 40
+        boxes.append(s)
+    ns = np.array([s.n for s in boxes])
+    pick = sorted({int(np.argmin(ns)), int(np.argmax(ns)), 0, 137, 300, 511})
+    ref = {i: alone(boxes[i], _lib.FP32C) for i in pick}
+    with Arena(int(ns.sum()), precision=_lib.FP32C) as ar:
+        ids = [ar.add(s) for s in boxes]
+        run_to_completion(ar, ids)
+        for i in pick:
+            got = ar.result(ids[i])
+            for k in ("r", "v", "u", "rho"):
+                assert np.array_equal(got[k], ref[i][k]), (i, k)
+
+
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32C])
 def test_boxes_smaller_than_a_tile(precision):
     # Boxes of 16-60 slots with a freed gap among them: one 64-slot tile
