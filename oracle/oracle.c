@@ -25,14 +25,21 @@
  *                                unit integral, W(h)=W'(h)=0, finite differences
  *   orc_density                  pinned: lattice closed sums, brute long-double
  *   orc_eos                      pinned: vdw_temp(vdw_energy(T)) == T, p=2,pco=-2
- *   orc_gradients                parity unpinned vs reference (the Fortran
- *                                splib.calc_grad_v / calc_heat_flux_3d is absent);
- *                                checked by physics: linear fields, uniform T
+ *   orc_gradients                pinned by lattice closed forms (the reference's
+ *                                Fortran splib.calc_grad_v / calc_heat_flux_3d is
+ *                                absent): linear and quadratic T and v profiles
+ *                                give grad T, grad v and q = -k grad T exactly up
+ *                                to one lattice factor c computed independently
  *   orc_rates                    pressure/cohesion pinned by two-particle
  *                                analytic case, momentum and energy balance;
- *                                viscosity and conduction parity unpinned vs
- *                                reference (absent Fortran sphforce3d), checked
- *                                by shear-flow and hot-spot physics
+ *                                viscosity (eta and zeta), conduction magnitude
+ *                                and sign, viscous force assembly and heating
+ *                                pinned by lattice closed forms: quadratic T
+ *                                (udot = 2 k beta c^2/rho), uniform dilation
+ *                                (Pi = -2 zeta eps c I), quadratic shear
+ *                                (a_x = 2 eta gamma c^2/rho and its heating);
+ *                                planted mistakes (factor 2, sign, transposed
+ *                                operand) fail them
  *   orc_run                      leapfrog defined here (the reference has none);
  *                                checked by energy drift and step-size order
  *   orc_splash_map_2d            pinned: lattice rendering at particle sites is
@@ -417,8 +424,9 @@ void orc_eos(int64_t n, const double *rho, const double *rho_lr, const double *u
 /* b5 / c3 gradients.  grad_i W_ij = -dwdx_ij with dwdx built from r_j - r_i */
 /*   gradv_i[a][b] = sum_j (m_j/rho_j) (v_j - v_i)_a (grad_i W_ij)_b          */
 /*   gradT_i[b]    = sum_j (m_j/rho_j) (T_j - T_i)   (grad_i W_ij)_b          */
-/* Parity unpinned: the reference's splib.calc_grad_v / calc_heat_flux_3d    */
-/* (f_properties.py:111,130) are absent.  Layout gradv[n][2][2], gradT[n][2]. */
+/* The reference's splib.calc_grad_v / calc_heat_flux_3d (f_properties.py: */
+/* 111,130) are absent; lattice closed forms pin these sums (linear and     */
+/* quadratic profiles).  Layout gradv[n][2][2], gradT[n][2].                */
 typedef struct {
     const double *m, *rho, *v, *T, *rij, *drij;
     const int64_t *iap;

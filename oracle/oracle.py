@@ -2,6 +2,7 @@
 (oracle/oracle.c).  Only tests/, __graft_entry__.smoke() and bench.py's CPU
 baseline leg may import it; the product path never does.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -30,10 +31,24 @@ def build():
 
 
 _libs = {}
+_override = []
+
+
+@contextlib.contextmanager
+def library(path):
+    """Route the serial checker's calls to another build of oracle.c at
+    path while the block runs (the mutation tests build copies of the
+    source with one planted mistake each and check that the closed-form
+    tests reject them)."""
+    _override.append(path)
+    try:
+        yield
+    finally:
+        _override.pop()
 
 
 def lib(threaded=False):
-    path = LIB_OMP_PATH if threaded else LIB_PATH
+    path = LIB_OMP_PATH if threaded else (_override[-1] if _override else LIB_PATH)
     if path not in _libs:
         if not os.path.exists(path):
             build()
