@@ -34,18 +34,18 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-# Algorithmic HBM bytes per particle of each kernel (fixed accounting: every
-# per-particle array a logical pass must read or write counted once at its
-# storage width; neighbour re-reads and list words not counted; sort/permute
-# counted).  Per logical pass: build 124 (FP32C 116), density r, m -> rho,
-# rho_lr 40 (28), gradients 104 (72), rates 112 (76), leapfrog writes 40.
-# The density kernel also computes the cohesive part of the rates, so it
-# takes over the rates' velocity read (16 B, FP32C 8 B).
+# Algorithmic HBM bytes per particle of each kernel: the fixed per-pass
+# figures (every per-particle array a logical pass must read or write counted
+# once at its storage width; neighbour re-reads and list words not counted;
+# sort/permute counted): build 124 (FP32C 116), density r, m -> rho, rho_lr
+# 40 (28), gradients 104 (72), rates 112 (76; inviscid 64 / 52), leapfrog
+# writes 40.  The density kernel also computes the cohesive part of the rates
+# and the rate kernel adds it; the figures stay those of the logical passes.
 B_PS = {
-    ("fp64", True): dict(build=124, density=56, gradients=104, rates=96, integrate=40),
-    ("fp64", False): dict(build=124, density=56, gradients=0, rates=48, integrate=40),
-    ("fp32c", True): dict(build=116, density=36, gradients=72, rates=68, integrate=40),
-    ("fp32c", False): dict(build=116, density=36, gradients=0, rates=44, integrate=40),
+    ("fp64", True): dict(build=124, density=40, gradients=104, rates=112, integrate=40),
+    ("fp64", False): dict(build=124, density=40, gradients=0, rates=64, integrate=40),
+    ("fp32c", True): dict(build=116, density=28, gradients=72, rates=76, integrate=40),
+    ("fp32c", False): dict(build=116, density=28, gradients=0, rates=52, integrate=40),
 }
 KERNEL = {"fp64": dict(density="k_density64", gradients="k_grad64", rates="k_rates64"),
           "fp32c": dict(density="k_density32", gradients="k_grad32", rates="k_rates32")}
@@ -126,6 +126,7 @@ def run_gpu(s, steps, warmup, timing=True, unroll=0, kernel_steps=200, launch_re
     t0 = time.perf_counter()
     b.step(steps)  # returns after the stream is drained (health check)
     t1 = time.perf_counter()
+    run_stats = b.run_stats()  # list builds and postponed builds inside the timed steps
     E1 = b.energy()[2]
     tm = None
     if timing:
@@ -140,6 +141,8 @@ def run_gpu(s, steps, warmup, timing=True, unroll=0, kernel_steps=200, launch_re
         # state the steps left (no per-launch event overhead)
         tm["launch_us"] = {p: b.time_pass(p, reps=launch_reps) for p in b.PASSES}
         tm["launch_reps"] = launch_reps
+    if tm is not None:
+        tm["timed_run"] = run_stats
     return b, t1 - t0, tm, (E0, E1)
 
 
@@ -276,6 +279,18 @@ def leg_c3(steps):
     return out
 
 
+def sample_errors(systems, states, unroll=0):
+    """Per-field FP32C error against the FP64 oracle at the given states
+    (r, v, u of each system): the maximum over the sample of each field's
+    normwise error."""
+    worst = {}
+    for s, st in zip(systems, states):
+        e = error_vs_oracle(s, r=st["r"], v=st["v"], u=st["u"], unroll=unroll)
+        for k, x in e.items():
+            worst[k] = max(worst.get(k, 0.0), x)
+    return worst
+
+
 def run_arena(boxes, precision, chunk=25):
     """All boxes in one arena until every box is done; (seconds, results)."""
     from pyticles_amd import autotune
@@ -316,6 +331,13 @@ def leg_c4(nbox=512, plain_every=4):
         out[name] = {"value": ps / el, "unit": "particle-steps/s", "seconds": el, "failed_boxes": failed,
                      "energy_drift_rel_median": float(np.median(drift)), "energy_drift_rel_max": float(drift.max()),
                      "schedule": sc}
+        if prec == _lib.FP32C:
+            pick = list(range(0, nbox, max(1, nbox // 8)))
+            log("c4 fp32c errors vs oracle on %d boxes" % len(pick))
+            out[name]["rel_err_vs_fp64_oracle_final_states"] = sample_errors(
+                [bx[i] for i in pick], [res[i] for i in pick], unroll=sc["unroll"])
+            out[name]["error_sample"] = "boxes %s at their final (step %d) states, one FP32C rate evaluation each " \
+                                        "against the FP64 oracle; maximum over the sample per field" % (pick, steps)
     sample = list(range(0, nbox, plain_every))
     el = 0.0
     pss = 0
@@ -362,6 +384,21 @@ def leg_c5(njobs, warm, rate, max_resident, budget):
         sm["saturated"] = bool(sm["particle_steps_per_s"] < 0.9 * offered)
         return sm
 
+    # per-field FP32C error on jobs of the stream's distribution: four jobs
+    # of different sizes stepped by the FP64 device path (their own step
+    # counts, at most 400), then one FP32C rate evaluation against the oracle
+    from pyticles_amd.device import from_system
+    sample, states = [], []
+    for j in (3, 11, 29, 47):
+        s64 = configs.c5_job(j, precision=_lib.FP64)
+        with from_system(s64) as b:
+            b.step(min(s64.steps, 400))
+            states.append(b.download("r", "v", "u"))
+        sample.append(configs.c5_job(j))
+    log("c5 fp32c errors vs oracle on jobs %s" % [s.n for s in sample])
+    out["rel_err_vs_fp64_oracle"] = sample_errors(sample, states, unroll=sc["unroll"])
+    out["error_sample"] = "jobs 3, 11, 29, 47 (N = %s) after min(their steps, 400) FP64 device steps; one FP32C " \
+                          "rate evaluation each against the FP64 oracle; maximum per field" % [s.n for s in sample]
     out["stream_200"] = serve(njobs, warm, rate, "stream")
     cap = out["stream_200"]["particle_steps_per_s"]
     mean_work = np.mean([j.n * j.steps for j in (configs.c5_job(k) for k in range(200))])
@@ -505,8 +542,11 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+kahan/f64",
         "data": "synthetic (NumPy PCG64 seeded jittered lattice, Gaussian velocities)",
         "config": {"workload": desc, "N": s.n, "box": [s.prm["Lx"], s.prm["Ly"]], "precision": args.precision,
-                   "timed": "%d leapfrog steps (%d rate evaluations incl. the closing half kick)" % (args.steps, evals),
+                   "timed": "%d leapfrog steps (%d rate evaluations incl. the closing half kick) in one sph_step "
+                            "call; step graphs captured at box set-up, outside the timed region" % (args.steps, evals),
                    "schedule": sched},
+        "timed_list_builds": int(tm["timed_run"]["builds"]),
+        "timed_builds_postponed": int(tm["timed_run"]["extends"]),
         "roofline": roof,
         "pair_interactions_per_s": nip * args.steps / el,
         "nip": nip,

@@ -63,6 +63,16 @@ typedef struct {
                           step); r_cut = min(sqrt(rc^2 + tol^2), max(h, hl))  */
     unsigned terms;    /* SPH_* rate terms used by sph_step                    */
     int precision;     /* SPH_FP64 or SPH_FP32C                                */
+    double fp32_box_relative_max;
+                       /* SPH_FP32C neighbour records of a box whose sides are
+                          both <= this hold box-relative 32-bit fixed-point
+                          positions with the velocity in the same 16 bytes
+                          (one gather per neighbour in the density pass;
+                          position resolution side / 2^32, which leaves
+                          ~1e-6 of force error on relaxed states of boxes
+                          over ~200 per side); otherwise (0, the default)
+                          build-cell-relative positions, whose separations
+                          are exact.  No counterpart in the reference.    */
 } sph_params;
 
 /* ---- context ------------------------------------------------------------ */
@@ -77,14 +87,6 @@ const char *sph_version(void);
 /* Per-pass GPU timing with HIP events (off by default; on disables graph
  * replay so every pass can be bracketed). */
 int sph_ctx_set_timing(sph_ctx *ctx, int on);
-/* FP32C position mirror: boxes whose sides are all <= L (default 512) keep
- * box-relative 32-bit fixed-point positions with the velocity in the same
- * 16-byte neighbour record (resolution L / 2^32); larger boxes keep
- * build-cell-relative positions (2^29 units per cell).  Applies to boxes
- * created or re-parameterised afterwards, process-wide; L = 0 selects the
- * cell-relative form for every box.  No counterpart in the reference. */
-int sph_set_fp32_box_relative_max(double L);
-int sph_get_fp32_box_relative_max(double *L);
 
 /* ---- one box ------------------------------------------------------------ */
 /* Device-resident state of n particles.  Replaces the arrays of
@@ -98,7 +100,8 @@ int sph_box_upload(sph_box *box, const double *r, const double *v,
                    const double *m, const double *u);
 /* Download state and derived fields in the caller's particle order (any
  * pointer may be NULL).  rho..P hold the values of the last density /
- * gradient / rate evaluation.  Replaces the write-backs of
+ * gradient / rate evaluation.  A download of r alone leaves an open run
+ * (sph_box_advance) open; any other field closes it first.  Replaces the write-backs of
  * spam_complete_force.py:171-183 and f_properties.py:138-147. */
 int sph_box_download(sph_box *box, double *r, double *v, double *u,
                      double *rho, double *rho_lr, double *T, double *p,
@@ -120,6 +123,12 @@ int sph_separations(sph_box *box, int64_t nip, const int64_t *iap,
 /* Minimum image of np displacements in place (dr[np][2]).  Replaces
  * NeighbourList.minimum_image (neighbour_list.py:105-123). */
 int sph_minimum_image(sph_ctx *ctx, int64_t np, double *dr, double Lx, double Ly);
+/* The FP32C passes' compensated accumulator on n terms x[n] (n <= 2^24):
+ * the terms rounded to FP32, summed in FP32 rounds of `unroll` terms, each
+ * round added to a Kahan total (*kahan); *plain gets the plain FP32 running
+ * sum of the same terms.  No counterpart in the reference: it lets a test
+ * compare the compensated scheme with plain FP32 accumulation. */
+int sph_fp32_accumulate(sph_ctx *ctx, int64_t n, const double *x, int unroll, double *kahan, double *plain);
 
 /* ---- (b) kernel, density, EOS ------------------------------------------- */
 /* Lucy kernel W and dW/dx for np pairs in 1-3 dimensions (dx[np][dim]).
@@ -149,6 +158,26 @@ int sph_init_thermal(sph_box *box, const double *T, double T0);
  * with periodic wrap.  Replaces SmoothParticleSystem.update
  * (particles.py:459-494) for the periodic case. */
 int sph_step(sph_box *box, int64_t nsteps);
+/* Advance nsteps leapfrog steps and leave the run open: r at the new integer
+ * time, v and u at the half step, one rate evaluation per step.  Successive
+ * calls continue one run; k steps taken in any number of calls give the
+ * bits of sph_step(box, k) once the run is closed.  A run is closed by
+ * sph_box_sync and by any call that reads or replaces the state (downloads,
+ * uploads, single passes, energies, pair export, snapshots); sph_step
+ * continues an open run by its nsteps and closes it.  Without an open run
+ * the first step starts one (k = 0, half kick).  Replaces the per-tick
+ * SmoothParticleSystem.update (particles.py:459-494) as the reference's
+ * driver calls it, once per step (toybox.py:101-103). */
+int sph_box_advance(sph_box *box, int64_t nsteps);
+/* Close an open run: the rate evaluation at the current step and the closing
+ * half kick, so v and u are synchronised with r (no-op without an open run).
+ * Reports a numerical failure of the run as SPH_ERR_NUMERIC. */
+int sph_box_sync(sph_box *box);
+/* Counters of the box's current or last run: steps taken, neighbour-list
+ * builds, scheduled builds postponed by the list review, and open = 1 while
+ * a run is open (any pointer may be NULL).  No counterpart in the
+ * reference: the bench reports the builds its timed steps held. */
+int sph_box_run_stats(sph_box *box, int64_t *steps, int64_t *builds, int64_t *extends, int *open);
 /* Totals at the current (synchronised) state: sum m v^2/2, sum m u, sum. */
 int sph_energy(sph_box *box, double *ke, double *ue, double *etot);
 /* Accumulated GPU time per pass in ms: [0] cell build (bin, sort, lists),

@@ -44,7 +44,8 @@ class Params(ctypes.Structure):
     _fields_ = [(name, ctypes.c_double) for name in
                 ("Lx", "Ly", "h", "hl", "rc", "tol", "a", "b", "kB", "eta", "zeta", "kcond", "dt",
                  "skin")] + \
-               [("terms", ctypes.c_uint), ("precision", ctypes.c_int)]
+               [("terms", ctypes.c_uint), ("precision", ctypes.c_int),
+                ("fp32_box_relative_max", ctypes.c_double)]
 
 
 _P = ctypes.c_void_p
@@ -58,8 +59,6 @@ _SIGS = {
     "sph_last_error": [],
     "sph_version": [],
     "sph_ctx_set_timing": [_P, ctypes.c_int],
-    "sph_set_fp32_box_relative_max": [ctypes.c_double],
-    "sph_get_fp32_box_relative_max": [_D],
     "sph_box_create": [_P, _I64, ctypes.POINTER(Params), ctypes.POINTER(_P)],
     "sph_box_destroy": [_P],
     "sph_box_set_params": [_P, ctypes.POINTER(Params)],
@@ -68,12 +67,17 @@ _SIGS = {
     "sph_nlist_build": [_P, _I64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
     "sph_separations": [_P, _I64, ctypes.POINTER(ctypes.c_int64), _D, _D, _D, _D],
     "sph_minimum_image": [_P, _I64, _D, ctypes.c_double, ctypes.c_double],
+    "sph_fp32_accumulate": [_P, _I64, _D, ctypes.c_int, _D, _D],
     "sph_kernel_lucy": [_P, _I64, _D, _D, ctypes.c_double, ctypes.c_int, _D, _D],
     "sph_density": [_P],
     "sph_gradients": [_P],
     "sph_rates": [_P, ctypes.c_uint],
     "sph_init_thermal": [_P, _D, ctypes.c_double],
     "sph_step": [_P, _I64],
+    "sph_box_advance": [_P, _I64],
+    "sph_box_sync": [_P],
+    "sph_box_run_stats": [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)],
     "sph_energy": [_P, _D, _D, _D],
     "sph_box_timing": [_P, _D, ctypes.c_int, ctypes.c_int],
     "sph_box_time_pass": [_P, ctypes.c_int, ctypes.c_int, _D],

@@ -56,9 +56,6 @@ void dfree(T *&p)
     p = nullptr;
 }
 
-// Largest box side with the box-relative FP32C mirror (sph_set_fp32_box_relative_max).
-double g_fixbox_max_l = FIXBOX_MAX_L;
-
 int bits_for(int nkeys)
 {
     int b = 1;
@@ -115,6 +112,10 @@ static size_t arena_bytes(size_t cap, int prec, int lcap)
     b += radix_sort_temp_bytes((int)cap);
     return b;
 }
+
+// Step count of an open run (sph_box_advance): no step reaches it, so the
+// run keeps kicking by dt until sph_box_sync sets its closing step.
+static constexpr int64_t OPEN_STEPS = (int64_t)1 << 60;
 
 struct Arena {
     sph_ctx *ctx = nullptr;
@@ -195,6 +196,10 @@ struct Arena {
     double t_acc[T_N] = {0, 0, 0, 0, 0, 0};
     double steps_done = 0;
     bool stepped = false;  // a box (single-box use) has taken a sph_step
+    // open run of a single box (sph_box_advance / sph_box_sync): the box
+    // runs with an unbounded step count, so no step is its closing one
+    bool open_run = false;
+    int64_t open_plain = 0;  // next steps known not to build; -1: read the schedule first
 
     // captured graphs of one arena step without / with a list build
     // step graphs: [0] one step without the build stage, [1] one step with
@@ -203,6 +208,16 @@ struct Arena {
     // GPU between graphs)
     static constexpr int PLAIN_LOG2 = 3;
     hipGraphExec_t graph[2 + PLAIN_LOG2] = {};
+    // What the captured launches bake in: the view (pointers, slot, key and
+    // segment counts), the box count, the sort path and the pass depths.  A
+    // relayout that leaves it unchanged (a box's terms or constants changed:
+    // the kernels read those from the box table) keeps the graphs.
+    struct GraphSig {
+        ArenaView view;
+        int nbox, small, unroll, unroll_rates;
+    };
+    GraphSig graph_sig;
+    bool graph_sig_set = false;
     int use_graph = 1;
     int always_build = 0;  // arenas: build stage every step (per-box schedule on device)
     bool small_boxes = false;  // all live boxes fit the per-box sort (set by relayout)
@@ -351,6 +366,7 @@ struct Arena {
 
     void invalidate_graph()
     {
+        graph_sig_set = false;
         bool any = false;
         for (auto g : graph) any = any || g;
         if (!any) return;
@@ -359,6 +375,62 @@ struct Arena {
             if (g) (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
+    }
+
+    GraphSig signature() const
+    {
+        GraphSig sg;
+        memset(&sg, 0, sizeof(sg));
+        sg.view = view();
+        sg.nbox = (int)boxes.size();
+        sg.small = small_boxes ? 1 : 0;
+        sg.unroll = unroll;
+        sg.unroll_rates = unroll_rates;
+        return sg;
+    }
+
+    // Drop the captured graphs if what they bake in has changed since they
+    // were captured.
+    void check_graph_sig()
+    {
+        const GraphSig now = signature();
+        if (graph_sig_set && memcmp(&now, &graph_sig, sizeof(now)) == 0) return;
+        invalidate_graph();
+        graph_sig = now;
+        graph_sig_set = true;
+    }
+
+    // Capture and instantiate step graph idx: [0] one step without the build
+    // stage, [1] one step with it, [1 + lg] 2^lg steps without it.
+    int capture_graph(int idx)
+    {
+        hipGraphExec_t &g = graph[idx];
+        if (g) return SPH_OK;
+        check_graph_sig();
+        const bool with_build = idx == 1;
+        const int per = idx >= 2 ? 1 << (idx - 1) : 1;
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        for (int j = 0; j < per; ++j) enqueue_step(with_build);
+        HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
+        hipError_t e1 = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e1 != hipSuccess) {
+            g = nullptr;
+            return fail(SPH_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e1));
+        }
+        return SPH_OK;
+    }
+
+    // Set-up time capture of every step graph a single box replays, so no
+    // capture or instantiation falls inside a stepping call.
+    int prepare_graphs()
+    {
+        if (!use_graph || ctx->timing) return SPH_OK;  // timed runs launch directly
+        HIPCHK(hipSetDevice(ctx->device));
+        for (int idx = 0; idx < 2 + PLAIN_LOG2; ++idx)
+            if (int rc = capture_graph(idx)) return rc;
+        return SPH_OK;
     }
 
     // Recompute key bases in slot order, the high-water mark, and upload the
@@ -409,7 +481,7 @@ struct Arena {
         // returns, so the tables may change right away; no wait
         // the slot count, key count and segment count are baked into
         // captured launches
-        invalidate_graph();
+        check_graph_sig();
         return SPH_OK;
     }
 
@@ -530,22 +602,13 @@ struct Arena {
             if (graphs && !with_build)
                 while (lg < PLAIN_LOG2 && (int64_t)2 << lg <= count - i) ++lg;
             const int per = 1 << lg;
-            hipGraphExec_t &g = graph[lg > 0 ? 1 + lg : (with_build ? 1 : 0)];
+            const int idx = lg > 0 ? 1 + lg : (with_build ? 1 : 0);
             i += per;
             if (graphs) {
-                if (!g) {
-                    hipGraph_t gr;
-                    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                    for (int j = 0; j < per; ++j) enqueue_step(with_build);
-                    HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
-                    hipError_t e1 = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(gr);
-                    if (e1 != hipSuccess) {
-                        g = nullptr;
-                        return fail(SPH_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e1));
-                    }
-                }
-                HIPCHK(hipGraphLaunch(g, ctx->stream));
+                // single boxes capture at set-up (prepare_graphs); arenas
+                // capture here, at the first step after a relayout
+                if (int rc = capture_graph(idx)) return rc;
+                HIPCHK(hipGraphLaunch(graph[idx], ctx->stream));
             } else {
                 enqueue_step(with_build);
             }
@@ -586,6 +649,67 @@ struct Arena {
         return collect_timing();
     }
 
+    // Advance box b's open run by nsteps steps (one rate evaluation each),
+    // starting the run (k = 0: half kick) if none is open.  The launches are
+    // those sph_step makes for the same steps: plain steps until the box's
+    // scheduled build, then a step with the build stage and one read-back of
+    // the schedule, so k steps in any number of calls give the bits of one
+    // run of k steps.
+    int advance(int b, int64_t nsteps)
+    {
+        if (!open_run) {
+            BoxDyn D;
+            memset(&D, 0, sizeof(D));
+            D.state = BOX_RUN;
+            D.k = 0;
+            D.nsteps = OPEN_STEPS;
+            D.next_rebuild = 0;
+            if (int rc = write_dyn(b, D)) return rc;
+            open_run = true;
+            open_plain = 0;
+        }
+        int64_t done = 0;
+        while (done < nsteps) {
+            if (open_plain < 0) {
+                if (int rc = read_dyn()) return rc;
+                const BoxDyn &D = h_dyn[b];
+                if (D.health) {
+                    open_run = false;
+                    return SPH_OK;  // the caller's health check reports it
+                }
+                open_plain = D.next_rebuild > D.k ? D.next_rebuild - D.k : 0;
+            }
+            if (open_plain == 0) {
+                if (int rc = launch_steps(1, true)) return rc;
+                open_plain = -1;
+                ++done;
+            } else {
+                const int64_t p = std::min(open_plain, nsteps - done);
+                if (int rc = launch_steps(p, false)) return rc;
+                open_plain -= p;
+                done += p;
+            }
+        }
+        steps_done += (double)nsteps;
+        stepped = true;
+        return collect_timing();
+    }
+
+    // Close box b's open run: the rate evaluation at the current step with
+    // the closing half kick, so v and u are synchronised with r again.
+    int close_run(int b)
+    {
+        if (!open_run) return SPH_OK;
+        open_run = false;
+        if (int rc = read_dyn()) return rc;  // also waits for the run's steps
+        BoxDyn D = h_dyn[b];
+        if (D.health || D.state != BOX_RUN) return SPH_OK;
+        D.nsteps = D.k;
+        if (int rc = write_dyn(b, D)) return rc;
+        if (int rc = launch_steps(1, true)) return rc;
+        return collect_timing();
+    }
+
     int read_dyn()
     {
         if (boxes.empty()) return SPH_OK;
@@ -618,6 +742,8 @@ int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
     if (p->precision != SPH_FP64 && p->precision != SPH_FP32C)
         return fail(SPH_ERR_ARG, "invalid precision");
     if (!(p->skin >= 0) || !std::isfinite(p->skin)) return fail(SPH_ERR_ARG, "invalid skin (need >= 0)");
+    if (!(p->fp32_box_relative_max >= 0) || !std::isfinite(p->fp32_box_relative_max))
+        return fail(SPH_ERR_ARG, "invalid fp32_box_relative_max (need a finite side >= 0)");
     memset(&B, 0, sizeof(B));
     const double PI = 3.14159265358979323846;
     B.Lx = p->Lx; B.Ly = p->Ly; B.hLx = p->Lx / 2.; B.hLy = p->Ly / 2.;
@@ -634,7 +760,8 @@ int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
     B.rin = 2.0 * (rcut + p->skin);
     // FP32C boxes up to the box-relative limit per side mirror positions as
     // box-relative fixed point (2^32 units per side)
-    B.fixbox = (p->precision == SPH_FP32C && p->Lx <= g_fixbox_max_l && p->Ly <= g_fixbox_max_l) ? 1 : 0;
+    B.fixbox = (p->precision == SPH_FP32C && p->Lx <= p->fp32_box_relative_max &&
+                p->Ly <= p->fp32_box_relative_max) ? 1 : 0;
     // cells cover both the pair-export radius and the list radius
     const double Rc = std::max(std::sqrt(B.R2list), rcut + p->skin) * (1.0 + 1e-9);
     int ncx = (int)std::floor(p->Lx / Rc), ncy = (int)std::floor(p->Ly / Rc);
@@ -678,6 +805,17 @@ int Arena::add_box(int64_t n, const sph_params *prm, int *box_out)
     BoxDesc B;
     if (int rc = fill_desc(prm, n, B)) return rc;
     if (prm->precision != precision) return fail(SPH_ERR_ARG, "box precision differs from the arena's");
+    {
+        // mean neighbours within r_cut + skin at the box's number density:
+        // a box whose mean alone takes over half the list entries is refused
+        // here (a liquid region at twice the mean density would overflow
+        // them) instead of failing its first build with HEALTH_LISTCAP
+        const double mean = 3.14159265358979323846 * B.rnl2 * (double)n / (prm->Lx * prm->Ly);
+        if (mean > 0.5 * lcap)
+            return fail(SPH_ERR_ARG, "box too dense for the neighbour lists: " + std::to_string(mean) +
+                                         " mean neighbours within r_cut + skin against " + std::to_string(lcap) +
+                                         " list entries per particle");
+    }
     // first fit over free segments
     int si = -1;
     for (size_t i = 0; i < segs.size(); ++i)
@@ -791,6 +929,19 @@ struct sph_arena {
     Arena ar;
 };
 
+namespace {
+// Close a single box's open run (sph_box_advance) before a call that reads
+// or replaces its state: v and u are then synchronised with r again.
+int settle(sph_box *box)
+{
+    Arena &A = box->ar;
+    if (!A.open_run) return SPH_OK;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    if (int rc = A.close_run(box->b)) return rc;
+    return check_health(A, box->b);
+}
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -832,20 +983,6 @@ int sph_ctx_destroy(sph_ctx *ctx)
     return SPH_OK;
 }
 
-int sph_set_fp32_box_relative_max(double L)
-{
-    if (!(L >= 0.0) || !std::isfinite(L)) return fail(SPH_ERR_ARG, "invalid box side (need a finite L >= 0)");
-    g_fixbox_max_l = L;
-    return SPH_OK;
-}
-
-int sph_get_fp32_box_relative_max(double *L)
-{
-    if (!L) return fail(SPH_ERR_ARG, "null argument");
-    *L = g_fixbox_max_l;
-    return SPH_OK;
-}
-
 int sph_ctx_set_timing(sph_ctx *ctx, int on)
 {
     if (!ctx) return fail(SPH_ERR_ARG, "null context");
@@ -866,6 +1003,7 @@ int sph_box_create(sph_ctx *ctx, int64_t n, const sph_params *prm, sph_box **out
     bx->prm = *prm;
     int rc = bx->ar.alloc((int)n, prm->precision, list_entries_for(prm, n));
     if (!rc) rc = bx->ar.add_box(n, prm, &bx->b);
+    if (!rc) rc = bx->ar.prepare_graphs();  // no capture inside a later sph_step
     if (rc) {
         std::string keep = g_err;
         bx->ar.release();
@@ -890,6 +1028,7 @@ int sph_box_destroy(sph_box *box)
 int sph_box_set_params(sph_box *box, const sph_params *prm)
 {
     if (!box || !prm) return fail(SPH_ERR_ARG, "null argument");
+    if (int rc = settle(box)) return rc;
     if (prm->precision != box->ar.precision) return fail(SPH_ERR_ARG, "precision cannot change");
     BoxDesc B;
     if (int rc = fill_desc(prm, box->ar.boxes[box->b].n, B)) return rc;
@@ -897,7 +1036,8 @@ int sph_box_set_params(sph_box *box, const sph_params *prm)
     B.T0 = box->ar.boxes[box->b].T0;
     box->ar.boxes[box->b] = B;
     box->prm = *prm;
-    return box->ar.relayout();
+    if (int rc = box->ar.relayout()) return rc;
+    return box->ar.prepare_graphs();
 }
 
 static int check_positions(const sph_box *box, const double *r)
@@ -916,6 +1056,7 @@ static int check_positions(const sph_box *box, const double *r)
 int sph_box_upload(sph_box *box, const double *r, const double *v, const double *m, const double *u)
 {
     if (!box) return fail(SPH_ERR_ARG, "null box");
+    if (int rc = settle(box)) return rc;
     HIPCHK(hipSetDevice(box->ar.ctx->device));
     if (r) {
         if (int rc = check_positions(box, r)) return rc;
@@ -932,6 +1073,11 @@ int sph_box_download(sph_box *box, double *r, double *v, double *u, double *rho,
                      double *jq, double *P)
 {
     if (!box) return fail(SPH_ERR_ARG, "null box");
+    // positions are at the current integer time in an open run too; every
+    // other field needs the run closed first
+    const bool only_r = !v && !u && !rho && !rho_lr && !T && !p && !pco && !acc && !udot && !gradv && !jq && !P;
+    if (!only_r)
+        if (int rc = settle(box)) return rc;
     HIPCHK(hipSetDevice(box->ar.ctx->device));
     struct F { double *dst; int field, nc; } fs[] = {
         {r, 0, 2}, {v, 1, 2}, {u, 3, 1}, {rho, 4, 1}, {rho_lr, 5, 1}, {T, 6, 1}, {p, 7, 1},
@@ -947,6 +1093,7 @@ int sph_box_download(sph_box *box, double *r, double *v, double *u, double *rho,
 int sph_nlist_build(sph_box *box, int64_t cap, int64_t *iap, int64_t *nip_out)
 {
     if (!box || !nip_out || cap < 0 || (cap > 0 && !iap)) return fail(SPH_ERR_ARG, "invalid argument");
+    if (int rc = settle(box)) return rc;
     Arena &A = box->ar;
     hipStream_t st = A.ctx->stream;
     HIPCHK(hipSetDevice(A.ctx->device));
@@ -999,6 +1146,7 @@ int sph_separations(sph_box *box, int64_t nip, const int64_t *iap, double *drij,
                     double *rsq, double *dv)
 {
     if (!box || nip < 0 || (nip > 0 && !iap)) return fail(SPH_ERR_ARG, "invalid argument");
+    if (int rc = settle(box)) return rc;
     if (nip == 0) return SPH_OK;
     Arena &A = box->ar;
     hipStream_t st = A.ctx->stream;
@@ -1025,6 +1173,25 @@ int sph_separations(sph_box *box, int64_t nip, const int64_t *iap, double *drij,
     if (rij) HIPCHK(hipMemcpyAsync(rij, d_rij, sizeof(double) * nip, hipMemcpyDeviceToHost, st));
     if (rsq) HIPCHK(hipMemcpyAsync(rsq, d_rsq, sizeof(double) * nip, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return SPH_OK;
+}
+
+int sph_fp32_accumulate(sph_ctx *ctx, int64_t n, const double *x, int unroll, double *kahan, double *plain)
+{
+    if (!ctx || n < 0 || n > (1ll << 24) || (n > 0 && !x) || !kahan || !plain || unroll < 1 || unroll > 64)
+        return fail(SPH_ERR_ARG, "invalid argument (n <= 2^24 terms, unroll 1..64)");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(double) * ((size_t)n + 2);
+    if (ctx->scratch(bytes)) return SPH_ERR_NOMEM;
+    double *d_x = ctx->d_scratch, *d_out = ctx->d_scratch + n;
+    if (n > 0) HIPCHK(hipMemcpyAsync(d_x, x, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    launch_fp32_accumulate(n, d_x, unroll, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    double out[2];
+    HIPCHK(hipMemcpyAsync(out, d_out, sizeof(out), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *kahan = out[0];
+    *plain = out[1];
     return SPH_OK;
 }
 
@@ -1069,6 +1236,7 @@ int sph_kernel_lucy(sph_ctx *ctx, int64_t np, const double *r, const double *dx,
 // the current state; every derived field ends in the new slot order.
 static int eval_box(sph_box *box, int grad, int rates)
 {
+    if (int rc = settle(box)) return rc;
     Arena &A = box->ar;
     hipStream_t st = A.ctx->stream;
     HIPCHK(hipSetDevice(A.ctx->device));
@@ -1153,6 +1321,11 @@ int sph_step(sph_box *box, int64_t nsteps)
     if (nsteps == 0) return SPH_OK;
     Arena &A = box->ar;
     HIPCHK(hipSetDevice(A.ctx->device));
+    if (A.open_run) {  // continue the open run by nsteps, then close it
+        if (int rc = A.advance(box->b, nsteps)) return rc;
+        if (int rc = A.close_run(box->b)) return rc;
+        return check_health(A, box->b);
+    }
     BoxDyn D;
     memset(&D, 0, sizeof(D));
     D.state = BOX_RUN;
@@ -1167,9 +1340,41 @@ int sph_step(sph_box *box, int64_t nsteps)
     return check_health(A, box->b);
 }
 
+int sph_box_advance(sph_box *box, int64_t nsteps)
+{
+    if (!box || nsteps < 0) return fail(SPH_ERR_ARG, "invalid argument");
+    if (nsteps == 0) return SPH_OK;
+    Arena &A = box->ar;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    if (int rc = A.advance(box->b, nsteps)) return rc;
+    if (!A.open_run) return check_health(A, box->b);  // a health bit ended the run
+    return SPH_OK;
+}
+
+int sph_box_sync(sph_box *box)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    return settle(box);
+}
+
+int sph_box_run_stats(sph_box *box, int64_t *steps, int64_t *builds, int64_t *extends, int *open)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    Arena &A = box->ar;
+    HIPCHK(hipSetDevice(A.ctx->device));
+    if (int rc = A.read_dyn()) return rc;
+    const BoxDyn &D = A.h_dyn[box->b];
+    if (steps) *steps = D.k;
+    if (builds) *builds = D.builds;
+    if (extends) *extends = D.extends;
+    if (open) *open = A.open_run ? 1 : 0;
+    return SPH_OK;
+}
+
 int sph_energy(sph_box *box, double *ke, double *ue, double *etot)
 {
     if (!box) return fail(SPH_ERR_ARG, "null box");
+    if (int rc = settle(box)) return rc;
     Arena &A = box->ar;
     HIPCHK(hipSetDevice(A.ctx->device));
     double out[3];
@@ -1206,6 +1411,7 @@ int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch)
 {
     if (!box || !us_per_launch || pass < 0 || pass > 2 || reps < 1 || reps > 100000)
         return fail(SPH_ERR_ARG, "invalid argument (pass 0..2, reps 1..100000)");
+    if (int rc = settle(box)) return rc;
     Arena &A = box->ar;
     HIPCHK(hipSetDevice(A.ctx->device));
     if (!A.stepped) return fail(SPH_ERR_ARG, "box has not been stepped (step it once before timing a pass)");
@@ -1419,7 +1625,7 @@ int sph_box_set_graph(sph_box *box, int on)
     if (!box) return fail(SPH_ERR_ARG, "null box");
     box->ar.use_graph = on ? 1 : 0;
     box->ar.invalidate_graph();
-    return SPH_OK;
+    return box->ar.prepare_graphs();
 }
 
 int sph_host_alloc(int64_t bytes, void **out)
@@ -1454,6 +1660,7 @@ int sph_box_snapshot_wait(sph_box *box)
 int sph_box_snapshot(sph_box *box, double *r, double *v, double *u, double *m)
 {
     if (!box) return fail(SPH_ERR_ARG, "null box");
+    if (int rc = settle(box)) return rc;
     Arena &A = box->ar;
     HIPCHK(hipSetDevice(A.ctx->device));
     if (int rc = sph_box_snapshot_wait(box)) return rc;  // the staging buffer is reused
@@ -1563,7 +1770,7 @@ static int set_schedule(Arena &A, int unroll)
     if (unroll != 0 && unroll != 1 && unroll != 2 && unroll != 4)
         return fail(SPH_ERR_ARG, "unroll must be 0 (default), 1, 2 or 4");
     A.unroll = unroll;
-    A.invalidate_graph();
+    A.check_graph_sig();
     return SPH_OK;
 }
 
@@ -1576,7 +1783,8 @@ int sph_arena_set_schedule(sph_arena *ar, int unroll)
 int sph_box_set_schedule(sph_box *box, int unroll)
 {
     if (!box) return fail(SPH_ERR_ARG, "null box");
-    return set_schedule(box->ar, unroll);
+    if (int rc = set_schedule(box->ar, unroll)) return rc;
+    return box->ar.prepare_graphs();
 }
 
 static int set_rates_unroll(Arena &A, int unroll)
@@ -1584,7 +1792,7 @@ static int set_rates_unroll(Arena &A, int unroll)
     if (unroll != 0 && unroll != 1 && unroll != 2 && unroll != 4)
         return fail(SPH_ERR_ARG, "rate-pass unroll must be 0 (as the other passes), 1, 2 or 4");
     A.unroll_rates = unroll;
-    A.invalidate_graph();
+    A.check_graph_sig();
     return SPH_OK;
 }
 
@@ -1597,7 +1805,8 @@ int sph_arena_set_rates_unroll(sph_arena *ar, int unroll)
 int sph_box_set_rates_unroll(sph_box *box, int unroll)
 {
     if (!box) return fail(SPH_ERR_ARG, "null box");
-    return set_rates_unroll(box->ar, unroll);
+    if (int rc = set_rates_unroll(box->ar, unroll)) return rc;
+    return box->ar.prepare_graphs();
 }
 
 int sph_arena_footprint(int64_t particle_capacity, int precision, int64_t *bytes)

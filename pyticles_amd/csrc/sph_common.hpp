@@ -43,13 +43,6 @@ enum Health : int {
     HEALTH_LISTCAP = 8,     // a particle had more neighbours than the list holds
 };
 
-// Largest box side whose FP32C position mirror is box-relative fixed point
-// (resolution FIXBOX_MAX_L / 2^32 = 1.2e-7); larger boxes use the
-// build-cell-relative mirror (exact separations).  At side 1024 the
-// box-relative form raised the force error against the FP64 oracle to
-// 8.9e-7 (cell-relative: 5.3e-7); at 512 it is 6.5e-7.
-constexpr double FIXBOX_MAX_L = 512.0;
-
 // Static description of one box (host-owned, uploaded when it changes).
 struct BoxDesc {
     double Lx, Ly, hLx, hLy;       // sides and half sides (minimum image)

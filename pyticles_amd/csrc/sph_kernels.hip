@@ -18,6 +18,8 @@
 //                                       spam_complete_force.py:158-165]
 //   P4     leapfrog kick/drift, predictor, periodic wrap, skin check
 #include <math.h>
+#include <type_traits>
+#include "pyticles_hip.h"
 #include "sph_common.hpp"
 #include "sph_kernels.hpp"
 #include "sph_sort.hpp"
@@ -1003,9 +1005,13 @@ __global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
     const double p = vdw_p(rho, T, B);
     A.rho[s] = make_double2(rho, rhol);
     A.T[s] = T;
-    const double ps = (B.terms & 1u) ? p / (rho * rho) : 0.0;
-    A.S[s] = make_double4(ps, 0.0, ps, mi);  // .w carries m for the rate pass
-    A.Q[s] = make_double2(0.0, 0.0);
+    // the stress record (m in .w for the rate pass) only when the gradient
+    // pass will not write it; q/rho^2 is read only with conduction, when the
+    // gradient pass writes it
+    if (!(B.terms & 4u)) {
+        const double ps = (B.terms & 1u) ? p / (rho * rho) : 0.0;
+        A.S[s] = make_double4(ps, 0.0, ps, mi);
+    }
     A.vt[s] = make_double2(mi / rho, T);
     A.aco[s] = coh ? make_double2(-2.0 * B.a * d.gx, -2.0 * B.a * d.gy) : make_double2(0.0, 0.0);
     A.uco[s] = coh ? -B.a * d.ev : 0.0;
@@ -1091,8 +1097,7 @@ __global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int 
         pyy = -B.eta * (2.0 * g.g11 - tr) - B.zeta * tr;
         pxy = -B.eta * (g.g01 + g.g10);
         const double pr = (B.terms & 1u) ? vdw_p(rho, Ti, B) : 0.0;
-        const double4 S0 = A.S[s];
-        A.S[s] = make_double4((pr + pxx) / r2i, pxy / r2i, (pr + pyy) / r2i, S0.w);
+        A.S[s] = make_double4((pr + pxx) / r2i, pxy / r2i, (pr + pyy) / r2i, A.m[s]);
     }
     double qx = 0, qy = 0;
     if (B.terms & 8u) {
@@ -1192,6 +1197,9 @@ __global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
 // coordinates, Kahan-compensated sums, FP64 EOS / integration / energy.
 // ===========================================================================
 
+using true_t = std::integral_constant<bool, true>;
+using false_t = std::integral_constant<bool, false>;
+
 struct Kahan {
     float s, c;
     __device__ __forceinline__ Kahan() : s(0.f), c(0.f) {}
@@ -1206,8 +1214,34 @@ struct Kahan {
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
+// The FP32C walks' accumulation scheme on a plain sequence, for tests: the
+// terms rounded to FP32, summed in FP32 rounds of `unroll` terms, each round
+// added to a Kahan total (out2[0]); beside it the plain FP32 running sum of
+// the same terms (out2[1]).  One work-item does the sums.
+__global__ __launch_bounds__(64) void k_fp32_accumulate(long long n, const double *__restrict__ x, int unroll,
+                                                        double *__restrict__ out2)
+{
+    if (threadIdx.x != 0) return;
+    Kahan k;
+    float plain = 0.f;
+    for (long long i = 0; i < n; i += unroll) {
+        float part = 0.f;
+        for (int u = 0; u < unroll && i + u < n; ++u) {
+            const float t = (float)x[i + u];
+            part += t;
+            plain += t;
+        }
+        k.add(part);
+    }
+    out2[0] = k.value();
+    out2[1] = (double)plain;
+}
+
 struct BoxF {
     double sclx, scly;  // length of one fixed-point unit
+    // FP64 constants of the short-range terms (short density, gradients,
+    // rates), which take FP64 arithmetic on exact separations
+    double h2d, cutsd, ihd, ih3d, fhd;
     float cut2, cutl, h2, cuts;
     float ih, ihl, qh, qhl, fh, fhl, munif;
     float ih3, ihl3;  // 3/h, 3/h_l: 1 + 3q = fma(r, 3/h, 1)
@@ -1226,6 +1260,11 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
     }
     f.munif = (float)B.munif;
     f.fixbox = B.fixbox;
+    f.h2d = B.h2;
+    f.cutsd = B.R2list < B.h2 ? B.R2list : B.h2;
+    f.ihd = B.ih;
+    f.ih3d = 3.0 * B.ih;
+    f.fhd = B.fh;
     f.cut2 = (float)B.cut2; f.cutl = (float)(B.cut2 < B.hl2 ? B.cut2 : B.hl2); f.h2 = (float)B.h2;
     f.cuts = (float)(B.R2list < B.h2 ? B.R2list : B.h2);
     f.ih = (float)B.ih; f.ihl = (float)B.ihl; f.qh = (float)B.qh; f.qhl = (float)B.qhl;
@@ -1234,6 +1273,8 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
     f.ncx = B.ncx; f.ncy = B.ncy; f.sx = B.sx; f.sy = B.sy; f.terms = B.terms;
     if (uniform) {
         f.sclx = wave_uniform(f.sclx); f.scly = wave_uniform(f.scly);
+        f.h2d = wave_uniform(f.h2d); f.cutsd = wave_uniform(f.cutsd); f.ihd = wave_uniform(f.ihd);
+        f.ih3d = wave_uniform(f.ih3d); f.fhd = wave_uniform(f.fhd);
         f.cut2 = wave_uniform(f.cut2); f.cutl = wave_uniform(f.cutl); f.h2 = wave_uniform(f.h2);
         f.cuts = wave_uniform(f.cuts); f.ih = wave_uniform(f.ih); f.ihl = wave_uniform(f.ihl);
         f.qh = wave_uniform(f.qh); f.qhl = wave_uniform(f.qhl); f.fh = wave_uniform(f.fh); f.fhl = wave_uniform(f.fhl);
@@ -1248,18 +1289,19 @@ __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
 // Separation of rj from ri (fixed-point records): the integer difference
 // plus the cell offset (with the periodic wrap of the cell index) is exact in
 // an int, and so is the minimum image of an axis with a single cell (period
-// 2^29 units); one FP64 scale and one rounding to FP32 at the end.
+// 2^29 units); one FP64 scale gives the separation of the two recorded
+// positions to FP64 precision (sep32 rounds it once more to FP32).
 // WRAP: 0 = interior wave (no wrap), 1 = wrap of cell offsets, 2 = also the
 // fixed-point wrap of single-cell axes, 3 = box-relative records (the 32-bit
 // difference is the minimum image).
 template <int WRAP>
-__device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const BoxF &F, float &dx, float &dy)
+__device__ __forceinline__ void sep64(const float4 &ri, const float4 &rj, const BoxF &F, double &dx, double &dy)
 {
     if constexpr (WRAP == 3) {
         const int ix = (int)(__float_as_uint(rj.x) - __float_as_uint(ri.x));
         const int iy = (int)(__float_as_uint(rj.y) - __float_as_uint(ri.y));
-        dx = (float)((double)ix * F.sclx);
-        dy = (float)((double)iy * F.scly);
+        dx = (double)ix * F.sclx;
+        dy = (double)iy * F.scly;
         return;
     }
     const int ci = __float_as_int(ri.w), cj = __float_as_int(rj.w);
@@ -1278,8 +1320,18 @@ __device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const 
         iy = iy > (1 << (FIX_SHIFT - 1)) ? iy - (1 << FIX_SHIFT) : iy;
         iy = iy < -(1 << (FIX_SHIFT - 1)) ? iy + (1 << FIX_SHIFT) : iy;
     }
-    dx = (float)((double)ix * F.sclx);
-    dy = (float)((double)iy * F.scly);
+    dx = (double)ix * F.sclx;
+    dy = (double)iy * F.scly;
+}
+
+// The same separation rounded once to FP32 (the long-range walk).
+template <int WRAP>
+__device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const BoxF &F, float &dx, float &dy)
+{
+    double x, y;
+    sep64<WRAP>(ri, rj, F, x, y);
+    dx = (float)x;
+    dy = (float)y;
 }
 
 // Wrap mode of sep32 for the active lanes: 3 for box-relative records, 0
@@ -1298,7 +1350,8 @@ __device__ __forceinline__ int wave_wrap32(const float4 &ri, const BoxF &F)
 }
 
 struct Dens32 {
-    Kahan rho, rhol, gx, gy, ev;
+    double rho;  // short-range density: FP64 terms on exact separations
+    Kahan rhol, gx, gy, ev;
 };
 
 // The FP32 walks take pair distances from the hardware square root
@@ -1307,14 +1360,17 @@ struct Dens32 {
 
 // FP32 sums: the terms of one unrolled round (U32 list entries) are added in
 // FP32, and each round's partial sum, times the kernel constant it leaves
-// out, goes into a Kahan-compensated total.
-// UNIM (box-relative records only): all masses equal, none fetched.
+// out, goes into a Kahan-compensated total.  The short-range density (r < h:
+// the density the EOS turns into T and p, where a relaxed liquid amplifies
+// any error about tenfold) takes FP64 terms on the exact separations of the
+// same records; its constants scale the FP64 total at the end.
+// UNIM: all masses equal, none fetched (the records' mass is not read).
 template <int U32, bool COH, int WRAP, bool UNIM>
 __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F, int s, Dens32 &d)
 {
     // the kernel constants (and, with UNIM, the mass) scale each round's
     // partial sums instead of every term
-    const float cqh = UNIM ? F.qh * F.munif : F.qh, cqhl = UNIM ? F.qhl * F.munif : F.qhl;
+    const float cqhl = UNIM ? F.qhl * F.munif : F.qhl;
     const float cfhl = UNIM ? F.fhl * F.munif : F.fhl;
     const float4 ri = A.rf[s];
     const float2 vi = A.vf[s];
@@ -1334,13 +1390,14 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
             if (WRAP == 3 && !UNIM) mm[u] = (float)A.m[t[u]];
         }
         rounds.next(k0);
-        float grho = 0.f, grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
+        float grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
-            float dx, dy;
-            sep32<WRAP>(ri, rj[u], F, dx, dy);
+            double dxd, dyd;
+            sep64<WRAP>(ri, rj[u], F, dxd, dyd);
+            const float dx = (float)dxd, dy = (float)dyd;
             const float r2 = dx * dx + dy * dy;
-            const float mj = WRAP == 3 ? (UNIM ? 1.f : mm[u]) : rj[u].z;
+            const float mj = UNIM ? 1.f : (WRAP == 3 ? mm[u] : rj[u].z);
             const float vjx = WRAP == 3 ? rj[u].z : vv[u].x, vjy = WRAP == 3 ? rj[u].w : vv[u].y;
             if (k0 + u < c.x && r2 < F.cut2) {
                 const float rr = __builtin_amdgcn_sqrtf(r2);
@@ -1354,13 +1411,13 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
                         gev = fmaf(g, (vjx - vi.x) * dx + (vjy - vi.y) * dy, gev);
                     }
                 }
-                if (r2 < F.h2) {
-                    const float w = fmaf(-rr, F.ih, 1.f);
-                    grho = fmaf(mj * fmaf(rr, F.ih3, 1.f), w * w * w, grho);
-                }
+            }
+            const double r2d = rsq_exact(dxd, dyd);
+            if (k0 + u < c.x && r2d < F.h2d) {
+                const double rr = sqrt_pair(r2d), w = fma(-rr, F.ihd, 1.0);
+                d.rho = fma((double)mj * fma(rr, F.ih3d, 1.0), w * w * w, d.rho);
             }
         }
-        d.rho.add(cqh * grho);
         d.rhol.add(cqhl * grhol);
         if (COH) {
             d.gx.add(cfhl * ggx);
@@ -1390,7 +1447,7 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
             float dx, dy;
             sep32<WRAP>(ri, rj[u], F, dx, dy);
             const float r2 = dx * dx + dy * dy;
-            const float mj = WRAP == 3 ? (UNIM ? 1.f : mm[u]) : rj[u].z;
+            const float mj = UNIM ? 1.f : (WRAP == 3 ? mm[u] : rj[u].z);
             const float vjx = WRAP == 3 ? rj[u].z : vv[u].x, vjy = WRAP == 3 ? rj[u].w : vv[u].y;
             if (k0 + u < c.y && r2 < F.cutl) {
                 const float rr = __builtin_amdgcn_sqrtf(r2);
@@ -1429,28 +1486,28 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     Dens32 d;
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
-        d.rho = Kahan(F.qh);
+        d.rho = 0.0;
         d.rhol = Kahan(F.qhl);
         const int wrap = wave_wrap32(A.rf[s], F);
+        auto walk = [&](auto coh_t, auto unim_t) {
+            constexpr bool C = decltype(coh_t)::value, M = decltype(unim_t)::value;
+            if (wrap == 3) density32_walk<U32, C, 3, M>(A, F, s, d);
+            else if (wrap == 0) density32_walk<U32, C, 0, M>(A, F, s, d);
+            else if (wrap == 1) density32_walk<U32, C, 1, M>(A, F, s, d);
+            else density32_walk<U32, C, 2, M>(A, F, s, d);
+        };
         const bool unim = F.munif > 0.f;
         if (F.terms & 2) {
-            if (wrap == 3) {
-                if (unim) density32_walk<U32, true, 3, true>(A, F, s, d);
-                else density32_walk<U32, true, 3, false>(A, F, s, d);
-            } else if (wrap == 0) density32_walk<U32, true, 0, false>(A, F, s, d);
-            else if (wrap == 1) density32_walk<U32, true, 1, false>(A, F, s, d);
-            else density32_walk<U32, true, 2, false>(A, F, s, d);
+            if (unim) walk(true_t{}, true_t{});
+            else walk(true_t{}, false_t{});
         } else {
-            if (wrap == 3) {
-                if (unim) density32_walk<U32, false, 3, true>(A, F, s, d);
-                else density32_walk<U32, false, 3, false>(A, F, s, d);
-            } else if (wrap == 0) density32_walk<U32, false, 0, false>(A, F, s, d);
-            else if (wrap == 1) density32_walk<U32, false, 1, false>(A, F, s, d);
-            else density32_walk<U32, false, 2, false>(A, F, s, d);
+            if (unim) walk(false_t{}, true_t{});
+            else walk(false_t{}, false_t{});
         }
     });
     const float2 vi = A.vf[s];
-    const double rd = d.rho.value(), rl = d.rhol.value();
+    // the short-range sum leaves out q_h (and the common mass)
+    const double rd = fma(B.qh * (B.munif > 0.0 ? B.munif : 1.0), d.rho, B.qh), rl = d.rhol.value();
     const double mi = A.m[s];
     double ui = A.up[s];
     if (st == BOX_INIT) {
@@ -1463,15 +1520,18 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     const double p = vdw_p(rd, T, B);
     A.rho[s] = make_double2(rd, rl);
     A.T[s] = T;
-    const double ps = (B.terms & 1u) ? p / (rd * rd) : 0.0;
-    A.S[s] = make_double4(ps, 0.0, ps, mi);
-    A.S32[s] = make_float4((float)ps, 0.f, (float)ps, (float)mi);
-    A.Q[s] = make_double2(0.0, 0.0);
+    // records the gradient pass rewrites are written here only when it will
+    // not run (stress without viscosity; the rate pass's v / q record
+    // without viscosity and conduction)
+    if (!(B.terms & 4u)) {
+        const double ps = (B.terms & 1u) ? p / (rd * rd) : 0.0;
+        A.S32[s] = make_float4((float)ps, 0.f, (float)ps, (float)mi);
+    }
     if (B.fixbox) {  // rf carries v
-        A.q32[s] = make_float2(0.f, 0.f);
+        if (!(B.terms & 12u)) A.q32[s] = make_float2(0.f, 0.f);
         A.mt32[s] = make_float2((float)(mi / rd), (float)T);
     } else {
-        A.vq32[s] = make_float4(vi.x, vi.y, 0.f, 0.f);
+        if (!(B.terms & 12u)) A.vq32[s] = make_float4(vi.x, vi.y, 0.f, 0.f);
         A.vt32[s] = make_float4(vi.x, vi.y, (float)(mi / rd), (float)T);
     }
     A.aco[s] = coh ? make_double2(-2.0 * B.a * d.gx.value(), -2.0 * B.a * d.gy.value()) : make_double2(0.0, 0.0);
@@ -1481,82 +1541,81 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
 }
 
 struct Grad32 {
-    Kahan g00, g01, g10, g11, gt0, gt1;
+    double g00, g01, g10, g11, gt0, gt1;
 };
 
+// P2 walk of the FP32C form: FP32 neighbour records, FP64 arithmetic on the
+// exact separations of the fixed-point positions.  On CDNA4 an FP64 FMA
+// issues at the FP32 rate, so the short-range terms keep their precision at
+// no VALU premium beyond the conversions; FP32 rounding of these terms (and
+// of the temperature difference) left up to 4e-6 of force and heating error
+// on relaxed liquid states, where the terms cancel to a small net rate.
+// The temperature comes from the FP64 T of the density pass.
 template <int U32, int WRAP>
 __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, int s, Grad32 &g)
 {
-    // WRAP 3 (box-relative records): v from rf, (m/rho, T) from the 8-byte
-    // mt32, so a neighbour costs 24 bytes of gathers instead of 32
+    // WRAP 3 (box-relative records): v from rf and m/rho from mt32;
+    // otherwise v and m/rho from vt32
     const float4 ri = A.rf[s];
-    float2 vi;
-    float Ti;
+    double vix, viy;
     if (WRAP == 3) {
-        vi = make_float2(ri.z, ri.w);
-        Ti = A.mt32[s].y;
+        vix = ri.z;
+        viy = ri.w;
     } else {
         const float4 vti = A.vt32[s];
-        vi = make_float2(vti.x, vti.y);
-        Ti = vti.w;
+        vix = vti.x;
+        viy = vti.y;
     }
+    const double Ti = A.T[s];
     const int ns = A.cnt[s].x;
     const unsigned *nb = list_base(A, s);
     ListRounds<U32, false, false> rounds(nb, A.lcap, ns, s);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
-        float4 rj[U32], vt[U32];
+        float4 rj[U32];
+        float3 vt[U32];  // v, m/rho
+        double Tj[U32];
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
             if (WRAP == 3) {
-                const float2 mt = A.mt32[t[u]];
-                vt[u] = make_float4(0.f, 0.f, mt.x, mt.y);
+                vt[u] = make_float3(rj[u].z, rj[u].w, A.mt32[t[u]].x);
             } else {
-                vt[u] = A.vt32[t[u]];
+                const float4 q = A.vt32[t[u]];
+                vt[u] = make_float3(q.x, q.y, q.z);
             }
+            Tj[u] = A.T[t[u]];
         }
         rounds.next(k0);
-        float a00 = 0.f, a01 = 0.f, a10 = 0.f, a11 = 0.f, at0 = 0.f, at1 = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
-            float dx, dy;
-            sep32<WRAP>(ri, rj[u], F, dx, dy);
-            const float r2 = dx * dx + dy * dy;
-            if (k0 + u < ns && r2 < F.cuts) {
-                const float rr = __builtin_amdgcn_sqrtf(r2);
-                const float q = rr * F.ih, w = 1.f - q;
-                const float Fk = F.fh * w * w;
-                const float gx = -Fk * dx, gy = -Fk * dy;
-                const float vj = vt[u].z;
-                const float vjx = WRAP == 3 ? rj[u].z : vt[u].x, vjy = WRAP == 3 ? rj[u].w : vt[u].y;
-                const float dvx = vjx - vi.x, dvy = vjy - vi.y;
-                const float dT = vt[u].w - Ti;
-                a00 += vj * dvx * gx;
-                a01 += vj * dvx * gy;
-                a10 += vj * dvy * gx;
-                a11 += vj * dvy * gy;
-                at0 += vj * dT * gx;
-                at1 += vj * dT * gy;
+            double dx, dy;
+            sep64<WRAP>(ri, rj[u], F, dx, dy);
+            const double r2 = rsq_exact(dx, dy);
+            if (k0 + u < ns && r2 < F.cutsd) {
+                const double rr = sqrt_pair(r2), w = fma(-rr, F.ihd, 1.0);
+                const double Fk = F.fhd * w * w;
+                const double gx = -Fk * dx, gy = -Fk * dy;
+                const double vj = vt[u].z;
+                const double dvx = (double)vt[u].x - vix, dvy = (double)vt[u].y - viy;
+                const double dT = Tj[u] - Ti;
+                g.g00 += vj * dvx * gx;
+                g.g01 += vj * dvx * gy;
+                g.g10 += vj * dvy * gx;
+                g.g11 += vj * dvy * gy;
+                g.gt0 += vj * dT * gx;
+                g.gt1 += vj * dT * gy;
             }
         }
-        g.g00.add(a00);
-        g.g01.add(a01);
-        g.g10.add(a10);
-        g.g11.add(a11);
-        g.gt0.add(at0);
-        g.gt1.add(at1);
     }
 }
 
-// P2 in FP32; neighbour record: rf and (v, m/rho, T) from P1, or rf and
-// (m/rho, T) for box-relative records.  Writes the rate pass's (v, q/rho^2)
-// record (q/rho^2 alone for box-relative records).
-// Five waves per SIMD (94 VGPRs, no spill) instead of the compiler's four:
-// 4% faster at C4.  The other pass kernels gained nothing or spilled.
+// P2 of the FP32C form; neighbour record: rf, (v, m/rho) from P1 and the
+// FP64 T, or rf, m/rho and T for box-relative records.  Writes the rate
+// pass's (v, q/rho^2) record (q/rho^2 alone for box-relative records).
 template <int U32>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_grad32(ArenaView A, int state_mask, int diag)
+__global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int diag)
 {
     const int s = xcd_block() * NT + threadIdx.x;
     if (s >= A.n) return;
@@ -1565,7 +1624,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     if (!state_in(A.dyn[b].state, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
     if (!diag && !(B.terms & 12u)) return;
-    Grad32 g;
+    Grad32 g = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
         const int wrap = wave_wrap32(A.rf[s], F);
@@ -1575,7 +1634,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
         else grad32_walk<U32, 2>(A, F, s, g);
     });
     const float2 vi = A.vf[s];
-    const double G00 = g.g00.value(), G01 = g.g01.value(), G10 = g.g10.value(), G11 = g.g11.value();
+    const double G00 = g.g00, G01 = g.g01, G10 = g.g10, G11 = g.g11;
     const double rho = A.rho[s].x, r2i = rho * rho;
     double qx = 0, qy = 0;
     if (B.terms & 4u) {
@@ -1584,15 +1643,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
         const double pyy = -B.eta * (2.0 * G11 - tr) - B.zeta * tr;
         const double pxy = -B.eta * (G01 + G10);
         const double pr = (B.terms & 1u) ? vdw_p(rho, A.T[s], B) : 0.0;
-        const double4 S0 = A.S[s];
-        const double4 Sn = make_double4((pr + pxx) / r2i, pxy / r2i, (pr + pyy) / r2i, S0.w);
-        A.S[s] = Sn;
-        A.S32[s] = make_float4((float)Sn.x, (float)Sn.y, (float)Sn.z, (float)Sn.w);
+        A.S32[s] = make_float4((float)((pr + pxx) / r2i), (float)(pxy / r2i), (float)((pr + pyy) / r2i),
+                               (float)A.m[s]);
     }
     if (B.terms & 8u) {
-        qx = -B.kcond * g.gt0.value();
-        qy = -B.kcond * g.gt1.value();
-        A.Q[s] = make_double2(qx / r2i, qy / r2i);
+        qx = -B.kcond * g.gt0;
+        qy = -B.kcond * g.gt1;
     }
     if (B.fixbox) A.q32[s] = make_float2((float)(qx / r2i), (float)(qy / r2i));
     else A.vq32[s] = make_float4(vi.x, vi.y, (float)(qx / r2i), (float)(qy / r2i));
@@ -1603,9 +1659,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
 }
 
 struct Rate32 {
-    Kahan ax, ay, ud;
+    double ax, ay, ud;
 };
 
+// P3 walk of the FP32C form: FP32 neighbour records, FP64 arithmetic on the
+// exact separations (see grad32_walk).
 template <int U32, bool COND, int WRAP>
 __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, int s, int ns, Rate32 &o)
 {
@@ -1638,36 +1696,34 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
             }
         }
         rounds.next(k0);
-        float px = 0.f, py = 0.f, pu = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
-            float dx, dy;
-            sep32<WRAP>(ri, rj[u], F, dx, dy);
-            const float r2 = dx * dx + dy * dy;
-            if (k0 + u < ns && r2 < F.cuts) {
-                const float rr = __builtin_amdgcn_sqrtf(r2);
-                const float q = rr * F.ih, w = 1.f - q;
-                const float Fk = F.fh * w * w;
-                const float wx = Fk * dx, wy = Fk * dy;
-                const float axx = Si.x + Sj[u].x, axy = Si.y + Sj[u].y, ayy = Si.z + Sj[u].z;
-                const float fx = axx * wx + axy * wy;
-                const float fy = axy * wx + ayy * wy;
-                const float cc = COND ? (vqi.z + vq[u].z) * wx + (vqi.w + vq[u].w) * wy : 0.f;
-                const float mj = Sj[u].w;
-                const float e = 0.5f * ((vq[u].x - vqi.x) * fx + (vq[u].y - vqi.y) * fy);
-                px += mj * fx;
-                py += mj * fy;
-                pu += mj * (e + cc);
+            double dx, dy;
+            sep64<WRAP>(ri, rj[u], F, dx, dy);
+            const double r2 = rsq_exact(dx, dy);
+            if (k0 + u < ns && r2 < F.cutsd) {
+                const double rr = sqrt_pair(r2), w = fma(-rr, F.ihd, 1.0);
+                const double Fk = F.fhd * w * w;
+                const double wx = Fk * dx, wy = Fk * dy;
+                const double axx = (double)Si.x + (double)Sj[u].x, axy = (double)Si.y + (double)Sj[u].y;
+                const double ayy = (double)Si.z + (double)Sj[u].z;
+                const double fx = axx * wx + axy * wy;
+                const double fy = axy * wx + ayy * wy;
+                const double cc = COND ? ((double)vqi.z + (double)vq[u].z) * wx + ((double)vqi.w + (double)vq[u].w) * wy
+                                       : 0.0;
+                const double mj = Sj[u].w;
+                const double e = 0.5 * (((double)vq[u].x - (double)vqi.x) * fx + ((double)vq[u].y - (double)vqi.y) * fy);
+                o.ax += mj * fx;
+                o.ay += mj * fy;
+                o.ud += mj * (e + cc);
             }
         }
-        o.ax.add(px);
-        o.ay.add(py);
-        o.ud.add(pu);
     }
 }
 
-// P3 in FP32 over the short entries; neighbour record: rf, (v, q/rho^2),
-// (S, m); for box-relative records rf, (S, m) and, with conduction, q/rho^2.  Adds the cohesive part from P1.
+// P3 of the FP32C form over the short entries; neighbour record: rf,
+// (v, q/rho^2), (S, m); for box-relative records rf, (S, m) and, with
+// conduction, q/rho^2.  Adds the cohesive part from P1.
 template <int U32>
 __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
 {
@@ -1678,7 +1734,7 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
     if (!state_in(A.dyn[b].state, state_mask)) return;
     const BoxDesc &B = A.boxes[b];
     const int ns = (B.terms & 13u) ? A.cnt[s].x : 0;
-    Rate32 o;
+    Rate32 o = {0.0, 0.0, 0.0};
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
         const int wrap = wave_wrap32(A.rf[s], F);
@@ -1695,7 +1751,7 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
         }
     });
     const double2 ac = A.aco[s];
-    const double axd = o.ax.value() + ac.x, ayd = o.ay.value() + ac.y, udd = o.ud.value() + A.uco[s];
+    const double axd = o.ax + ac.x, ayd = o.ay + ac.y, udd = o.ud + A.uco[s];
     if (!isfinite(axd) || !isfinite(ayd) || !isfinite(udd)) flag_health(A.dyn, b, HEALTH_NONFINITE);
     A.acc[s] = make_double2(axd, ayd);
     A.udot[s] = udd;
@@ -1741,6 +1797,9 @@ __device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, i
     rn.x = wrap_coord(r.x + B.dt * vx, B.Lx);
     rn.y = wrap_coord(r.y + B.dt * vy, B.Ly);
     A.r[s] = rn;
+    // a non-finite velocity (or energy) of a particle without neighbours
+    // reaches no rate; catch it in the state
+    if (!isfinite(rn.x) || !isfinite(rn.y) || !isfinite(un)) flag_health(A.dyn, b, HEALTH_NONFINITE);
     const double hdt = 0.5 * B.dt;  // predictor to the next integer time
     const double2 vp = make_double2(vx + hdt * a.x, vy + hdt * a.y);
     A.vp[s] = vp;
@@ -1763,7 +1822,10 @@ __device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, i
 // wholly inside the box, in its dyn.dmax2 for the waves it shares.
 __device__ __forceinline__ bool list_review(const BoxDyn &D)
 {
-    return D.state == BOX_RUN && !building(D, 0) && D.k + 1 < D.nsteps && D.k + 1 >= D.next_rebuild;
+    // not at the closing half kick (k = nsteps), which moves nothing; the
+    // step before it reviews as any other, so the schedule of a run does
+    // not depend on where its closing step lies (an open run has none)
+    return D.state == BOX_RUN && !building(D, 0) && D.k < D.nsteps && D.k + 1 >= D.next_rebuild;
 }
 
 // Integrate every running slot; on a box's review step also record the
@@ -1833,10 +1895,11 @@ __device__ __forceinline__ int64_t reuse_steps(const BoxDesc &B, const BoxDyn &D
 }
 
 // Steps a box may still reuse its list at a scheduled build step: the
-// largest c with  c dt v + (c dt)^2 a <= 0.8 (skin/2 - d)  for the speed and
-// acceleration measured at the last build and the largest current
-// displacement d from the build positions (reuse_steps' bound applied to the
-// margin that is left),
+// largest c with  c dt v + (c dt)^2 a <= 0.8 (skin/2 - d)  for the largest
+// current displacement d from the build positions, the acceleration a
+// measured at the last build and the speed v that bounds the particles now:
+// the speed measured at the last build plus what that acceleration adds over
+// the list's age (reuse_steps' bound applied to the margin that is left),
 // capped so that a list lives at most 64 steps; 0 = build now.  Exceeding
 // half the skin anyway raises HEALTH_SKIN in k_integrate, as before.
 __device__ __forceinline__ int64_t extend_steps(const BoxDesc &B, const BoxDyn &D, double dmax2)
@@ -1847,9 +1910,10 @@ __device__ __forceinline__ int64_t extend_steps(const BoxDesc &B, const BoxDyn &
     const double d = sqrt(dmax2);
     const double C = 0.8 * (0.5 * B.skin - d);
     if (!(C > 0.0)) return 0;
-    const double Bq = B.dt * D.vlast, Aq = B.dt * B.dt * D.alast;
+    const int64_t age = D.k + 1 - D.built_at;  // list age at the next step
+    const double Bq = B.dt * (D.vlast + (double)age * B.dt * D.alast), Aq = B.dt * B.dt * D.alast;
     const double c = (-Bq + sqrt(Bq * Bq + 4.0 * Aq * C)) / (2.0 * Aq);
-    const int64_t room = 64 - (D.k + 1 - D.built_at);  // list age at the next step
+    const int64_t room = 64 - age;
     int64_t e = (int64_t)floor(c);
     e = e > room ? room : e;
     return e < 0 ? 0 : e;
@@ -2003,7 +2067,13 @@ __global__ __launch_bounds__(NT) void k_download(ArenaView A, int box, int field
     case 12: { const double2 q = A.jq[s]; dst[2 * k] = q.x; dst[2 * k + 1] = q.y; } break;
     case 13: {  // P = p_rev + p_rev_lr + pi_irr (spam_complete_force.py:178)
         const double rho = A.rho[s].x, rl = A.rho[s].y, r2 = rho * rho;
-        const double4 S = A.S[s];
+        double4 S;
+        if (B.precision == SPH_FP32C) {  // the FP32C form keeps its stress record only
+            const float4 f = A.S32[s];
+            S = make_double4(f.x, f.y, f.z, f.w);
+        } else {
+            S = A.S[s];
+        }
         const double pco = (B.terms & 2u) ? -B.a * rl * rl : 0.0;
         dst[4 * k] = S.x * r2 + pco;
         dst[4 * k + 1] = S.y * r2;
@@ -2429,6 +2499,10 @@ void launch_set_u(const ArenaView &A, int box, int n, const double *T, double T0
 void launch_energy_now(const ArenaView &A, int box, double *out3, hipStream_t st)
 {
     hipLaunchKernelGGL(k_energy_now, dim3(1), dim3(NT), 0, st, A, box, out3);
+}
+void launch_fp32_accumulate(long long n, const double *x, int unroll, double *out2, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fp32_accumulate, dim3(1), dim3(64), 0, st, n, x, unroll, out2);
 }
 
 }  // namespace sph

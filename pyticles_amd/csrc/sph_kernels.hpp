@@ -44,6 +44,7 @@ void launch_lucy(long long np, const double *r, const double *dx, double h, int 
 void launch_min_image(long long np, double *dr, double Lx, double Ly, hipStream_t st);
 void launch_set_u(const ArenaView &A, int box, int n, const double *T, double T0, hipStream_t st);
 void launch_energy_now(const ArenaView &A, int box, double *out3, hipStream_t st);
+void launch_fp32_accumulate(long long n, const double *x, int unroll, double *out2, hipStream_t st);
 void launch_splash(long long n, const double *r, const double *m, const double *rho, const double *a,
                    const double *h, const SplashGrid &G, double4 *rec, double4 *rec_sorted, unsigned *keys,
                    unsigned *k_out, unsigned *v_out, unsigned *k_alt, unsigned *v_alt, void *sort_tmp, int key_bits,

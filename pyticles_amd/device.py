@@ -98,6 +98,23 @@ class DeviceBox:
     def step(self, nsteps):
         _lib.check(_lib.lib.sph_step(self._h, int(nsteps)))
 
+    def advance(self, nsteps=1):
+        """nsteps steps of an open run (one rate evaluation each); v and u
+        stay at the half step until sync() or any call that reads them."""
+        _lib.check(_lib.lib.sph_box_advance(self._h, int(nsteps)))
+
+    def sync(self):
+        """Close an open run (closing half kick)."""
+        _lib.check(_lib.lib.sph_box_sync(self._h))
+
+    def run_stats(self):
+        """Counters of the current or last run: steps, builds, extends, open."""
+        k, b, e = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        o = ctypes.c_int()
+        _lib.check(_lib.lib.sph_box_run_stats(self._h, ctypes.byref(k), ctypes.byref(b), ctypes.byref(e),
+                                              ctypes.byref(o)))
+        return dict(steps=k.value, builds=b.value, extends=e.value, open=bool(o.value))
+
     def energy(self):
         ke, ue, et = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _lib.check(_lib.lib.sph_energy(self._h, ctypes.byref(ke), ctypes.byref(ue), ctypes.byref(et)))

@@ -28,14 +28,15 @@ DONE = 3
 
 
 class JobRecord:
-    __slots__ = ("job_id", "n", "steps", "arrival", "admitted", "finished", "box", "result", "health")
+    __slots__ = ("job_id", "n", "steps", "arrival", "admitted", "finished", "box", "result", "health", "error")
 
     def __init__(self, job_id, n, steps, arrival):
         self.job_id, self.n, self.steps, self.arrival = job_id, n, steps, arrival
         self.admitted = self.finished = None
         self.box = None
         self.result = None
-        self.health = 0
+        self.health = 0      # health bits of a job evicted on a numerical failure
+        self.error = None    # message of a job refused at admission
 
     @property
     def latency(self):
@@ -85,6 +86,7 @@ class Server:
         self.arena_steps = 0
         self.particle_steps = 0
         self.max_running = 0
+        self.peak_resident = 0
 
     def close(self):
         self.arena.close()
@@ -107,13 +109,21 @@ class Server:
             except _lib.SphError as e:
                 if e.code == _lib.SPH_ERR_NOMEM:  # no contiguous room yet
                     break
-                raise
+                if e.code != _lib.SPH_ERR_ARG:
+                    raise
+                # a job the arena refuses (invalid or too dense for its
+                # lists) fails alone; the stream goes on
+                self.queue.popleft()
+                rec.error = str(e)
+                rec.finished = now
+                continue
             self.queue.popleft()
             rec.box = bid
             rec.admitted = now
             self.running[bid] = rec
             self.resident += rec.n
             self.max_running = max(self.max_running, len(self.running))
+            self.peak_resident = max(self.peak_resident, self.resident)
 
     def _retire(self, now):
         if not self.running:
@@ -170,7 +180,8 @@ class Server:
                     last_log, sum(r.finished is not None for r in recs), len(self.running), len(self.queue),
                     self.resident))
         self.records = recs
-        self.particle_steps = int(sum(r.n * r.steps for r in recs if r.finished is not None))
+        self.particle_steps = int(sum(r.n * r.steps for r in recs
+                                      if r.finished is not None and not r.health and r.error is None))
         return recs
 
 
@@ -194,8 +205,9 @@ def latency_summary(recs, warmup=0):
         return dict(jobs=0)
     first = min(r.arrival for r in recs)
     last = max(r.finished for r in done)
-    ps = sum(r.n * r.steps for r in done)
+    ps = sum(r.n * r.steps for r in done if not r.health and r.error is None)  # completed work only
     return dict(jobs=int(lat.size), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                 mean_s=float(lat.mean()), max_s=float(lat.max()), window_s=float(last - first),
                 particle_steps=int(ps), particle_steps_per_s=float(ps / max(last - first, 1e-9)),
-                failed=int(sum(1 for r in recs if r.health)))
+                failed=int(sum(1 for r in recs if r.health)),
+                refused=int(sum(1 for r in recs if r.error is not None)))

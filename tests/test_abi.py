@@ -33,26 +33,11 @@ def test_version_and_error_strings_without_gpu():
 
 
 def test_params_struct_layout():
-    # 14 doubles + unsigned + int, no padding surprises
-    assert ctypes.sizeof(_lib.Params) == 14 * 8 + 8
+    # 14 doubles + unsigned + int + the FP32C record limit, no padding surprises
+    assert ctypes.sizeof(_lib.Params) == 15 * 8 + 8
+    assert _lib.Params.fp32_box_relative_max.offset == 14 * 8 + 8
 
 
 def test_library_is_gfx950_code_object():
     data = open(_lib.LIB_PATH, "rb").read()
     assert b"gfx950" in data
-
-
-def test_fp32_box_relative_limit_knob_without_gpu():
-    old = ctypes.c_double()
-    assert _lib.lib.sph_get_fp32_box_relative_max(ctypes.byref(old)) == 0
-    assert old.value == 512.0  # the default: box-relative FP32C mirror up to 512 per side
-    try:
-        assert _lib.lib.sph_set_fp32_box_relative_max(100.0) == 0
-        got = ctypes.c_double()
-        _lib.lib.sph_get_fp32_box_relative_max(ctypes.byref(got))
-        assert got.value == 100.0
-        assert _lib.lib.sph_set_fp32_box_relative_max(-1.0) == _lib.SPH_ERR_ARG
-        assert _lib.lib.sph_set_fp32_box_relative_max(float("inf")) == _lib.SPH_ERR_ARG
-        assert _lib.lib.sph_get_fp32_box_relative_max(None) == _lib.SPH_ERR_ARG
-    finally:
-        _lib.lib.sph_set_fp32_box_relative_max(old.value)

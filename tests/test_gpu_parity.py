@@ -297,19 +297,15 @@ def test_positions_outside_box_rejected():
 # ------------------------------------------------------------------ FP32-compensated form
 @pytest.fixture(params=["box_relative", "cell_relative"])
 def fp32_mirror(request):
-    """Both FP32C position mirrors: box-relative fixed point (boxes up to the
-    process-wide side limit, the default) and build-cell-relative (limit 0)."""
-    old = ctypes.c_double()
-    _lib.check(_lib.lib.sph_get_fp32_box_relative_max(ctypes.byref(old)))
-    if request.param == "cell_relative":
-        _lib.check(_lib.lib.sph_set_fp32_box_relative_max(0.0))
-    yield request.param
-    _lib.check(_lib.lib.sph_set_fp32_box_relative_max(old.value))
+    """Both FP32C position mirrors: box-relative fixed point (a box's
+    fp32_box_relative_max at least its side) and build-cell-relative (0, the
+    default); the value is the parameter that selects it."""
+    return 1e9 if request.param == "box_relative" else 0.0
 
 
 def test_fp32_compensated_error_vs_fp64(fp32_mirror):
     r, v, m, prm = _system(64, seed=12)
-    prm32 = dict(prm, precision=_lib.FP32C)
+    prm32 = dict(prm, precision=_lib.FP32C, fp32_box_relative_max=fp32_mirror)
     u = np.full(r.shape[0], -1.1)
     ref = O.derivatives(r, v, m, u, prm, terms=15)
     with _box(r, v, m, prm32, u=u) as b:
@@ -354,7 +350,7 @@ def test_rectangular_boxes(nx, ny, Lx, Ly, fp32_mirror):
         got = b.download("rho", "acc", "udot")
     assert rel_norm(got["rho"], ref["rho"]) < 1e-13
     assert rel_norm(got["acc"], ref["acc"]) < TOL64 and rel_norm(got["udot"], ref["udot"]) < TOL64
-    with _box(r, v, m, dict(prm, precision=_lib.FP32C), u=u) as b:
+    with _box(r, v, m, dict(prm, precision=_lib.FP32C, fp32_box_relative_max=fp32_mirror), u=u) as b:
         b.rates(15)
         got = b.download("rho", "acc")
     assert rel_norm(got["rho"], ref["rho"]) < 1e-6 and rel_norm(got["acc"], ref["acc"]) < 2e-6

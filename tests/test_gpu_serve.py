@@ -101,3 +101,56 @@ def test_autotune_small_workload(tmp_path):
     best, table = autotune.tune(boxes, _lib.FP32C, "batched", steps=3, warm=1, path=p)
     assert autotune.valid(best)
     assert set(autotune.load_cache(p)) == {"fp64:box", "fp32c:batched"}
+
+
+def test_c5_jobs_served_equal_alone():
+    # jobs of the served workload itself (N log-uniform in [1024, 65536],
+    # T in [0.3, 1.2], their own seeds) with their step counts cut to 20-60
+    # so the test stays short; they arrive while others run
+    prec = _lib.FP32C
+    jobs = []
+    for j in range(24):
+        s = configs.c5_job(j)
+        s.steps = 20 + (7 * j) % 41
+        jobs.append(s)
+    assert max(s.n for s in jobs) > 40000
+    arr = np.linspace(0.0, 0.02, len(jobs))
+    sched = autotune.schedule_for(prec, "batched")
+    srv = Server(200e9, precision=prec, max_resident=4_000_000, chunk=4, keep_results=True, schedule=sched)
+    try:
+        recs = srv.run(jobs, arr)
+    finally:
+        srv.close()
+    assert all(r.finished is not None and r.health == 0 for r in recs), [(r.job_id, r.health) for r in recs]
+    assert srv.max_running >= 2
+    print("[serve] c5 jobs: N %d..%d, %d resident at most" % (
+        min(s.n for s in jobs), max(s.n for s in jobs), srv.peak_resident), flush=True)
+    for rec in recs:
+        ref = alone(jobs[rec.job_id], prec, unroll=sched["unroll"], skin=sched["skin"],
+                    rates_unroll=sched.get("rates_unroll", 0))
+        for k in ("r", "v", "u", "rho"):
+            assert np.array_equal(rec.result[k], ref[k]), (rec.job_id, k)
+
+
+def test_budget_admission_binds_on_device():
+    # an HBM budget that holds ~7,000 particle slots: jobs of 2,000-4,000
+    # particles queue until earlier ones leave, the resident count never
+    # passes what the budget holds, and every result is what it is alone
+    from pyticles_amd.arena import capacity_for, footprint
+    prec = _lib.FP32C
+    jobs = [configs.lognormal_box(900 + j, 2000, 4000, 0.4, 1.0, steps=15 + 3 * j) for j in range(10)]
+    budget = footprint(7000, prec)
+    srv = Server(budget, precision=prec, chunk=2, keep_results=True)
+    try:
+        recs = srv.run(jobs, np.zeros(len(jobs)))
+    finally:
+        srv.close()
+    assert srv.capacity == capacity_for(budget, prec)
+    assert srv.peak_resident <= srv.capacity and footprint(srv.peak_resident, prec) <= budget
+    waited = [r for r in recs if r.admitted > r.arrival + 1e-4]
+    assert len(waited) >= 5, "the budget never held a job back"
+    assert all(r.health == 0 and r.error is None for r in recs)
+    for rec in recs:
+        ref = alone(jobs[rec.job_id], prec)
+        for k in ("r", "v", "u", "rho"):
+            assert np.array_equal(rec.result[k], ref[k]), (rec.job_id, k)
