@@ -63,16 +63,6 @@ typedef struct {
                           step); r_cut = min(sqrt(rc^2 + tol^2), max(h, hl))  */
     unsigned terms;    /* SPH_* rate terms used by sph_step                    */
     int precision;     /* SPH_FP64 or SPH_FP32C                                */
-    double fp32_box_relative_max;
-                       /* SPH_FP32C neighbour records of a box whose sides are
-                          both <= this hold box-relative 32-bit fixed-point
-                          positions with the velocity in the same 16 bytes
-                          (one gather per neighbour in the density pass;
-                          position resolution side / 2^32, which leaves
-                          ~1e-6 of force error on relaxed states of boxes
-                          over ~200 per side); otherwise (0, the default)
-                          build-cell-relative positions, whose separations
-                          are exact.  No counterpart in the reference.    */
 } sph_params;
 
 /* ---- context ------------------------------------------------------------ */
@@ -135,6 +125,13 @@ int sph_fp32_accumulate(sph_ctx *ctx, int64_t n, const double *x, int unroll, do
  * Replaces spkernel.lucy_kernel (spkernel.py:86-118). */
 int sph_kernel_lucy(sph_ctx *ctx, int64_t np, const double *r, const double *dx,
                     double h, int dim, double *w, double *dwdx);
+/* The reference's C-level 3-D Lucy kernel for one separation, with its
+ * signature (c_properties.pxd:1, c_properties.pyx:50-80), so Cython code
+ * that cimports it links against this library: w = W(|r|, h), dwdx[3] =
+ * dW/dr dx / r (0 at r = 0 and for |r| > h), q = 105 / (16 pi h^3), in FP32
+ * as the reference's inline.  A scalar host helper for callers that loop
+ * over pairs themselves; batches of separations go to sph_kernel_lucy. */
+void lucy_kernel_3d(float r, const float *dx, float h, float *w, float *dwdx);
 /* Summation density (short and long h) and EOS (T, p, pco).  Replaces
  * properties.spam_properties (properties.py:63-120) and the density / EOS
  * part of f_properties.spam_properties (f_properties.py:100-120). */

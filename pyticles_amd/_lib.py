@@ -44,8 +44,7 @@ class Params(ctypes.Structure):
     _fields_ = [(name, ctypes.c_double) for name in
                 ("Lx", "Ly", "h", "hl", "rc", "tol", "a", "b", "kB", "eta", "zeta", "kcond", "dt",
                  "skin")] + \
-               [("terms", ctypes.c_uint), ("precision", ctypes.c_int),
-                ("fp32_box_relative_max", ctypes.c_double)]
+               [("terms", ctypes.c_uint), ("precision", ctypes.c_int)]
 
 
 _P = ctypes.c_void_p

@@ -26,15 +26,11 @@ SKIN = 0.5  # neighbour-list skin (lists cover r_cut + skin)
 
 
 def params(L, eta=1.0, zeta=0.0, kcond=5.0, terms=_lib.ALL_TERMS, precision=_lib.FP64,
-           h=H, hl=HL, rc=RC, tol=TOL, a=A_VDW, b=B_VDW, kB=KB, dt=DT, Ly=None, skin=SKIN,
-           fp32_box_relative_max=0.0):
-    """sph_params for a periodic square (or Lx x Ly) box.  FP32C boxes keep
-    build-cell-relative neighbour records (exact separations) unless their
-    sides are within fp32_box_relative_max."""
+           h=H, hl=HL, rc=RC, tol=TOL, a=A_VDW, b=B_VDW, kB=KB, dt=DT, Ly=None, skin=SKIN):
+    """sph_params for a periodic square (or Lx x Ly) box."""
     return dict(Lx=float(L), Ly=float(L if Ly is None else Ly), h=h, hl=hl, rc=rc, tol=tol,
                 a=a, b=b, kB=kB, eta=eta, zeta=zeta, kcond=kcond, dt=dt, skin=skin,
-                terms=int(terms), precision=int(precision),
-                fp32_box_relative_max=float(fp32_box_relative_max))
+                terms=int(terms), precision=int(precision))
 
 
 def to_struct(p):

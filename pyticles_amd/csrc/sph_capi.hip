@@ -105,7 +105,7 @@ static size_t arena_bytes(size_t cap, int prec, int lcap)
     b += (cap + 63) / 64 * 64 * (size_t)lcap * sizeof(unsigned);  // nbr, in tiles of 64 slots
     b += cap * (3 * d2 + 2 * d1 + 2 * sizeof(double4));  // rho Q jq, T e, S gradv
     b += cap * (2 * d2 + d1);                             // vt, aco, uco
-    if (prec == SPH_FP32C) b += cap * (4 * sizeof(float4) + 3 * sizeof(float2));  // rf S32 vq32 vt32, vf mt32 q32
+    if (prec == SPH_FP32C) b += cap * (2 * sizeof(float4) + sizeof(unsigned) + 2 * sizeof(float2));  // rf S32, rlo, mt32 q32
     b += cap * 5 * sizeof(unsigned);             // sort keys / values
     b += ((cap + 63) / 64 + 1) * i1;             // building tiles
     b += (cap + 63) / 64 * sizeof(double);       // per-wave displacement
@@ -139,9 +139,7 @@ struct Arena {
     double *T = nullptr, *e = nullptr;
     double4 *S = nullptr, *gradv = nullptr;
     float4 *rf = nullptr, *S32 = nullptr;
-    float2 *vf = nullptr;
-    float4 *vt32 = nullptr;
-    float4 *vq32 = nullptr;
+    unsigned *rlo = nullptr;
     float2 *mt32 = nullptr, *q32 = nullptr;
     double2 *vt = nullptr, *aco = nullptr;
     double *uco = nullptr;
@@ -236,7 +234,7 @@ struct Arena {
         A.u_s = u_s; A.m_s = m_s; A.up_s = up_s; A.udot_s = udot_s; A.id_s = id_s;
         A.pcell = pcell; A.r_ref = r_ref; A.slot_box = slot_box; A.nbr = nbr; A.cnt = cnt;
         A.rho = rho; A.T = T; A.S = S; A.Q = Q; A.gradv = gradv; A.jq = jq; A.e = e;
-        A.rf = rf; A.vf = vf; A.S32 = S32; A.vq32 = vq32; A.vt32 = vt32; A.mt32 = mt32; A.q32 = q32;
+        A.rf = rf; A.rlo = rlo; A.S32 = S32; A.mt32 = mt32; A.q32 = q32;
         A.vt = vt; A.aco = aco; A.uco = uco;
         A.cell_start = cell_start;
         A.keys = k_out; A.vals = v_out; A.keys_in = keys_in;
@@ -270,8 +268,7 @@ struct Arena {
         rc |= dalloc(&S, cap); rc |= dalloc(&gradv, cap);
         rc |= dalloc(&vt, cap); rc |= dalloc(&aco, cap); rc |= dalloc(&uco, cap);
         if (prec == SPH_FP32C) {
-            rc |= dalloc(&rf, cap); rc |= dalloc(&S32, cap); rc |= dalloc(&vf, cap);
-            rc |= dalloc(&vq32, cap); rc |= dalloc(&vt32, cap);
+            rc |= dalloc(&rf, cap); rc |= dalloc(&rlo, cap); rc |= dalloc(&S32, cap);
             rc |= dalloc(&mt32, cap); rc |= dalloc(&q32, cap);
         }
         rc |= dalloc(&keys_in, cap); rc |= dalloc(&k_out, cap); rc |= dalloc(&v_out, cap);
@@ -307,7 +304,7 @@ struct Arena {
         dfree(r_s); dfree(v_s); dfree(vp_s); dfree(acc_s); dfree(u_s); dfree(m_s); dfree(up_s); dfree(udot_s);
         dfree(id_s); dfree(pcell); dfree(slot_box); dfree(r_ref); dfree(nbr); dfree(cnt);
         dfree(rho); dfree(Q); dfree(jq); dfree(T); dfree(e); dfree(S); dfree(gradv);
-        dfree(rf); dfree(S32); dfree(vf); dfree(vq32); dfree(vt32); dfree(mt32); dfree(q32);
+        dfree(rf); dfree(rlo); dfree(S32); dfree(mt32); dfree(q32);
         dfree(vt); dfree(aco); dfree(uco);
         dfree(keys_in); dfree(k_out); dfree(v_out); dfree(k_alt); dfree(v_alt);
         dfree(plan_n); dfree(box_coff); dfree(tile_list); dfree(wdisp);
@@ -742,8 +739,6 @@ int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
     if (p->precision != SPH_FP64 && p->precision != SPH_FP32C)
         return fail(SPH_ERR_ARG, "invalid precision");
     if (!(p->skin >= 0) || !std::isfinite(p->skin)) return fail(SPH_ERR_ARG, "invalid skin (need >= 0)");
-    if (!(p->fp32_box_relative_max >= 0) || !std::isfinite(p->fp32_box_relative_max))
-        return fail(SPH_ERR_ARG, "invalid fp32_box_relative_max (need a finite side >= 0)");
     memset(&B, 0, sizeof(B));
     const double PI = 3.14159265358979323846;
     B.Lx = p->Lx; B.Ly = p->Ly; B.hLx = p->Lx / 2.; B.hLy = p->Ly / 2.;
@@ -760,8 +755,7 @@ int fill_desc(const sph_params *p, int64_t n, BoxDesc &B)
     B.rin = 2.0 * (rcut + p->skin);
     // FP32C boxes up to the box-relative limit per side mirror positions as
     // box-relative fixed point (2^32 units per side)
-    B.fixbox = (p->precision == SPH_FP32C && p->Lx <= p->fp32_box_relative_max &&
-                p->Ly <= p->fp32_box_relative_max) ? 1 : 0;
+    B.lolong = (p->precision == SPH_FP32C && (p->Lx > FP32C_HI_ONLY_MAX_L || p->Ly > FP32C_HI_ONLY_MAX_L)) ? 1 : 0;
     // cells cover both the pair-export radius and the list radius
     const double Rc = std::max(std::sqrt(B.R2list), rcut + p->skin) * (1.0 + 1e-9);
     int ncx = (int)std::floor(p->Lx / Rc), ncy = (int)std::floor(p->Ly / Rc);
@@ -1174,6 +1168,27 @@ int sph_separations(sph_box *box, int64_t nip, const int64_t *iap, double *drij,
     if (rsq) HIPCHK(hipMemcpyAsync(rsq, d_rsq, sizeof(double) * nip, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return SPH_OK;
+}
+
+void lucy_kernel_3d(float r, const float *dx, float h, float *w, float *dwdx)
+{
+    const float q = 105.0f / (3.14159265358979323846f * 16.0f * (h * h * h));
+    r = std::fabs(r);
+    if (r > h) {
+        *w = 0.0f;
+        dwdx[0] = dwdx[1] = dwdx[2] = 0.0f;
+        return;
+    }
+    const float t = 1.0f - r / h;
+    *w = q * (1.0f + 3.0f * r / h) * (t * t * t);
+    if (r == 0.0f) {
+        dwdx[0] = dwdx[1] = dwdx[2] = 0.0f;
+        return;
+    }
+    // dW/dr = q (-12 r^3 / h^4 + 24 r^2 / h^3 - 12 r / h^2)
+    const float dwdr = q * ((-12.0f / (h * h * h * h)) * (r * r * r) + (24.0f / (h * h * h)) * (r * r) -
+                            12.0f * r / (h * h));
+    for (int c = 0; c < 3; ++c) dwdx[c] = dwdr * dx[c] / r;
 }
 
 int sph_fp32_accumulate(sph_ctx *ctx, int64_t n, const double *x, int unroll, double *kahan, double *plain)
@@ -1694,6 +1709,21 @@ int sph_box_snapshot(sph_box *box, double *r, double *v, double *u, double *m)
     return SPH_OK;
 }
 
+// Cell grid of the rendering: cells at least one cutoff wide, periods
+// (xmax, ymax) as interpolate.py:79 uses them.
+static int splash_grid(const double *bounds, double cutoff, SplashGrid &G)
+{
+    const double Lx = bounds[1], Ly = bounds[3];
+    if (!(Lx > 0) || !(Ly > 0)) return fail(SPH_ERR_ARG, "bounds need xmax, ymax > 0");
+    G.Lx = Lx; G.Ly = Ly; G.cutoff = cutoff;
+    G.ncx = (int)std::floor(Lx / (cutoff * (1.0 + 1e-9)));
+    G.ncy = (int)std::floor(Ly / (cutoff * (1.0 + 1e-9)));
+    G.ncx = G.ncx < 3 ? 1 : std::min(G.ncx, 4096);
+    G.ncy = G.ncy < 3 ? 1 : std::min(G.ncy, 4096);
+    G.cwx = Lx / G.ncx; G.cwy = Ly / G.ncy;
+    return SPH_OK;
+}
+
 int sph_splash_map_2d(sph_ctx *ctx, int64_t n, const double *r, const double *m, const double *rho,
                       const double *a, const double *h, int nx, const double *x, int ny, const double *y,
                       const double *bounds, double cutoff, double *Z)
@@ -1703,17 +1733,10 @@ int sph_splash_map_2d(sph_ctx *ctx, int64_t n, const double *r, const double *m,
         return fail(SPH_ERR_ARG, "invalid argument");
     if (n > (1ll << 30) || (long long)nx * ny > (1ll << 31))
         return fail(SPH_ERR_ARG, "too many particles or pixels");
-    const double Lx = bounds[1], Ly = bounds[3];  // periods as interpolate.py:79 uses them
-    if (!(Lx > 0) || !(Ly > 0)) return fail(SPH_ERR_ARG, "bounds need xmax, ymax > 0");
+    SplashGrid G;
+    if (int rc = splash_grid(bounds, cutoff, G)) return rc;
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    SplashGrid G;
-    G.Lx = Lx; G.Ly = Ly; G.cutoff = cutoff;
-    G.ncx = (int)std::floor(Lx / (cutoff * (1.0 + 1e-9)));
-    G.ncy = (int)std::floor(Ly / (cutoff * (1.0 + 1e-9)));
-    G.ncx = G.ncx < 3 ? 1 : std::min(G.ncx, 4096);
-    G.ncy = G.ncy < 3 ? 1 : std::min(G.ncy, 4096);
-    G.cwx = Lx / G.ncx; G.cwy = Ly / G.ncy;
     const int ncell = G.ncx * G.ncy;
     const size_t np = (size_t)nx * ny;
     double *d_in = nullptr, *d_xy = nullptr, *d_Z = nullptr;

@@ -17,12 +17,10 @@
 //   derived          rho (double2: rho, rho_lr), T, S (double4: Sxx, Sxy,
 //                    Syy = (p I + Pi)/rho^2, and m), Q (double2: q/rho^2),
 //                    vt (double2: m/rho, T), aco / uco (cohesive rates)
-//   FP32 mirrors     rf: box-relative fixed-point x, y (2^32 units per side)
-//                    and float v, or (boxes wider than the box-relative
-//                    limit) x, y fixed point relative to the build cell
-//                    corner, float m, cell column|row; vf (float2 v),
-//                    S32 (float4: S, m), vq32 (float4: v, q/rho^2),
-//                    vt32 (float4: v, m/rho, T)
+//   FP32 records     rf: the high 32 bits of x, y as 48-bit box-relative
+//                    fixed point (2^48 units per side) and float v; rlo:
+//                    their low 16 bits; S32 (float4: S, m), mt32 (float2:
+//                    m/rho, T), q32 (float2: q/rho^2)
 // A box is a contiguous slot range [part_off, part_off + n) plus a key range
 // [key_base, key_base + ncx*ncy) of cells (key_base + ncx*ncy, the next
 // segment's first key, ends its cell_start range); free slot ranges own one
@@ -43,6 +41,12 @@ enum Health : int {
     HEALTH_LISTCAP = 8,     // a particle had more neighbours than the list holds
 };
 
+// Largest box side whose FP32C long-range walk takes separations from the
+// high 32 bits of the positions alone (resolution side / 2^32, 2.4e-7 at
+// 1024, about one FP32 rounding of a separation); wider boxes read the low
+// bits too, as the short-range terms always do.
+constexpr double FP32C_HI_ONLY_MAX_L = 1024.0;
+
 // Static description of one box (host-owned, uploaded when it changes).
 struct BoxDesc {
     double Lx, Ly, hLx, hLy;       // sides and half sides (minimum image)
@@ -60,7 +64,8 @@ struct BoxDesc {
     int ncx, ncy, sx, sy;          // cell grid and stencil half widths (0/1)
     int key_base, part_off, n, terms;
     int precision;
-    int fixbox;                    // FP32C: box-relative 32-bit fixed-point mirror (see fix_rec)
+    int lolong;                    // FP32C: the long-range walk reads the low position bits too
+                                   // (sides over FP32C_HI_ONLY_MAX_L)
     int pad1, pad2;
 };
 
@@ -116,15 +121,11 @@ struct ArenaView {
     double4 *gradv;
     double2 *jq;
     double *e;
-    // FP32 mirrors
-    float4 *rf;
-    float2 *vf;
-    float4 *S32;
-    float4 *vq32;        // (v, q / rho^2): neighbour record of the rate pass
-    float4 *vt32;        // (v, m / rho, T): the gradient pass's neighbour record
-    // boxes with box-relative records (rf = x, y, v): the velocity comes with
-    // the position, so the other records drop it
-    float2 *mt32;        // (m / rho, T) for the gradient pass
+    // FP32C neighbour records (see fix_rec)
+    float4 *rf;          // high 32 bits of the 48-bit box-relative x, y; predicted v
+    unsigned *rlo;       // low 16 bits of x (bits 0-15) and y (bits 16-31)
+    float4 *S32;         // (Sxx, Sxy, Syy, m): the rate pass's stress record
+    float2 *mt32;        // (m / rho, T): the gradient pass reads m / rho
     float2 *q32;         // q / rho^2 for the rate pass
     // cells and sort
     int *cell_start;

@@ -180,40 +180,28 @@ __device__ __forceinline__ void max_bits(unsigned long long *addr, double x)
     if (b > *(volatile unsigned long long *)addr) atomicMax(addr, b);
 }
 
-// FP32 mirror of a position: fixed point relative to the corner of the
-// particle's build cell, 2^29 units per cell width, plus its mass and the
-// cell's (x, y) index.  Separations taken from two such records are exact
-// integers plus whole cells, so the FP32 separation is the exact one
-// rounded once (FP32 coordinates would round each position first and leave
-// several ulps of the coordinate in every separation).  Offsets are clamped
-// at 1.25 cell widths, far beyond the half skin a particle may move between
-// builds before the skin check trips.  The clamp keeps a difference of two
-// offsets plus one cell width (at most 3.5 * 2^29) inside an int.
-static constexpr double FIX_ONE = 536870912.0;  // 2^29
-static constexpr int FIX_SHIFT = 29;
+// FP32C neighbour record of a particle: its position as 48-bit box-relative
+// fixed point (2^48 units per side), split into the high 32 bits, which go
+// with the predicted velocity into one 16-byte record rf = (X_hi, Y_hi, vx,
+// vy), and the low 16 bits of each axis, packed into rlo.  The 32-bit
+// difference of two X_hi, read as a signed int, is the minimum-image
+// separation in units of side / 2^32 (the FP32 long-range walk's
+// separation: 1.2e-7 at a side of 512, about one FP32 rounding); with the
+// low bits the 48-bit difference is the minimum-image separation of the two
+// recorded positions exactly (units of side / 2^48: 1.8e-12 at 512), which
+// the short-range terms take in FP64.  Coordinates are in [0, L); one that
+// rounds up to L wraps to 0, as the box does.
+static constexpr double FIX48 = 281474976710656.0;  // 2^48
+static constexpr double FIX32 = 4294967296.0;       // 2^32
 
-// FP32 neighbour record of a particle.  In a box with fixbox set the record
-// is (x, y, vx, vy) with x, y box-relative fixed point, 2^32 units per side:
-// the 32-bit difference of two coordinates, read as a signed int, is the
-// minimum-image separation in units exactly, so the record needs no cell
-// and carries the velocity instead (one 16-byte gather per neighbour in the
-// density pass).  Otherwise (larger boxes) it is the build-cell-relative
-// form below: (x, y, m, cell).
-__device__ __forceinline__ float4 fix_rec(double2 r, double2 v, int cell, double m, const BoxDesc &B)
+__device__ __forceinline__ void fix_rec(double2 r, double2 v, const BoxDesc &B, float4 &hi, unsigned &lo)
 {
-    if (B.fixbox) {
-        const double ux = rint(r.x * (4294967296.0 / B.Lx)), uy = rint(r.y * (4294967296.0 / B.Ly));
-        return make_float4(__uint_as_float((unsigned)(unsigned long long)(long long)ux),
-                           __uint_as_float((unsigned)(unsigned long long)(long long)uy), (float)v.x, (float)v.y);
-    }
-    const int cx = cell % B.ncx, cy = cell / B.ncx;
-    double xl = r.x - cx * B.cwx, yl = r.y - cy * B.cwy;
-    xl = xl > B.hLx ? xl - B.Lx : (xl < -B.hLx ? xl + B.Lx : xl);
-    yl = yl > B.hLy ? yl - B.Ly : (yl < -B.hLy ? yl + B.Ly : yl);
-    const double fx = fmin(fmax(xl * B.icwx, -1.25), 1.25) * FIX_ONE;
-    const double fy = fmin(fmax(yl * B.icwy, -1.25), 1.25) * FIX_ONE;
-    return make_float4(__int_as_float((int)rint(fx)), __int_as_float((int)rint(fy)), (float)m,
-                       __int_as_float(cx | (cy << 16)));
+    const unsigned long long M = (1ull << 48) - 1ull;
+    const unsigned long long X = (unsigned long long)fmax(rint(r.x * (FIX48 / B.Lx)), 0.0) & M;
+    const unsigned long long Y = (unsigned long long)fmax(rint(r.y * (FIX48 / B.Ly)), 0.0) & M;
+    hi = make_float4(__uint_as_float((unsigned)(X >> 16)), __uint_as_float((unsigned)(Y >> 16)), (float)v.x,
+                     (float)v.y);
+    lo = (unsigned)(X & 0xffffull) | ((unsigned)(Y & 0xffffull) << 16);
 }
 
 // Wave-uniform values: a value the lanes agree on, moved to a scalar
@@ -742,10 +730,7 @@ __device__ __forceinline__ void build_slot(const ArenaView &A, int s, int force_
     A.id[s] = A.id_s[s];
     A.r_ref[s] = ri;
     const int cell = A.pcell[s];
-    if (fp32) {
-        A.rf[s] = fix_rec(ri, vpi, cell, mi, B);
-        A.vf[s] = make_float2((float)vpi.x, (float)vpi.y);
-    }
+    if (fp32) fix_rec(ri, vpi, B, A.rf[s], A.rlo[s]);
     unsigned *lb = list_base(A, s);
     int ns = 0, nl = 0;
     bool lost = false;
@@ -1193,8 +1178,9 @@ __global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
 }
 
 // ===========================================================================
-// FP32-compensated form: pair arithmetic in FP32 on build-cell-relative
-// coordinates, Kahan-compensated sums, FP64 EOS / integration / energy.
+// FP32-compensated form: FP32 neighbour records (48-bit box-relative fixed
+// point positions), FP32 long-range pair arithmetic with Kahan-compensated
+// sums, FP64 short-range terms, EOS, integration and energy.
 // ===========================================================================
 
 using true_t = std::integral_constant<bool, true>;
@@ -1238,115 +1224,67 @@ __global__ __launch_bounds__(64) void k_fp32_accumulate(long long n, const doubl
 }
 
 struct BoxF {
-    double sclx, scly;  // length of one fixed-point unit
+    double sclx, scly;    // side / 2^32: the unit of the high 32 bits
+    double s48x, s48y;    // side / 2^48: the unit of the full 48 bits
     // FP64 constants of the short-range terms (short density, gradients,
     // rates), which take FP64 arithmetic on exact separations
     double h2d, cutsd, ihd, ih3d, fhd;
-    float cut2, cutl, h2, cuts;
-    float ih, ihl, qh, qhl, fh, fhl, munif;
-    float ih3, ihl3;  // 3/h, 3/h_l: 1 + 3q = fma(r, 3/h, 1)
-    int ncx, ncy, sx, sy, terms, fixbox;
+    float cut2, cutl;
+    float ihl, qhl, fhl, munif;
+    float ihl3;  // 3/h_l: 1 + 3q = fma(r, 3/h_l, 1)
+    int terms;
 };
 
 __device__ __forceinline__ BoxF boxf(const BoxDesc &B, bool uniform)
 {
     BoxF f;
-    if (B.fixbox) {
-        f.sclx = B.Lx / 4294967296.0;
-        f.scly = B.Ly / 4294967296.0;
-    } else {
-        f.sclx = B.cwx / FIX_ONE;
-        f.scly = B.cwy / FIX_ONE;
-    }
-    f.munif = (float)B.munif;
-    f.fixbox = B.fixbox;
+    f.sclx = B.Lx / FIX32;
+    f.scly = B.Ly / FIX32;
+    f.s48x = B.Lx / FIX48;
+    f.s48y = B.Ly / FIX48;
     f.h2d = B.h2;
     f.cutsd = B.R2list < B.h2 ? B.R2list : B.h2;
     f.ihd = B.ih;
     f.ih3d = 3.0 * B.ih;
     f.fhd = B.fh;
-    f.cut2 = (float)B.cut2; f.cutl = (float)(B.cut2 < B.hl2 ? B.cut2 : B.hl2); f.h2 = (float)B.h2;
-    f.cuts = (float)(B.R2list < B.h2 ? B.R2list : B.h2);
-    f.ih = (float)B.ih; f.ihl = (float)B.ihl; f.qh = (float)B.qh; f.qhl = (float)B.qhl;
-    f.fh = (float)B.fh; f.fhl = (float)B.fhl;
-    f.ih3 = (float)(3.0 * B.ih); f.ihl3 = (float)(3.0 * B.ihl);
-    f.ncx = B.ncx; f.ncy = B.ncy; f.sx = B.sx; f.sy = B.sy; f.terms = B.terms;
+    f.munif = (float)B.munif;
+    f.cut2 = (float)B.cut2;
+    f.cutl = (float)(B.cut2 < B.hl2 ? B.cut2 : B.hl2);
+    f.ihl = (float)B.ihl; f.qhl = (float)B.qhl; f.fhl = (float)B.fhl;
+    f.ihl3 = (float)(3.0 * B.ihl);
+    f.terms = B.terms;
     if (uniform) {
         f.sclx = wave_uniform(f.sclx); f.scly = wave_uniform(f.scly);
+        f.s48x = wave_uniform(f.s48x); f.s48y = wave_uniform(f.s48y);
         f.h2d = wave_uniform(f.h2d); f.cutsd = wave_uniform(f.cutsd); f.ihd = wave_uniform(f.ihd);
         f.ih3d = wave_uniform(f.ih3d); f.fhd = wave_uniform(f.fhd);
-        f.cut2 = wave_uniform(f.cut2); f.cutl = wave_uniform(f.cutl); f.h2 = wave_uniform(f.h2);
-        f.cuts = wave_uniform(f.cuts); f.ih = wave_uniform(f.ih); f.ihl = wave_uniform(f.ihl);
-        f.qh = wave_uniform(f.qh); f.qhl = wave_uniform(f.qhl); f.fh = wave_uniform(f.fh); f.fhl = wave_uniform(f.fhl);
-        f.ih3 = wave_uniform(f.ih3); f.ihl3 = wave_uniform(f.ihl3);
-        f.ncx = wave_uniform(f.ncx); f.ncy = wave_uniform(f.ncy); f.sx = wave_uniform(f.sx); f.sy = wave_uniform(f.sy);
+        f.cut2 = wave_uniform(f.cut2); f.cutl = wave_uniform(f.cutl);
+        f.ihl = wave_uniform(f.ihl); f.qhl = wave_uniform(f.qhl); f.fhl = wave_uniform(f.fhl);
+        f.ihl3 = wave_uniform(f.ihl3); f.munif = wave_uniform(f.munif);
         f.terms = wave_uniform(f.terms);
-        f.munif = wave_uniform(f.munif); f.fixbox = wave_uniform(f.fixbox);
     }
     return f;
 }
 
-// Separation of rj from ri (fixed-point records): the integer difference
-// plus the cell offset (with the periodic wrap of the cell index) is exact in
-// an int, and so is the minimum image of an axis with a single cell (period
-// 2^29 units); one FP64 scale gives the separation of the two recorded
-// positions to FP64 precision (sep32 rounds it once more to FP32).
-// WRAP: 0 = interior wave (no wrap), 1 = wrap of cell offsets, 2 = also the
-// fixed-point wrap of single-cell axes, 3 = box-relative records (the 32-bit
-// difference is the minimum image).
-template <int WRAP>
-__device__ __forceinline__ void sep64(const float4 &ri, const float4 &rj, const BoxF &F, double &dx, double &dy)
+// Minimum-image separation of record j from record i (fix_rec): the signed
+// 32-bit difference of the high words (wrapping at the box side), with LO
+// also the low 16 bits, which makes it the exact 48-bit difference of the
+// two recorded positions; then one FP64 scale.
+template <bool LO>
+__device__ __forceinline__ void sep_fix(const float4 &ri, unsigned loi, const float4 &rj, unsigned loj, const BoxF &F,
+                                        double &dx, double &dy)
 {
-    if constexpr (WRAP == 3) {
-        const int ix = (int)(__float_as_uint(rj.x) - __float_as_uint(ri.x));
-        const int iy = (int)(__float_as_uint(rj.y) - __float_as_uint(ri.y));
-        dx = (double)ix * F.sclx;
-        dy = (double)iy * F.scly;
-        return;
+    const int hx = (int)(__float_as_uint(rj.x) - __float_as_uint(ri.x));
+    const int hy = (int)(__float_as_uint(rj.y) - __float_as_uint(ri.y));
+    if constexpr (LO) {
+        const long long ex = (long long)hx * 65536 + ((int)(loj & 0xffffu) - (int)(loi & 0xffffu));
+        const long long ey = (long long)hy * 65536 + ((int)(loj >> 16) - (int)(loi >> 16));
+        dx = (double)ex * F.s48x;
+        dy = (double)ey * F.s48y;
+    } else {
+        dx = (double)hx * F.sclx;
+        dy = (double)hy * F.scly;
     }
-    const int ci = __float_as_int(ri.w), cj = __float_as_int(rj.w);
-    int ox = (cj & 0xffff) - (ci & 0xffff), oy = (cj >> 16) - (ci >> 16);
-    if (WRAP) {
-        ox = ox > 1 ? ox - F.ncx : (ox < -1 ? ox + F.ncx : ox);
-        oy = oy > 1 ? oy - F.ncy : (oy < -1 ? oy + F.ncy : oy);
-    }
-    int ix = (__float_as_int(rj.x) - __float_as_int(ri.x)) + ox * (1 << FIX_SHIFT);
-    int iy = (__float_as_int(rj.y) - __float_as_int(ri.y)) + oy * (1 << FIX_SHIFT);
-    if (WRAP == 2 && F.sx == 0) {
-        ix = ix > (1 << (FIX_SHIFT - 1)) ? ix - (1 << FIX_SHIFT) : ix;
-        ix = ix < -(1 << (FIX_SHIFT - 1)) ? ix + (1 << FIX_SHIFT) : ix;
-    }
-    if (WRAP == 2 && F.sy == 0) {
-        iy = iy > (1 << (FIX_SHIFT - 1)) ? iy - (1 << FIX_SHIFT) : iy;
-        iy = iy < -(1 << (FIX_SHIFT - 1)) ? iy + (1 << FIX_SHIFT) : iy;
-    }
-    dx = (double)ix * F.sclx;
-    dy = (double)iy * F.scly;
-}
-
-// The same separation rounded once to FP32 (the long-range walk).
-template <int WRAP>
-__device__ __forceinline__ void sep32(const float4 &ri, const float4 &rj, const BoxF &F, float &dx, float &dy)
-{
-    double x, y;
-    sep64<WRAP>(ri, rj, F, x, y);
-    dx = (float)x;
-    dy = (float)y;
-}
-
-// Wrap mode of sep32 for the active lanes: 3 for box-relative records, 0
-// when every lane's build cell
-// is at least one cell from each side (the stencil of such a cell needs no
-// wrap, so every listed neighbour's cell offset is already -1, 0 or 1 and
-// the separations are the same as with the wrap), 2 for a box with a
-// single-cell axis, else 1.
-__device__ __forceinline__ int wave_wrap32(const float4 &ri, const BoxF &F)
-{
-    if (F.fixbox) return 3;
-    const int c = __float_as_int(ri.w), cx = c & 0xffff, cy = c >> 16;
-    const bool in = cx >= 1 && cx <= F.ncx - 2 && cy >= 1 && cy <= F.ncy - 2;
-    if (__ballot(!in) == 0ull) return 0;
-    return (F.sx == 0 || F.sy == 0) ? 2 : 1;
 }
 
 struct Dens32 {
@@ -1358,14 +1296,18 @@ struct Dens32 {
 // (v_sqrt_f32, about 1 ulp; the correctly rounded sequence costs ~14 more
 // instructions per entry and moved no error against the FP64 oracle).
 
-// FP32 sums: the terms of one unrolled round (U32 list entries) are added in
-// FP32, and each round's partial sum, times the kernel constant it leaves
-// out, goes into a Kahan-compensated total.  The short-range density (r < h:
-// the density the EOS turns into T and p, where a relaxed liquid amplifies
-// any error about tenfold) takes FP64 terms on the exact separations of the
-// same records; its constants scale the FP64 total at the end.
-// UNIM: all masses equal, none fetched (the records' mass is not read).
-template <int U32, bool COH, int WRAP, bool UNIM>
+// P1 walk of the FP32C form.  The long-range terms (rho_lr and the
+// cohesive rates, ~80% of the entries) are FP32: the terms of one unrolled
+// round (U32 list entries) add in FP32 and each round's partial sum, times
+// the kernel constant it leaves out, goes into a Kahan-compensated total.
+// The short-range density (r < h: the density the EOS turns into T and p,
+// where a relaxed liquid amplifies any error about tenfold) takes FP64 terms
+// on the exact separations of the same records; its constants scale the
+// FP64 total at the end.  A short entry gathers rf and rlo (20 bytes), a
+// long entry rf alone (16), or rf and rlo in boxes wider than the side
+// up to which the high words keep the long-range terms within FP32 rounding
+// (LOLONG).  UNIM: all masses equal, none fetched.
+template <int U32, bool COH, bool UNIM, bool LOLONG>
 __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F, int s, Dens32 &d)
 {
     // the kernel constants (and, with UNIM, the mass) scale each round's
@@ -1373,43 +1315,40 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
     const float cqhl = UNIM ? F.qhl * F.munif : F.qhl;
     const float cfhl = UNIM ? F.fhl * F.munif : F.fhl;
     const float4 ri = A.rf[s];
-    const float2 vi = A.vf[s];
+    const unsigned loi = A.rlo[s];
     const int2 c = A.cnt[s];
     const unsigned *nb = list_base(A, s);
     ListRounds<U32, false, true> rounds(nb, A.lcap, c.x, s);
     for (int k0 = 0; k0 < c.x; k0 += U32) {
         int t[U32];
         float4 rj[U32];
-        float2 vv[U32];
+        unsigned lj[U32];
         float mm[U32];
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
-            if (COH && WRAP != 3) vv[u] = A.vf[t[u]];
-            if (WRAP == 3 && !UNIM) mm[u] = (float)A.m[t[u]];
+            lj[u] = A.rlo[t[u]];
+            if (!UNIM) mm[u] = (float)A.m[t[u]];
         }
         rounds.next(k0);
         float grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             double dxd, dyd;
-            sep64<WRAP>(ri, rj[u], F, dxd, dyd);
+            sep_fix<true>(ri, loi, rj[u], lj[u], F, dxd, dyd);
             const float dx = (float)dxd, dy = (float)dyd;
             const float r2 = dx * dx + dy * dy;
-            const float mj = UNIM ? 1.f : (WRAP == 3 ? mm[u] : rj[u].z);
-            const float vjx = WRAP == 3 ? rj[u].z : vv[u].x, vjy = WRAP == 3 ? rj[u].w : vv[u].y;
-            if (k0 + u < c.x && r2 < F.cut2) {
+            const float mj = UNIM ? 1.f : mm[u];
+            if (k0 + u < c.x && r2 < F.cutl) {
                 const float rr = __builtin_amdgcn_sqrtf(r2);
-                if (r2 < F.cutl) {
-                    const float w = fmaf(-rr, F.ihl, 1.f), w2 = w * w;
-                    grhol = fmaf(mj * fmaf(rr, F.ihl3, 1.f), w2 * w, grhol);
-                    if (COH) {
-                        const float g = mj * w2;
-                        ggx = fmaf(g, dx, ggx);
-                        ggy = fmaf(g, dy, ggy);
-                        gev = fmaf(g, (vjx - vi.x) * dx + (vjy - vi.y) * dy, gev);
-                    }
+                const float w = fmaf(-rr, F.ihl, 1.f), w2 = w * w;
+                grhol = fmaf(mj * fmaf(rr, F.ihl3, 1.f), w2 * w, grhol);
+                if (COH) {
+                    const float g = mj * w2;
+                    ggx = fmaf(g, dx, ggx);
+                    ggy = fmaf(g, dy, ggy);
+                    gev = fmaf(g, (rj[u].z - ri.z) * dx + (rj[u].w - ri.w) * dy, gev);
                 }
             }
             const double r2d = rsq_exact(dxd, dyd);
@@ -1431,24 +1370,24 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
     for (int k0 = 0; k0 < c.y; k0 += U32) {
         int t[U32];
         float4 rj[U32];
-        float2 vv[U32];
+        unsigned lj[U32];
         float mm[U32];
         lrounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
-            if (COH && WRAP != 3) vv[u] = A.vf[t[u]];
-            if (WRAP == 3 && !UNIM) mm[u] = (float)A.m[t[u]];
+            if (LOLONG) lj[u] = A.rlo[t[u]];
+            if (!UNIM) mm[u] = (float)A.m[t[u]];
         }
         lrounds.next(k0);
         float grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
-            float dx, dy;
-            sep32<WRAP>(ri, rj[u], F, dx, dy);
+            double dxd, dyd;
+            sep_fix<LOLONG>(ri, loi, rj[u], LOLONG ? lj[u] : 0u, F, dxd, dyd);
+            const float dx = (float)dxd, dy = (float)dyd;
             const float r2 = dx * dx + dy * dy;
-            const float mj = UNIM ? 1.f : (WRAP == 3 ? mm[u] : rj[u].z);
-            const float vjx = WRAP == 3 ? rj[u].z : vv[u].x, vjy = WRAP == 3 ? rj[u].w : vv[u].y;
+            const float mj = UNIM ? 1.f : mm[u];
             if (k0 + u < c.y && r2 < F.cutl) {
                 const float rr = __builtin_amdgcn_sqrtf(r2);
                 const float w = fmaf(-rr, F.ihl, 1.f), w2 = w * w;
@@ -1457,7 +1396,7 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
                     const float g = mj * w2;
                     ggx = fmaf(g, dx, ggx);
                     ggy = fmaf(g, dy, ggy);
-                    gev = fmaf(g, (vjx - vi.x) * dx + (vjy - vi.y) * dy, gev);
+                    gev = fmaf(g, (rj[u].z - ri.z) * dx + (rj[u].w - ri.w) * dy, gev);
                 }
             }
         }
@@ -1470,7 +1409,7 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
     }
 }
 
-// P1 in FP32: rf carries position and mass, vf the predicted velocity; the
+// P1 of the FP32C form: rf carries position and predicted velocity; the
 // cohesive rate part (see k_density64) is accumulated with the densities.
 template <int U32>
 __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
@@ -1485,16 +1424,14 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     const bool coh = (B.terms & 2u) != 0;
     Dens32 d;
     per_box(b, [&](int bu) {
-        const BoxF F = boxf(A.boxes[bu], true);
+        const BoxDesc &Bu = A.boxes[bu];
+        const BoxF F = boxf(Bu, true);
         d.rho = 0.0;
         d.rhol = Kahan(F.qhl);
-        const int wrap = wave_wrap32(A.rf[s], F);
         auto walk = [&](auto coh_t, auto unim_t) {
             constexpr bool C = decltype(coh_t)::value, M = decltype(unim_t)::value;
-            if (wrap == 3) density32_walk<U32, C, 3, M>(A, F, s, d);
-            else if (wrap == 0) density32_walk<U32, C, 0, M>(A, F, s, d);
-            else if (wrap == 1) density32_walk<U32, C, 1, M>(A, F, s, d);
-            else density32_walk<U32, C, 2, M>(A, F, s, d);
+            if (wave_uniform(Bu.lolong)) density32_walk<U32, C, M, true>(A, F, s, d);
+            else density32_walk<U32, C, M, false>(A, F, s, d);
         };
         const bool unim = F.munif > 0.f;
         if (F.terms & 2) {
@@ -1505,7 +1442,6 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
             else walk(false_t{}, false_t{});
         }
     });
-    const float2 vi = A.vf[s];
     // the short-range sum leaves out q_h (and the common mass)
     const double rd = fma(B.qh * (B.munif > 0.0 ? B.munif : 1.0), d.rho, B.qh), rl = d.rhol.value();
     const double mi = A.m[s];
@@ -1521,19 +1457,14 @@ __global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
     A.rho[s] = make_double2(rd, rl);
     A.T[s] = T;
     // records the gradient pass rewrites are written here only when it will
-    // not run (stress without viscosity; the rate pass's v / q record
-    // without viscosity and conduction)
+    // not run (stress without viscosity; q/rho^2 without viscosity and
+    // conduction)
     if (!(B.terms & 4u)) {
         const double ps = (B.terms & 1u) ? p / (rd * rd) : 0.0;
         A.S32[s] = make_float4((float)ps, 0.f, (float)ps, (float)mi);
     }
-    if (B.fixbox) {  // rf carries v
-        if (!(B.terms & 12u)) A.q32[s] = make_float2(0.f, 0.f);
-        A.mt32[s] = make_float2((float)(mi / rd), (float)T);
-    } else {
-        if (!(B.terms & 12u)) A.vq32[s] = make_float4(vi.x, vi.y, 0.f, 0.f);
-        A.vt32[s] = make_float4(vi.x, vi.y, (float)(mi / rd), (float)T);
-    }
+    if (!(B.terms & 12u)) A.q32[s] = make_float2(0.f, 0.f);
+    A.mt32[s] = make_float2((float)(mi / rd), (float)T);
     A.aco[s] = coh ? make_double2(-2.0 * B.a * d.gx.value(), -2.0 * B.a * d.gy.value()) : make_double2(0.0, 0.0);
     A.uco[s] = coh ? -B.a * d.ev.value() : 0.0;
     if (!(rd * B.b < 1.0)) flag_health(A.dyn, b, HEALTH_EOS_POLE);
@@ -1545,27 +1476,18 @@ struct Grad32 {
 };
 
 // P2 walk of the FP32C form: FP32 neighbour records, FP64 arithmetic on the
-// exact separations of the fixed-point positions.  On CDNA4 an FP64 FMA
-// issues at the FP32 rate, so the short-range terms keep their precision at
-// no VALU premium beyond the conversions; FP32 rounding of these terms (and
-// of the temperature difference) left up to 4e-6 of force and heating error
-// on relaxed liquid states, where the terms cancel to a small net rate.
-// The temperature comes from the FP64 T of the density pass.
-template <int U32, int WRAP>
+// exact separations.  On CDNA4 an FP64 FMA issues at the FP32 rate, so the
+// short-range terms keep their precision at no VALU premium beyond the
+// conversions; FP32 rounding of these terms (and of the temperature
+// difference) left up to 4e-6 of force and heating error on relaxed liquid
+// states, where the terms cancel to a small net rate.  The temperature comes
+// from the FP64 T of the density pass.  Per neighbour: rf, rlo, m/rho, T.
+template <int U32>
 __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, int s, Grad32 &g)
 {
-    // WRAP 3 (box-relative records): v from rf and m/rho from mt32;
-    // otherwise v and m/rho from vt32
     const float4 ri = A.rf[s];
-    double vix, viy;
-    if (WRAP == 3) {
-        vix = ri.z;
-        viy = ri.w;
-    } else {
-        const float4 vti = A.vt32[s];
-        vix = vti.x;
-        viy = vti.y;
-    }
+    const unsigned loi = A.rlo[s];
+    const double vix = ri.z, viy = ri.w;
     const double Ti = A.T[s];
     const int ns = A.cnt[s].x;
     const unsigned *nb = list_base(A, s);
@@ -1573,32 +1495,29 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32];
-        float3 vt[U32];  // v, m/rho
+        unsigned lj[U32];
+        float Vj[U32];
         double Tj[U32];
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
-            if (WRAP == 3) {
-                vt[u] = make_float3(rj[u].z, rj[u].w, A.mt32[t[u]].x);
-            } else {
-                const float4 q = A.vt32[t[u]];
-                vt[u] = make_float3(q.x, q.y, q.z);
-            }
+            lj[u] = A.rlo[t[u]];
+            Vj[u] = A.mt32[t[u]].x;
             Tj[u] = A.T[t[u]];
         }
         rounds.next(k0);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             double dx, dy;
-            sep64<WRAP>(ri, rj[u], F, dx, dy);
+            sep_fix<true>(ri, loi, rj[u], lj[u], F, dx, dy);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < ns && r2 < F.cutsd) {
                 const double rr = sqrt_pair(r2), w = fma(-rr, F.ihd, 1.0);
                 const double Fk = F.fhd * w * w;
                 const double gx = -Fk * dx, gy = -Fk * dy;
-                const double vj = vt[u].z;
-                const double dvx = (double)vt[u].x - vix, dvy = (double)vt[u].y - viy;
+                const double vj = Vj[u];
+                const double dvx = (double)rj[u].z - vix, dvy = (double)rj[u].w - viy;
                 const double dT = Tj[u] - Ti;
                 g.g00 += vj * dvx * gx;
                 g.g01 += vj * dvx * gy;
@@ -1611,9 +1530,8 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
     }
 }
 
-// P2 of the FP32C form; neighbour record: rf, (v, m/rho) from P1 and the
-// FP64 T, or rf, m/rho and T for box-relative records.  Writes the rate
-// pass's (v, q/rho^2) record (q/rho^2 alone for box-relative records).
+// P2 of the FP32C form.  Writes the stress record (S32, with viscosity) and
+// the rate pass's q/rho^2 record.
 template <int U32>
 __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int diag)
 {
@@ -1627,13 +1545,8 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
     Grad32 g = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
-        const int wrap = wave_wrap32(A.rf[s], F);
-        if (wrap == 3) grad32_walk<U32, 3>(A, F, s, g);
-        else if (wrap == 0) grad32_walk<U32, 0>(A, F, s, g);
-        else if (wrap == 1) grad32_walk<U32, 1>(A, F, s, g);
-        else grad32_walk<U32, 2>(A, F, s, g);
+        grad32_walk<U32>(A, F, s, g);
     });
-    const float2 vi = A.vf[s];
     const double G00 = g.g00, G01 = g.g01, G10 = g.g10, G11 = g.g11;
     const double rho = A.rho[s].x, r2i = rho * rho;
     double qx = 0, qy = 0;
@@ -1650,8 +1563,7 @@ __global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int 
         qx = -B.kcond * g.gt0;
         qy = -B.kcond * g.gt1;
     }
-    if (B.fixbox) A.q32[s] = make_float2((float)(qx / r2i), (float)(qy / r2i));
-    else A.vq32[s] = make_float4(vi.x, vi.y, (float)(qx / r2i), (float)(qy / r2i));
+    A.q32[s] = make_float2((float)(qx / r2i), (float)(qy / r2i));
     if (diag) {
         A.gradv[s] = make_double4(G00, G01, G10, G11);
         A.jq[s] = make_double2(qx, qy);
@@ -1663,43 +1575,35 @@ struct Rate32 {
 };
 
 // P3 walk of the FP32C form: FP32 neighbour records, FP64 arithmetic on the
-// exact separations (see grad32_walk).
-template <int U32, bool COND, int WRAP>
+// exact separations (see grad32_walk).  Per neighbour: rf, rlo, (S, m) and,
+// with conduction, q/rho^2.
+template <int U32, bool COND>
 __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, int s, int ns, Rate32 &o)
 {
-    // WRAP 3 (box-relative records): v from rf and q/rho^2 from the 8-byte
-    // q32, fetched only with conduction
     const float4 ri = A.rf[s];
-    float4 vqi;
-    if (WRAP == 3) {
-        const float2 qi = COND ? A.q32[s] : make_float2(0.f, 0.f);
-        vqi = make_float4(ri.z, ri.w, qi.x, qi.y);
-    } else {
-        vqi = A.vq32[s];
-    }
+    const unsigned loi = A.rlo[s];
+    const float2 qi = COND ? A.q32[s] : make_float2(0.f, 0.f);
     const float4 Si = A.S32[s];
     const unsigned *nb = list_base(A, s);
     ListRounds<U32, false, false> rounds(nb, A.lcap, ns, s);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
-        float4 rj[U32], Sj[U32], vq[U32];
+        float4 rj[U32], Sj[U32];
+        unsigned lj[U32];
+        float2 qj[U32];
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = A.rf[t[u]];
+            lj[u] = A.rlo[t[u]];
             Sj[u] = A.S32[t[u]];
-            if (WRAP == 3) {
-                const float2 qj = COND ? A.q32[t[u]] : make_float2(0.f, 0.f);
-                vq[u] = make_float4(rj[u].z, rj[u].w, qj.x, qj.y);
-            } else {
-                vq[u] = A.vq32[t[u]];
-            }
+            qj[u] = COND ? A.q32[t[u]] : make_float2(0.f, 0.f);
         }
         rounds.next(k0);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             double dx, dy;
-            sep64<WRAP>(ri, rj[u], F, dx, dy);
+            sep_fix<true>(ri, loi, rj[u], lj[u], F, dx, dy);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < ns && r2 < F.cutsd) {
                 const double rr = sqrt_pair(r2), w = fma(-rr, F.ihd, 1.0);
@@ -1709,10 +1613,10 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
                 const double ayy = (double)Si.z + (double)Sj[u].z;
                 const double fx = axx * wx + axy * wy;
                 const double fy = axy * wx + ayy * wy;
-                const double cc = COND ? ((double)vqi.z + (double)vq[u].z) * wx + ((double)vqi.w + (double)vq[u].w) * wy
+                const double cc = COND ? ((double)qi.x + (double)qj[u].x) * wx + ((double)qi.y + (double)qj[u].y) * wy
                                        : 0.0;
                 const double mj = Sj[u].w;
-                const double e = 0.5 * (((double)vq[u].x - (double)vqi.x) * fx + ((double)vq[u].y - (double)vqi.y) * fy);
+                const double e = 0.5 * (((double)rj[u].z - (double)ri.z) * fx + ((double)rj[u].w - (double)ri.w) * fy);
                 o.ax += mj * fx;
                 o.ay += mj * fy;
                 o.ud += mj * (e + cc);
@@ -1721,9 +1625,7 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
     }
 }
 
-// P3 of the FP32C form over the short entries; neighbour record: rf,
-// (v, q/rho^2), (S, m); for box-relative records rf, (S, m) and, with
-// conduction, q/rho^2.  Adds the cohesive part from P1.
+// P3 of the FP32C form over the short entries; adds the cohesive part from P1.
 template <int U32>
 __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
 {
@@ -1737,18 +1639,8 @@ __global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
     Rate32 o = {0.0, 0.0, 0.0};
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
-        const int wrap = wave_wrap32(A.rf[s], F);
-        if (F.terms & 8) {
-            if (wrap == 3) rates32_walk<U32, true, 3>(A, F, s, ns, o);
-            else if (wrap == 0) rates32_walk<U32, true, 0>(A, F, s, ns, o);
-            else if (wrap == 1) rates32_walk<U32, true, 1>(A, F, s, ns, o);
-            else rates32_walk<U32, true, 2>(A, F, s, ns, o);
-        } else {
-            if (wrap == 3) rates32_walk<U32, false, 3>(A, F, s, ns, o);
-            else if (wrap == 0) rates32_walk<U32, false, 0>(A, F, s, ns, o);
-            else if (wrap == 1) rates32_walk<U32, false, 1>(A, F, s, ns, o);
-            else rates32_walk<U32, false, 2>(A, F, s, ns, o);
-        }
+        if (F.terms & 8) rates32_walk<U32, true>(A, F, s, ns, o);
+        else rates32_walk<U32, false>(A, F, s, ns, o);
     });
     const double2 ac = A.aco[s];
     const double axd = o.ax + ac.x, ayd = o.ay + ac.y, udd = o.ud + A.uco[s];
@@ -1786,10 +1678,7 @@ __device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, i
         A.e[s] = m * (0.5 * (vx * vx + vy * vy) + un);
         A.vp[s] = make_double2(vx, vy);
         A.up[s] = un;
-        if (fp32) {
-            A.vf[s] = make_float2((float)vx, (float)vy);
-            if (B.fixbox) A.rf[s] = fix_rec(A.r[s], make_double2(vx, vy), A.pcell[s], m, B);
-        }
+        if (fp32) fix_rec(A.r[s], make_double2(vx, vy), B, A.rf[s], A.rlo[s]);
         return -1.0;
     }
     const double2 r = A.r[s];
@@ -1809,10 +1698,7 @@ __device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, i
     min_image(ddx, ddy, B);
     const double d2 = ddx * ddx + ddy * ddy;
     if (d2 > B.hskin2) flag_health(A.dyn, b, HEALTH_SKIN);
-    if (fp32) {
-        A.rf[s] = fix_rec(rn, vp, A.pcell[s], m, B);
-        A.vf[s] = make_float2((float)vp.x, (float)vp.y);
-    }
+    if (fp32) fix_rec(rn, vp, B, A.rf[s], A.rlo[s]);
     return d2;
 }
 

@@ -33,11 +33,18 @@ def test_version_and_error_strings_without_gpu():
 
 
 def test_params_struct_layout():
-    # 14 doubles + unsigned + int + the FP32C record limit, no padding surprises
-    assert ctypes.sizeof(_lib.Params) == 15 * 8 + 8
-    assert _lib.Params.fp32_box_relative_max.offset == 14 * 8 + 8
+    # 14 doubles + unsigned + int, no padding surprises
+    assert ctypes.sizeof(_lib.Params) == 14 * 8 + 8
 
 
 def test_library_is_gfx950_code_object():
     data = open(_lib.LIB_PATH, "rb").read()
     assert b"gfx950" in data
+
+
+def test_reference_c_level_lucy_kernel_exported():
+    # c_properties.pxd:1: void lucy_kernel_3d(float r, float dx[3], float h, float* w, float dwdx[3])
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    assert hasattr(lib, "lucy_kernel_3d")
+    src = open(os.path.join(ROOT, "include", "pyticles_hip.h")).read()
+    assert "void lucy_kernel_3d(float r, const float *dx, float h, float *w, float *dwdx);" in src

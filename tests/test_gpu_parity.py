@@ -295,17 +295,9 @@ def test_positions_outside_box_rejected():
 
 
 # ------------------------------------------------------------------ FP32-compensated form
-@pytest.fixture(params=["box_relative", "cell_relative"])
-def fp32_mirror(request):
-    """Both FP32C position mirrors: box-relative fixed point (a box's
-    fp32_box_relative_max at least its side) and build-cell-relative (0, the
-    default); the value is the parameter that selects it."""
-    return 1e9 if request.param == "box_relative" else 0.0
-
-
-def test_fp32_compensated_error_vs_fp64(fp32_mirror):
+def test_fp32_compensated_error_vs_fp64():
     r, v, m, prm = _system(64, seed=12)
-    prm32 = dict(prm, precision=_lib.FP32C, fp32_box_relative_max=fp32_mirror)
+    prm32 = dict(prm, precision=_lib.FP32C)
     u = np.full(r.shape[0], -1.1)
     ref = O.derivatives(r, v, m, u, prm, terms=15)
     with _box(r, v, m, prm32, u=u) as b:
@@ -315,7 +307,7 @@ def test_fp32_compensated_error_vs_fp64(fp32_mirror):
     assert rel_norm(got["rho_lr"], ref["rho_lr"]) < 1e-6
     assert rel_norm(got["acc"], ref["acc"]) < 1e-6
     assert rel_norm(got["udot"], ref["udot"]) < 1e-6
-    print("fp32c (%s) errors rho %.2e rho_lr %.2e acc %.2e udot %.2e" % (fp32_mirror,
+    print("fp32c errors rho %.2e rho_lr %.2e acc %.2e udot %.2e" % (
         rel_norm(got["rho"], ref["rho"]), rel_norm(got["rho_lr"], ref["rho_lr"]),
         rel_norm(got["acc"], ref["acc"]), rel_norm(got["udot"], ref["udot"])))
 
@@ -337,7 +329,7 @@ def _rect_system(nx, ny, Lx, Ly, seed):
 
 @pytest.mark.parametrize("nx,ny,Lx,Ly", [(40, 13, 40.0, 13.0), (64, 8, 64.0, 8.0), (6, 50, 6.0, 50.0),
                                          (30, 30, 30.0, 30.5)])
-def test_rectangular_boxes(nx, ny, Lx, Ly, fp32_mirror):
+def test_rectangular_boxes(nx, ny, Lx, Ly):
     # elongated boxes, and boxes with a single cell along one axis (minimum
     # image instead of cell offsets, in the FP64 build and the FP32 mirrors)
     r, v, m, u = _rect_system(nx, ny, Lx, Ly, seed=nx * ny)
@@ -350,7 +342,7 @@ def test_rectangular_boxes(nx, ny, Lx, Ly, fp32_mirror):
         got = b.download("rho", "acc", "udot")
     assert rel_norm(got["rho"], ref["rho"]) < 1e-13
     assert rel_norm(got["acc"], ref["acc"]) < TOL64 and rel_norm(got["udot"], ref["udot"]) < TOL64
-    with _box(r, v, m, dict(prm, precision=_lib.FP32C, fp32_box_relative_max=fp32_mirror), u=u) as b:
+    with _box(r, v, m, dict(prm, precision=_lib.FP32C), u=u) as b:
         b.rates(15)
         got = b.download("rho", "acc")
     assert rel_norm(got["rho"], ref["rho"]) < 1e-6 and rel_norm(got["acc"], ref["acc"]) < 2e-6
@@ -394,3 +386,20 @@ def test_reused_lists_stay_complete_over_long_runs(precision):
     tol = 1e-13 if precision == _lib.FP64 else 1e-6
     assert rel_norm(got["rho"], fresh["rho"]) < tol
     assert rel_norm(got["rho_lr"], fresh["rho_lr"]) < tol
+
+
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32C])
+def test_energy_totals_at_a_fixed_state(precision):
+    # sph_energy at an uploaded state (no step): sum m v^2 / 2, sum m u and
+    # their sum against exactly rounded host sums, within 1e-12 (the totals
+    # are FP64 in both forms)
+    r, v, m, prm = _system(40, seed=23)
+    m = np.random.Generator(np.random.PCG64(24)).uniform(0.8, 1.2, size=r.shape[0])
+    u = np.random.Generator(np.random.PCG64(25)).uniform(-1.5, -0.8, size=r.shape[0])
+    with _box(r, v, m, dict(prm, precision=precision), u=u) as b:
+        ke, ue, et = b.energy()
+    ke0 = math.fsum(0.5 * m * (v[:, 0] ** 2 + v[:, 1] ** 2))
+    ue0 = math.fsum(m * u)
+    assert abs(ke - ke0) <= 1e-12 * abs(ke0)
+    assert abs(ue - ue0) <= 1e-12 * abs(ue0)
+    assert abs(et - (ke0 + ue0)) <= 1e-12 * (abs(ke0) + abs(ue0))
