@@ -268,6 +268,11 @@ int sph_box_set_schedule(sph_box *box, int unroll);
  * Tuned per workload class.  No counterpart in the reference. */
 int sph_arena_set_rates_unroll(sph_arena *ar, int unroll);
 int sph_box_set_rates_unroll(sph_box *box, int unroll);
+/* Work-items per block of the three pair passes: 128, 256 or 512 (0 = the
+ * default 256).  A launch schedule the autotuner picks per workload class;
+ * results do not depend on it.  No counterpart in the reference. */
+int sph_arena_set_pass_block(sph_arena *ar, int block);
+int sph_box_set_pass_block(sph_box *box, int block);
 /* Device bytes an arena of particle_capacity slots allocates (the serving
  * loop's HBM-budget admission).  No counterpart in the reference. */
 int sph_arena_footprint(int64_t particle_capacity, int precision, int64_t *bytes);

@@ -11,6 +11,9 @@ rounding):
             the same for the rate pass alone, whose walk keeps five records
             per entry in flight (two was best for one large FP64 box, four
             for batched boxes); tuned after the other two
+    block   work-items per block of the three pair passes (128, 256 or
+            512): how the slots, in cell order, are dealt to the XCDs and
+            CUs; tuned last
     skin    neighbour-list skin: lists hold pairs within r_cut + skin and are
             reused while no particle can have crossed half of it; a larger
             skin rebuilds less often and walks longer lists
@@ -34,6 +37,7 @@ import time
 from . import _lib
 
 UNROLLS = (1, 2, 4)
+BLOCKS = (128, 256, 512)
 SKINS = (0.3, 0.5, 0.75, 1.0, 1.25)
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "schedules.json")
 
@@ -82,7 +86,8 @@ def save_cache(data, path=CACHE):
 
 def valid(s):
     return (isinstance(s, dict) and s.get("unroll") in UNROLLS and isinstance(s.get("skin"), (int, float))
-            and 0.0 < float(s["skin"]) <= 2.0 and s.get("rates_unroll", 0) in (0,) + UNROLLS)
+            and 0.0 < float(s["skin"]) <= 2.0 and s.get("rates_unroll", 0) in (0,) + UNROLLS
+            and s.get("block", 0) in (0,) + BLOCKS)
 
 
 def schedule_for(precision, cls, path=CACHE):
@@ -92,9 +97,9 @@ def schedule_for(precision, cls, path=CACHE):
     if entry and valid(entry.get("best")):
         b = entry["best"]
         return dict(unroll=int(b["unroll"]), skin=float(b["skin"]), rates_unroll=int(b.get("rates_unroll", 0)),
-                    source="tuned")
+                    block=int(b.get("block", 0)), source="tuned")
     d = DEFAULT[int(precision)]
-    return dict(d, rates_unroll=0, source="default")
+    return dict(d, rates_unroll=0, block=0, source="default")
 
 
 def candidates():
@@ -189,6 +194,20 @@ def tune(systems, precision, cls=None, steps=None, warm=100, save=True, path=CAC
                 key(precision, cls), c["unroll"], c["skin"], ru, rate))
     best["rates_unroll"] = pick_rates(rtable, best["unroll"])
     table += rtable
+    # then the pair passes' block size at that schedule
+    btable = []
+    for bs in BLOCKS:
+        c = dict(best, block=bs)
+        if single:
+            rate = _time_box(systems[0], c, precision, steps, warm)
+        else:
+            rate = _time_batched(systems, c, precision, steps, warm)
+        btable.append(dict(c, rate=rate))
+        if log:
+            log("tune %s unroll=%d skin=%.2f rates_unroll=%d block=%d  %.3e particle-steps/s" % (
+                key(precision, cls), c["unroll"], c["skin"], c["rates_unroll"], bs, rate))
+    best["block"] = pick_block(btable)
+    table += btable
     if save:
         data = load_cache(path)
         data[key(precision, cls)] = dict(
@@ -216,6 +235,16 @@ def pick_rates(rtable, unroll, within=0.01):
     if same and same[0]["rate"] >= (1.0 - within) * top["rate"]:
         return 0
     return int(top["rates_unroll"])
+
+
+def pick_block(btable, within=0.01):
+    """The pair passes' block size: the fastest, unless the default (256)
+    is within `within` of it (then 0, the default)."""
+    top = max(btable, key=lambda e: e["rate"])
+    same = [e for e in btable if e["block"] == 256]
+    if same and same[0]["rate"] >= (1.0 - within) * top["rate"]:
+        return 0
+    return int(top["block"])
 
 
 def rate_spread(table):

@@ -220,6 +220,7 @@ struct Arena {
     int always_build = 0;  // arenas: build stage every step (per-box schedule on device)
     bool small_boxes = false;  // all live boxes fit the per-box sort (set by relayout)
     int unroll = 0;        // list entries in flight per lane in the pair passes (0 = default)
+    int pass_block = 256;  // work-items per block of the pair passes (128, 256, 512)
     int unroll_rates = 0;  // the same for the rate pass alone (0 = unroll)
     int rates_unroll() const { return unroll_rates ? unroll_rates : unroll; }
 
@@ -245,6 +246,7 @@ struct Arena {
         A.nkeys = nkeys;
         A.cap = cap;
         A.lcap = lcap;
+        A.pass_block = pass_block;
         return A;
     }
 
@@ -1817,6 +1819,28 @@ static int set_rates_unroll(Arena &A, int unroll)
     A.unroll_rates = unroll;
     A.check_graph_sig();
     return SPH_OK;
+}
+
+static int set_pass_block(Arena &A, int block)
+{
+    if (block != 0 && block != 128 && block != 256 && block != 512)
+        return fail(SPH_ERR_ARG, "pass block size must be 0 (default 256), 128, 256 or 512");
+    A.pass_block = block ? block : 256;
+    A.check_graph_sig();
+    return SPH_OK;
+}
+
+int sph_arena_set_pass_block(sph_arena *ar, int block)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    return set_pass_block(ar->ar, block);
+}
+
+int sph_box_set_pass_block(sph_box *box, int block)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    if (int rc = set_pass_block(box->ar, block)) return rc;
+    return box->ar.prepare_graphs();
 }
 
 int sph_arena_set_rates_unroll(sph_arena *ar, int unroll)

@@ -150,7 +150,7 @@ struct ArenaView {
     int nkeys;           // total keys
     int cap;             // slot capacity
     int lcap;            // list entries per slot
-    int pad;
+    int pass_block;      // work-items per block of the pair passes (128, 256 or 512)
 };
 
 // Grid of the splash rendering: particle cells at least one cutoff wide.

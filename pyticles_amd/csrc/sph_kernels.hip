@@ -28,6 +28,9 @@
 namespace sph {
 
 static constexpr int NT = 256;  // work-items per block (4 waves)
+// The pair passes take their block size at launch (ArenaView::pass_block,
+// a schedule the autotuner picks): 128, 256 or 512 work-items.
+static constexpr int PASS_NT_MAX = 512;
 static constexpr double SPH_PI = 3.14159265358979323846;  // math.pi (spkernel.py:15)
 // The pair passes are templates on the number of list entries whose loads
 // are in flight per lane (U64 / U32 = 1, 2 or 4); the schedule picks one at
@@ -955,9 +958,9 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
 //   du_co,i = -a  sum_j m_j (v_j - v_i) . dwdx_hl,ij      (dv . f / 2)
 // and the rate pass only walks the short entries (r < h + skin).
 template <int U64>
-__global__ __launch_bounds__(NT) void k_density64(ArenaView A, int state_mask)
+__global__ __launch_bounds__(PASS_NT_MAX) void k_density64(ArenaView A, int state_mask)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
+    const int s = xcd_block() * (int)blockDim.x + threadIdx.x;
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
@@ -1059,9 +1062,9 @@ __device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K,
 // flux q = -k grad T.  diag != 0 also stores grad v and q for download.
 // Neighbour record: position, velocity, (m/rho, T) from P1.
 template <int U64>
-__global__ __launch_bounds__(NT) void k_grad64(ArenaView A, int state_mask, int diag)
+__global__ __launch_bounds__(PASS_NT_MAX) void k_grad64(ArenaView A, int state_mask, int diag)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
+    const int s = xcd_block() * (int)blockDim.x + threadIdx.x;
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
@@ -1153,9 +1156,9 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
 //   a_i = sum_j m_j f + a_co,i;   udot_i = sum_j m_j (dv . f / 2 + (Q_i + Q_j) . dwdx_h) + du_co,i
 // Neighbour record: position, velocity, (Sxx, Sxy, Syy, m), Q.
 template <int U64>
-__global__ __launch_bounds__(NT) void k_rates64(ArenaView A, int state_mask)
+__global__ __launch_bounds__(PASS_NT_MAX) void k_rates64(ArenaView A, int state_mask)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
+    const int s = xcd_block() * (int)blockDim.x + threadIdx.x;
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
@@ -1412,9 +1415,9 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
 // P1 of the FP32C form: rf carries position and predicted velocity; the
 // cohesive rate part (see k_density64) is accumulated with the densities.
 template <int U32>
-__global__ __launch_bounds__(NT) void k_density32(ArenaView A, int state_mask)
+__global__ __launch_bounds__(PASS_NT_MAX) void k_density32(ArenaView A, int state_mask)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
+    const int s = xcd_block() * (int)blockDim.x + threadIdx.x;
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
@@ -1533,9 +1536,9 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, i
 // P2 of the FP32C form.  Writes the stress record (S32, with viscosity) and
 // the rate pass's q/rho^2 record.
 template <int U32>
-__global__ __launch_bounds__(NT) void k_grad32(ArenaView A, int state_mask, int diag)
+__global__ __launch_bounds__(PASS_NT_MAX) void k_grad32(ArenaView A, int state_mask, int diag)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
+    const int s = xcd_block() * (int)blockDim.x + threadIdx.x;
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
@@ -1627,9 +1630,9 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
 
 // P3 of the FP32C form over the short entries; adds the cohesive part from P1.
 template <int U32>
-__global__ __launch_bounds__(NT) void k_rates32(ArenaView A, int state_mask)
+__global__ __launch_bounds__(PASS_NT_MAX) void k_rates32(ArenaView A, int state_mask)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
+    const int s = xcd_block() * (int)blockDim.x + threadIdx.x;
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
@@ -2247,6 +2250,14 @@ __global__ __launch_bounds__(NT) void k_splash_pixels(int nx, int ny, const doub
 // ===========================================================================
 
 static inline dim3 grid_for(long long n) { return dim3((unsigned)((n + NT - 1) / NT > 0 ? (n + NT - 1) / NT : 1)); }
+static inline dim3 grid_for_bs(long long n, int bs)
+{
+    return dim3((unsigned)((n + bs - 1) / bs > 0 ? (n + bs - 1) / bs : 1));
+}
+static inline int pass_block_of(const ArenaView &A)
+{
+    return (A.pass_block == 128 || A.pass_block == 512) ? A.pass_block : NT;
+}
 
 void launch_slot_box(const ArenaView &A, hipStream_t st)
 {
@@ -2291,7 +2302,8 @@ void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_
 // unroll: entries in flight per lane (1, 2 or 4; anything else = default)
 #define SPH_DISPATCH(K64, K32, ...)                                                              \
     do {                                                                                         \
-        const dim3 g = grid_for(A.n), b(NT);                                                     \
+        const int bs_ = pass_block_of(A);                                                        \
+        const dim3 g = grid_for_bs(A.n, bs_), b(bs_);                                            \
         if (fp32) {                                                                              \
             if (unroll == 1) hipLaunchKernelGGL(K32<1>, g, b, 0, st, __VA_ARGS__);               \
             else if (unroll == 2) hipLaunchKernelGGL(K32<2>, g, b, 0, st, __VA_ARGS__);          \

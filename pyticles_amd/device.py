@@ -132,14 +132,16 @@ class DeviceBox:
     def snapshot_wait(self):
         _lib.check(_lib.lib.sph_box_snapshot_wait(self._h))
 
-    def set_schedule(self, unroll, rates_unroll=0):
+    def set_schedule(self, unroll, rates_unroll=0, block=0):
         """List entries in flight per lane in the pair passes (1, 2, 4; 0 =
-        default) and in the rate pass alone (0 = the same).  unroll may be a
-        schedule dict from autotune.schedule_for."""
+        default), in the rate pass alone (0 = the same) and the pair passes'
+        block size (128, 256, 512; 0 = 256).  unroll may be a schedule dict
+        from autotune.schedule_for."""
         if isinstance(unroll, dict):
-            unroll, rates_unroll = unroll["unroll"], unroll.get("rates_unroll", 0)
+            unroll, rates_unroll, block = unroll["unroll"], unroll.get("rates_unroll", 0), unroll.get("block", 0)
         _lib.check(_lib.lib.sph_box_set_schedule(self._h, int(unroll)))
         _lib.check(_lib.lib.sph_box_set_rates_unroll(self._h, int(rates_unroll)))
+        _lib.check(_lib.lib.sph_box_set_pass_block(self._h, int(block)))
 
     def timing(self, reset=False):
         out = np.zeros(7)

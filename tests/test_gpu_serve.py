@@ -13,14 +13,14 @@ from pyticles_amd.serve import Server, latency_summary
 pytestmark = pytest.mark.gpu
 
 
-def alone(sysm, precision, unroll=0, skin=None, rates_unroll=0):
+def alone(sysm, precision, unroll=0, skin=None, rates_unroll=0, block=0):
     prm = dict(sysm.prm, precision=precision)
     if skin is not None:
         prm["skin"] = skin
     # the schedule is set before the initial density pass (u = kB T - a rho):
     # the FP32 form's partial sums follow the unroll factor
     with DeviceBox(sysm.n, prm) as b:
-        b.set_schedule(unroll, rates_unroll)
+        b.set_schedule(unroll, rates_unroll, block)
         b.upload(r=sysm.r, v=sysm.v, m=sysm.m, u=np.zeros(sysm.n))
         b.init_thermal(sysm.T)
         b.step(sysm.steps)
@@ -68,6 +68,12 @@ def test_unroll_schedules_agree(precision):
         got = alone(s, precision, unroll=u, rates_unroll=ru)
         for k in ("r", "v", "u", "rho"):
             assert rel(got[k], base[k]) <= tol, (u, ru, k)
+    # the pair passes' block size changes which work-items share a block,
+    # never a slot's own walk: the same bits
+    for bs in (128, 512):
+        got = alone(s, precision, block=bs)
+        for k in ("r", "v", "u", "rho"):
+            assert np.array_equal(got[k], base[k]), (bs, k)
 
 
 def test_skin_changes_lists_not_physics():
@@ -93,8 +99,9 @@ def test_autotune_small_workload(tmp_path):
     p = str(tmp_path / "sched.json")
     s = configs.square_system(32, 0.9, 10, 10)
     best, table = autotune.tune(s, _lib.FP64, "box", steps=3, warm=1, path=p)
-    assert autotune.valid(best) and len(table) == len(autotune.candidates()) + len(autotune.UNROLLS)
-    assert best["rates_unroll"] in (0,) + autotune.UNROLLS
+    assert autotune.valid(best)
+    assert len(table) == len(autotune.candidates()) + len(autotune.UNROLLS) + len(autotune.BLOCKS)
+    assert best["rates_unroll"] in (0,) + autotune.UNROLLS and best["block"] in (0,) + autotune.BLOCKS
     assert all(e["rate"] > 0 for e in table)
     assert autotune.schedule_for(_lib.FP64, "box", path=p)["source"] == "tuned"
     boxes = [configs.lognormal_box(400 + j, 256, 600, 0.4, 0.9, steps=20) for j in range(4)]
@@ -127,7 +134,7 @@ def test_c5_jobs_served_equal_alone():
         min(s.n for s in jobs), max(s.n for s in jobs), srv.peak_resident), flush=True)
     for rec in recs:
         ref = alone(jobs[rec.job_id], prec, unroll=sched["unroll"], skin=sched["skin"],
-                    rates_unroll=sched.get("rates_unroll", 0))
+                    rates_unroll=sched.get("rates_unroll", 0), block=sched.get("block", 0))
         for k in ("r", "v", "u", "rho"):
             assert np.array_equal(rec.result[k], ref[k]), (rec.job_id, k)
 

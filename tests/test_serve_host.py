@@ -124,7 +124,8 @@ def test_autotune_cache_roundtrip(tmp_path):
     assert d["source"] == "default" and d["unroll"] == 2
     autotune.save_cache({autotune.key(_lib.FP32C, "batched"): {"best": {"unroll": 1, "skin": 0.3}}}, p)
     got = autotune.schedule_for(_lib.FP32C, "batched", path=p)
-    assert got == dict(unroll=1, skin=0.3, rates_unroll=0, source="tuned")  # a cache without rates_unroll
+    # a cache without rates_unroll or block: their defaults
+    assert got == dict(unroll=1, skin=0.3, rates_unroll=0, block=0, source="tuned")
     # an invalid entry falls back to the default
     autotune.save_cache({autotune.key(_lib.FP32C, "box"): {"best": {"unroll": 3, "skin": 0.3}}}, p)
     assert autotune.schedule_for(_lib.FP32C, "box", path=p)["source"] == "default"
