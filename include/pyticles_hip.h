@@ -205,6 +205,22 @@ int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch);
 int sph_splash_map_2d(sph_ctx *ctx, int64_t n, const double *r, const double *m, const double *rho,
                       const double *a, const double *h, int nx, const double *x, int ny, const double *y,
                       const double *bounds, double cutoff, double *Z);
+/* The same rendering without any allocation, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid it).  No counterpart in the reference.  Every array is a
+ * DEVICE pointer (r[n][2], m, rho, a, h, x[nx], y[ny] in; Z[nx][ny] out);
+ * bounds[4] and cutoff stay on the host.  The particle records, sort keys,
+ * cell starts and sort counters live in a fixed scratch area of the library
+ * (5.6 MB of device memory per GPU, held until the process ends): at most
+ * 65,536 particles and 65,536 grid cells (cells one cutoff wide); larger
+ * calls return SPH_ERR_ARG before launching anything.  The work is enqueued
+ * on the context's stream and the call returns without waiting.  All calls
+ * on one GPU share its scratch: calls that overlap on one GPU (from several
+ * contexts, streams or threads) race.  It calls no hipMalloc, hipFree or
+ * allocating library routine. */
+int sph_splash_map_2d_noalloc(sph_ctx *ctx, int64_t n, const double *r, const double *m, const double *rho,
+                              const double *a, const double *h, int nx, const double *x, int ny, const double *y,
+                              const double *bounds, double cutoff, double *Z);
 
 /* ---- trajectory snapshots ----------------------------------------------- */
 /* Pinned host memory (hipHostMalloc) for asynchronous snapshots. */

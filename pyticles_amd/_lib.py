@@ -108,6 +108,9 @@ _SIGS = {
     "sph_box_snapshot_wait": [_P],
     "sph_splash_map_2d": [_P, _I64, _D, _D, _D, _D, _D, ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_double,
                           _D],
+    # device pointers as integers (c_void_p); bounds on the host
+    "sph_splash_map_2d_noalloc": [_P, _I64, _P, _P, _P, _P, _P, ctypes.c_int, _P, ctypes.c_int, _P, _D,
+                                  ctypes.c_double, _P],
 }
 _RESTYPES = {"sph_last_error": ctypes.c_char_p, "sph_version": ctypes.c_char_p}
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,44 @@ int bits_for(int nkeys)
 }
 
 }  // namespace
+
+// ---- fixed scratch of the allocation-free rendering entry point -------------
+// sph_splash_map_2d_noalloc keeps its particle records, sort keys, cell starts
+// and radix-sort counters here instead of allocating: a file-scope device
+// array, one copy per GPU, held until the process ends.  Host code never
+// uses the name itself (on the host it is a stand-in, not a device address):
+// g_noalloc_scratch[device] holds the address hipGetSymbolAddress gives for
+// it on that device, taken once per device when a context is created.
+namespace {
+constexpr int64_t NOALLOC_MAX_PARTICLES = 65536;
+constexpr int NOALLOC_MAX_CELLS = 65536;
+constexpr size_t NOALLOC_SORT_BYTES = 64 * 1024;  // radix-sort counters of 65,536 keys: 33 KB
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NOALLOC_REC_BYTES = align256(sizeof(double4) * NOALLOC_MAX_PARTICLES);
+constexpr size_t NOALLOC_KEY_BYTES = align256(sizeof(unsigned) * 5 * NOALLOC_MAX_PARTICLES);
+constexpr size_t NOALLOC_CS_BYTES = align256(sizeof(int) * (NOALLOC_MAX_CELLS + 1));
+constexpr size_t NOALLOC_SCRATCH_BYTES = 2 * NOALLOC_REC_BYTES + NOALLOC_KEY_BYTES + NOALLOC_CS_BYTES +
+                                         NOALLOC_SORT_BYTES;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_scratch[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// The device address of `scratch` on the current device (once per device).
+static int noalloc_scratch_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(SPH_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_scratch[device]) return SPH_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SPH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_noalloc_scratch[device] = p;
+    return SPH_OK;
+}
 
 struct sph_ctx {
     int device = 0;
@@ -957,6 +996,7 @@ int sph_ctx_create(int device, sph_ctx **out)
     if (device < 0 || device >= ndev)
         return fail(SPH_ERR_ARG, "device " + std::to_string(device) + " not present (" + std::to_string(ndev) + " devices)");
     HIPCHK(hipSetDevice(device));
+    if (int rc = noalloc_scratch_setup(device)) return rc;
     sph_ctx *c = new sph_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1788,6 +1828,42 @@ int sph_splash_map_2d(sph_ctx *ctx, int64_t n, const double *r, const double *m,
     dfree(d_in); dfree(d_xy); dfree(d_Z); dfree(d_rec); dfree(d_rec2); dfree(d_keys); dfree(d_cs);
     if (d_tmp) (void)hipFree(d_tmp);
     return rc;
+}
+
+int sph_splash_map_2d_noalloc(sph_ctx *ctx, int64_t n, const double *r, const double *m, const double *rho,
+                              const double *a, const double *h, int nx, const double *x, int ny, const double *y,
+                              const double *bounds, double cutoff, double *Z)
+{
+    if (!ctx || n < 0 || nx < 1 || ny < 1 || !bounds || !x || !y || !Z || !(cutoff > 0) ||
+        (n > 0 && (!r || !m || !rho || !a || !h)))
+        return fail(SPH_ERR_ARG, "invalid argument");
+    if (n > NOALLOC_MAX_PARTICLES)
+        return fail(SPH_ERR_ARG, "sph_splash_map_2d_noalloc renders at most " + std::to_string(NOALLOC_MAX_PARTICLES) +
+                                     " particles (its fixed scratch); use sph_splash_map_2d");
+    if ((long long)nx * ny > (1ll << 31)) return fail(SPH_ERR_ARG, "too many pixels");
+    SplashGrid G;
+    if (int rc = splash_grid(bounds, cutoff, G)) return rc;
+    const int ncell = G.ncx * G.ncy;
+    if (ncell > NOALLOC_MAX_CELLS)
+        return fail(SPH_ERR_ARG, "sph_splash_map_2d_noalloc holds at most " + std::to_string(NOALLOC_MAX_CELLS) +
+                                     " grid cells (cells one cutoff wide); use sph_splash_map_2d");
+    const int N = (int)std::max<int64_t>(n, 1);
+    if (radix_sort_temp_bytes(N) > NOALLOC_SORT_BYTES) return fail(SPH_ERR_ARG, "sort counters exceed the scratch");
+    unsigned char *base = (unsigned char *)g_noalloc_scratch[ctx->device];
+    if (!base) return fail(SPH_ERR_STATE, "scratch address not set up for this device");
+    HIPCHK(hipSetDevice(ctx->device));
+    double4 *d_rec = (double4 *)base;
+    double4 *d_rec2 = (double4 *)(base + NOALLOC_REC_BYTES);
+    unsigned *d_keys = (unsigned *)(base + 2 * NOALLOC_REC_BYTES);
+    int *d_cs = (int *)(base + 2 * NOALLOC_REC_BYTES + NOALLOC_KEY_BYTES);
+    void *d_tmp = base + 2 * NOALLOC_REC_BYTES + NOALLOC_KEY_BYTES + NOALLOC_CS_BYTES;
+    // every scratch word a launch reads it first writes (records, keys, the
+    // cell starts of all ncell + 1 cells, the counters of every tile), so
+    // nothing needs clearing between calls
+    launch_splash(n, r, m, rho, a, h, G, d_rec, d_rec2, d_keys, d_keys + N, d_keys + 2 * N, d_keys + 3 * N,
+                  d_keys + 4 * N, d_tmp, bits_for(ncell), d_cs, nx, ny, x, y, Z, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
 }
 
 static int set_schedule(Arena &A, int unroll)
