@@ -141,6 +141,9 @@ def run_gpu(s, steps, warmup, timing=True, unroll=0, kernel_steps=200, launch_re
         # state the steps left (no per-launch event overhead)
         tm["launch_us"] = {p: b.time_pass(p, reps=launch_reps) for p in b.PASSES}
         tm["launch_reps"] = launch_reps
+        # the list build stage alone, back to back at the same state (it may
+        # reorder the slots, so after the pair passes)
+        tm["build_us"] = b.time_pass(b.BUILD, reps=20)
     if tm is not None:
         tm["timed_run"] = run_stats
     return b, t1 - t0, tm, (E0, E1)
@@ -455,6 +458,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip every extra leg")
     ap.add_argument("--tune", metavar="PATH", help="run the autotuner for every workload class, write PATH, exit")
+    ap.add_argument("--list-bits", type=int, default=0, choices=[0, 16, 32],
+                    help="neighbour-list entry width (0: the schedule's, else 16)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB (HBM counter calibration)")
     ap.add_argument("--c4-sample", type=int, default=0, metavar="STEPS",
@@ -502,6 +507,8 @@ def main():
 
     s, desc = workload(args.workload, args.precision)
     unroll, sched = schedule(s, "box")
+    if args.list_bits:
+        sched["list_bits"] = args.list_bits
     full = bool(s.prm["terms"] & 12)
     bps = B_PS[(args.precision, full)]
     log("%s, %s, %d steps after %d warmup, schedule %s" % (desc, args.precision, args.steps, args.warmup, sched))
@@ -546,6 +553,9 @@ def main():
                             "call; step graphs captured at box set-up, outside the timed region" % (args.steps, evals),
                    "schedule": sched},
         "timed_list_builds": int(tm["timed_run"]["builds"]),
+        # the whole list build stage (sort, permutation, lists) at the end
+        # state, 20 back-to-back launches between two events
+        "build_stage_us": round(tm["build_us"], 2),
         "timed_builds_postponed": int(tm["timed_run"]["extends"]),
         "roofline": roof,
         "pair_interactions_per_s": nip * args.steps / el,

@@ -188,9 +188,12 @@ int sph_box_timing(sph_box *box, double *out, int n, int reset);
  * the context stream around reps back-to-back launches.  The passes write
  * only derived fields (densities, EOS, gradients, rates), never r, v, u or
  * m; the next sph_step re-derives them at its first evaluation, so timing
- * does not change later steps.  The box must have been stepped at least
- * once (its lists and derived fields exist).  No counterpart in the
- * reference: the measurement behind the dominant kernel's roofline figure. */
+ * does not change later steps.  Pass 3 is the whole list build stage (cell
+ * sort, permutation, lists): it leaves every particle's values but may
+ * reorder the slots, and the next run builds at its first step.  The box
+ * must have been stepped at least once (its lists and derived fields
+ * exist).  No counterpart in the reference: the measurement behind the
+ * dominant kernel's roofline figure. */
 int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch);
 
 /* ---- rendering ------------------------------------------------------------ */
@@ -289,6 +292,17 @@ int sph_box_set_rates_unroll(sph_box *box, int unroll);
  * results do not depend on it.  No counterpart in the reference. */
 int sph_arena_set_pass_block(sph_arena *ar, int block);
 int sph_box_set_pass_block(sph_box *box, int block);
+/* Entry width of the neighbour lists the boxes' next builds write: 16
+ * (default: slot offsets, half the list bytes of 32; a box whose offsets do
+ * not fit 16 bits holds the step of that build and continues with 32-bit
+ * entries by itself) or 32 (slots).  Results are the same bits either way.
+ * sph_*_list_bits reports the width of a box's lists in memory (after its
+ * first build).  No counterpart in the reference (its lists are Python
+ * arrays, neighbour_list.py:32-35). */
+int sph_arena_set_list_bits(sph_arena *ar, int bits);
+int sph_box_set_list_bits(sph_box *box, int bits);
+int sph_arena_list_bits(sph_arena *ar, int box_id, int *bits);
+int sph_box_list_bits(sph_box *box, int *bits);
 /* Device bytes an arena of particle_capacity slots allocates (the serving
  * loop's HBM-budget admission).  No counterpart in the reference. */
 int sph_arena_footprint(int64_t particle_capacity, int precision, int64_t *bytes);

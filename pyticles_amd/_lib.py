@@ -101,6 +101,10 @@ _SIGS = {
     "sph_box_set_rates_unroll": [_P, ctypes.c_int],
     "sph_arena_set_pass_block": [_P, ctypes.c_int],
     "sph_box_set_pass_block": [_P, ctypes.c_int],
+    "sph_arena_set_list_bits": [_P, ctypes.c_int],
+    "sph_box_set_list_bits": [_P, ctypes.c_int],
+    "sph_arena_list_bits": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "sph_box_list_bits": [_P, ctypes.POINTER(ctypes.c_int)],
     "sph_arena_footprint": [_I64, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)],
     "sph_host_alloc": [_I64, ctypes.POINTER(_P)],
     "sph_host_free": [_P],

@@ -82,16 +82,30 @@ class Arena:
         keys = ("build_ms", "density_ms", "gradients_ms", "rates_ms", "pairs_ms", "steps", "integrate_ms")
         return dict(zip(keys, out))
 
-    def set_schedule(self, unroll, rates_unroll=0, block=0):
+    def set_schedule(self, unroll, rates_unroll=0, block=0, list_bits=0):
         """List entries in flight per lane in the pair passes (1, 2, 4; 0 =
-        default), in the rate pass alone (0 = the same) and the pair passes'
-        block size (128, 256, 512; 0 = 256).  unroll may be a schedule dict
-        from autotune.schedule_for."""
+        default), in the rate pass alone (0 = the same), the pair passes'
+        block size (128, 256, 512; 0 = 256) and the list entry width (16,
+        32; 0 = leave it).  unroll may be a schedule dict from
+        autotune.schedule_for."""
         if isinstance(unroll, dict):
-            unroll, rates_unroll, block = unroll["unroll"], unroll.get("rates_unroll", 0), unroll.get("block", 0)
+            unroll, rates_unroll, block, list_bits = (unroll["unroll"], unroll.get("rates_unroll", 0),
+                                                      unroll.get("block", 0), unroll.get("list_bits", 0))
         _lib.check(_lib.lib.sph_arena_set_schedule(self._h, int(unroll)))
         _lib.check(_lib.lib.sph_arena_set_rates_unroll(self._h, int(rates_unroll)))
         _lib.check(_lib.lib.sph_arena_set_pass_block(self._h, int(block)))
+        if list_bits:
+            self.set_list_bits(list_bits)
+
+    def set_list_bits(self, bits):
+        """Entry width of the lists the boxes' next builds write (16 or 32)."""
+        _lib.check(_lib.lib.sph_arena_set_list_bits(self._h, int(bits)))
+
+    def list_bits(self, box_id):
+        """Entry width of a box's lists in memory."""
+        b = ctypes.c_int()
+        _lib.check(_lib.lib.sph_arena_list_bits(self._h, int(box_id), ctypes.byref(b)))
+        return b.value
 
     def add(self, sysm, nsteps=None, T0=None):
         """Admit a configs.System; returns its box id.  Raises SphError with

@@ -1,8 +1,8 @@
 """Launch-schedule autotuner for the pair passes.
 
-Three knobs set how the device walks a box's neighbours, and none changes
+These knobs set how the device walks a box's neighbours, and none changes
 which pairs interact (only the summation order, so results agree to
-rounding):
+rounding; the list entry width not even that):
 
     unroll  list entries whose loads are in flight per lane in the density,
             gradient and rate passes (1, 2 or 4): more hides more gather
@@ -13,7 +13,11 @@ rounding):
             for batched boxes); tuned after the other two
     block   work-items per block of the three pair passes (128, 256 or
             512): how the slots, in cell order, are dealt to the XCDs and
-            CUs; tuned last
+            CUs
+    list_bits
+            neighbour-list entry width (16: slot offsets, half the list
+            bytes, one 16-byte load per eight entries; 32: slots); same
+            results either way; tuned last
     skin    neighbour-list skin: lists hold pairs within r_cut + skin and are
             reused while no particle can have crossed half of it; a larger
             skin rebuilds less often and walks longer lists
@@ -38,6 +42,7 @@ from . import _lib
 
 UNROLLS = (1, 2, 4)
 BLOCKS = (128, 256, 512)
+LIST_BITS = (16, 32)
 SKINS = (0.3, 0.5, 0.75, 1.0, 1.25)
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "schedules.json")
 
@@ -87,7 +92,7 @@ def save_cache(data, path=CACHE):
 def valid(s):
     return (isinstance(s, dict) and s.get("unroll") in UNROLLS and isinstance(s.get("skin"), (int, float))
             and 0.0 < float(s["skin"]) <= 2.0 and s.get("rates_unroll", 0) in (0,) + UNROLLS
-            and s.get("block", 0) in (0,) + BLOCKS)
+            and s.get("block", 0) in (0,) + BLOCKS and s.get("list_bits", 0) in (0,) + LIST_BITS)
 
 
 def schedule_for(precision, cls, path=CACHE):
@@ -97,9 +102,9 @@ def schedule_for(precision, cls, path=CACHE):
     if entry and valid(entry.get("best")):
         b = entry["best"]
         return dict(unroll=int(b["unroll"]), skin=float(b["skin"]), rates_unroll=int(b.get("rates_unroll", 0)),
-                    block=int(b.get("block", 0)), source="tuned")
+                    block=int(b.get("block", 0)), list_bits=int(b.get("list_bits", 0)), source="tuned")
     d = DEFAULT[int(precision)]
-    return dict(d, rates_unroll=0, block=0, source="default")
+    return dict(d, rates_unroll=0, block=0, list_bits=0, source="default")
 
 
 def candidates():
@@ -208,6 +213,20 @@ def tune(systems, precision, cls=None, steps=None, warm=100, save=True, path=CAC
                 key(precision, cls), c["unroll"], c["skin"], c["rates_unroll"], bs, rate))
     best["block"] = pick_block(btable)
     table += btable
+    # then the list entry width
+    ltable = []
+    for lb in LIST_BITS:
+        c = dict(best, list_bits=lb)
+        if single:
+            rate = _time_box(systems[0], c, precision, steps, warm)
+        else:
+            rate = _time_batched(systems, c, precision, steps, warm)
+        ltable.append(dict(c, rate=rate))
+        if log:
+            log("tune %s unroll=%d skin=%.2f rates_unroll=%d block=%d list_bits=%d  %.3e particle-steps/s" % (
+                key(precision, cls), c["unroll"], c["skin"], c["rates_unroll"], c["block"], lb, rate))
+    best["list_bits"] = pick_list_bits(ltable)
+    table += ltable
     if save:
         data = load_cache(path)
         data[key(precision, cls)] = dict(
@@ -251,3 +270,12 @@ def rate_spread(table):
     """max / min rate over a tuning table (how much the choice matters)."""
     r = [e["rate"] for e in table]
     return max(r) / min(r) if r and min(r) > 0 else math.nan
+
+
+def pick_list_bits(ltable, within=0.01):
+    """16-bit entries (half the list traffic) unless 32-bit ones are more
+    than `within` faster."""
+    r = {e["list_bits"]: e["rate"] for e in ltable}
+    if 16 in r and 32 in r and r[32] > (1.0 + within) * r[16]:
+        return 32
+    return 16

@@ -20,6 +20,12 @@ HIPCC_FLAGS = [
     # packed FP32 instructions stay off (hipcc would otherwise emit
     # v_pk_* forms from ordinary float code)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds fuse within a source expression only (written fma()
+    # calls are kept): with HIP's default (fast) the backend also fuses
+    # across statements as each instantiation's schedule allows, so two
+    # instantiations of one walk (list entry widths, say) could round the
+    # same terms differently
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -236,7 +237,15 @@ struct Arena {
     // open run of a single box (sph_box_advance / sph_box_sync): the box
     // runs with an unbounded step count, so no step is its closing one
     bool open_run = false;
-    int64_t open_plain = 0;  // next steps known not to build; -1: read the schedule first
+    int64_t open_plain = 0;  // next steps known not to build
+    // the single box's lists are still valid for its state: its last run
+    // ended without a health bit and no upload, parameter change, relayout
+    // or out-of-run list build came after it (run_start)
+    bool sched_kept = false;
+    // list entry width for new boxes and set_list_bits (16: 16-bit offsets,
+    // a box falling back to 32-bit slots when its offsets do not fit)
+    int list_bits = 16;
+    int64_t released = 0;  // boxes switched to 32-bit entries after a held build (read_dyn)
 
     // captured graphs of one arena step without / with a list build
     // step graphs: [0] one step without the build stage, [1] one step with
@@ -294,7 +303,7 @@ struct Arena {
         cap = capacity;
         precision = prec;
         lcap = list_entries;
-        if (lcap <= 0 || lcap % 4) return fail(SPH_ERR_ARG, "list entries must be a positive multiple of 4");
+        if (lcap <= 0 || lcap % 8) return fail(SPH_ERR_ARG, "list entries must be a positive multiple of 8");
         int rc = 0;
         rc |= dalloc(&r, cap); rc |= dalloc(&v, cap); rc |= dalloc(&vp, cap); rc |= dalloc(&acc, cap);
         rc |= dalloc(&u, cap); rc |= dalloc(&m, cap); rc |= dalloc(&up, cap); rc |= dalloc(&udot, cap);
@@ -477,6 +486,7 @@ struct Arena {
     {
         int key = 0;
         hwm = 0;
+        sched_kept = false;  // slots, constants or boxes change
         for (auto &sg : segs) {
             sg.key = key;
             if (sg.box >= 0) {
@@ -665,16 +675,56 @@ struct Arena {
         return collect_timing();
     }
 
+    // The schedule of a new run on box b (k = 0, nsteps steps).  With the
+    // lists kept, the run continues their schedule from the current step
+    // (next build, build step, speed and acceleration bounds, each shifted
+    // by the last run's length): a run that follows another builds where
+    // the schedule of one longer run would, not at once and again two steps
+    // later, as a box with fresh lists does.  plain = the first steps known
+    // not to build.
+    int run_start(int b, int64_t nsteps, BoxDyn &D, int64_t &plain)
+    {
+        memset(&D, 0, sizeof(D));
+        D.state = BOX_RUN;
+        D.k = 0;
+        D.nsteps = nsteps;
+        D.next_rebuild = 0;
+        plain = 0;
+        if (sched_kept) {
+            if (int rc = read_dyn()) return rc;
+            const BoxDyn &P = h_dyn[b];
+            D.next_rebuild = P.next_rebuild - P.k;  // <= 0: build at the first step
+            D.built_at = P.built_at - P.k;
+            D.vlast = P.vlast;
+            D.alast = P.alast;
+            D.list16 = P.list16;
+            plain = D.next_rebuild > 0 ? D.next_rebuild : 0;
+        }
+        sched_kept = false;  // the run's schedule is on the device until it ends
+        return SPH_OK;
+    }
+
     // One running box: steps without the build stage until the box's next
-    // scheduled build (read back after each build).
-    int step_single(int b, int64_t evals)
+    // scheduled build (read back after each build); the first `plain` steps
+    // are known not to build.
+    int step_single(int b, int64_t evals, int64_t plain0)
     {
         int64_t done = 0;
+        if (plain0 > 0) {
+            done = std::min(plain0, evals);
+            if (int rc = launch_steps(done, false)) return rc;
+        }
+        int held = 0;
         while (done < evals) {
+            const int64_t rel = released;
             if (int rc = launch_steps(1, true)) return rc;
+            if (int rc = read_dyn()) return rc;
+            if (released != rel) {  // the build held the box; it builds again in 32-bit entries
+                if (++held > 2) return fail(SPH_ERR_HIP, "list build held repeatedly");
+                continue;
+            }
             ++done;
             if (done >= evals) break;
-            if (int rc = read_dyn()) return rc;
             const BoxDyn &D = h_dyn[b];
             if (D.health) break;
             int64_t plain = D.next_rebuild - D.k;
@@ -697,30 +747,31 @@ struct Arena {
     {
         if (!open_run) {
             BoxDyn D;
-            memset(&D, 0, sizeof(D));
-            D.state = BOX_RUN;
-            D.k = 0;
-            D.nsteps = OPEN_STEPS;
-            D.next_rebuild = 0;
+            int64_t plain = 0;
+            if (int rc = run_start(b, OPEN_STEPS, D, plain)) return rc;
             if (int rc = write_dyn(b, D)) return rc;
             open_run = true;
-            open_plain = 0;
+            open_plain = plain;
         }
         int64_t done = 0;
+        int held = 0;
         while (done < nsteps) {
-            if (open_plain < 0) {
+            if (open_plain == 0) {
+                // a step with the build stage, then the schedule read back
+                const int64_t rel = released;
+                if (int rc = launch_steps(1, true)) return rc;
                 if (int rc = read_dyn()) return rc;
+                if (released != rel) {  // held build: again, in 32-bit entries
+                    if (++held > 2) return fail(SPH_ERR_HIP, "list build held repeatedly");
+                    continue;
+                }
+                ++done;
                 const BoxDyn &D = h_dyn[b];
                 if (D.health) {
                     open_run = false;
                     return SPH_OK;  // the caller's health check reports it
                 }
                 open_plain = D.next_rebuild > D.k ? D.next_rebuild - D.k : 0;
-            }
-            if (open_plain == 0) {
-                if (int rc = launch_steps(1, true)) return rc;
-                open_plain = -1;
-                ++done;
             } else {
                 const int64_t p = std::min(open_plain, nsteps - done);
                 if (int rc = launch_steps(p, false)) return rc;
@@ -744,16 +795,58 @@ struct Arena {
         if (D.health || D.state != BOX_RUN) return SPH_OK;
         D.nsteps = D.k;
         if (int rc = write_dyn(b, D)) return rc;
-        if (int rc = launch_steps(1, true)) return rc;
+        for (int held = 0;;) {
+            const int64_t rel = released;
+            if (int rc = launch_steps(1, true)) return rc;
+            if (int rc = read_dyn()) return rc;
+            if (released == rel) break;  // else a held build: again, in 32-bit entries
+            if (++held > 2) return fail(SPH_ERR_HIP, "list build held repeatedly");
+        }
         return collect_timing();
     }
 
+    // The box states, read back (waits for the stream); boxes held by a build
+    // are released on the way (release_held).
     int read_dyn()
     {
         if (boxes.empty()) return SPH_OK;
         HIPCHK(hipMemcpyAsync(h_dyn, d_dyn, sizeof(BoxDyn) * boxes.size(), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        return release_held();
+    }
+
+    // A box whose build met a neighbour offset beyond 16 bits held that step
+    // (BoxDyn::relist): switch it to 32-bit entries and clear the flag; its
+    // next step builds again, with the same entries in the same order.  The
+    // copies are stream-ordered before any later launch.
+    int release_held()
+    {
+        static const int zero = 0;
+        for (size_t b = 0; b < boxes.size(); ++b) {
+            if (!box_live[b] || !h_dyn[b].relist) continue;
+            boxes[b].build16 = 0;
+            HIPCHK(hipMemcpyAsync(d_boxes + b, &boxes[b], sizeof(BoxDesc), hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync((char *)(d_dyn + b) + offsetof(BoxDyn, relist), &zero, sizeof(int),
+                                  hipMemcpyHostToDevice, ctx->stream));
+            h_dyn[b].relist = 0;
+            ++released;
+        }
         return SPH_OK;
+    }
+
+    // Build stage outside a run (the stand-alone evaluations and the pair
+    // export) for the boxes in force_mask; a held build is repeated in
+    // 32-bit entries, so the lists are whole when this returns.
+    int build_now(int force_mask)
+    {
+        for (int held = 0;;) {
+            const int64_t rel = released;
+            enqueue_build(force_mask);
+            HIPCHK(hipGetLastError());
+            if (int rc = read_dyn()) return rc;
+            if (released == rel) return SPH_OK;
+            if (++held > 2) return fail(SPH_ERR_HIP, "list build held repeatedly");
+        }
     }
 
     // h_dyn is pinned and copies run in stream order (a later read_dyn lands
@@ -840,6 +933,7 @@ int Arena::add_box(int64_t n, const sph_params *prm, int *box_out)
     BoxDesc B;
     if (int rc = fill_desc(prm, n, B)) return rc;
     if (prm->precision != precision) return fail(SPH_ERR_ARG, "box precision differs from the arena's");
+    B.build16 = list_bits == 16 ? 1 : 0;
     {
         // mean neighbours within r_cut + skin at the box's number density:
         // a box whose mean alone takes over half the list entries is refused
@@ -909,6 +1003,7 @@ int Arena::remove_box(int b)
 int Arena::upload(int b, int field, const double *host, int ncomp)
 {
     const int n = boxes[b].n;
+    sched_kept = false;  // new positions or new speed and acceleration bounds
     const size_t bytes = sizeof(double) * (size_t)n * ncomp;
     if (stage(bytes)) return SPH_ERR_NOMEM;
     HIPCHK(hipMemcpyAsync(d_stage, host, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -973,7 +1068,9 @@ int settle(sph_box *box)
     if (!A.open_run) return SPH_OK;
     HIPCHK(hipSetDevice(A.ctx->device));
     if (int rc = A.close_run(box->b)) return rc;
-    return check_health(A, box->b);
+    if (int rc = check_health(A, box->b)) return rc;
+    A.sched_kept = true;
+    return SPH_OK;
 }
 }  // namespace
 
@@ -1070,6 +1167,7 @@ int sph_box_set_params(sph_box *box, const sph_params *prm)
     if (int rc = fill_desc(prm, box->ar.boxes[box->b].n, B)) return rc;
     B.munif = box->ar.boxes[box->b].munif;  // state, not parameters
     B.T0 = box->ar.boxes[box->b].T0;
+    B.build16 = box->ar.boxes[box->b].build16;
     box->ar.boxes[box->b] = B;
     box->prm = *prm;
     if (int rc = box->ar.relayout()) return rc;
@@ -1134,8 +1232,9 @@ int sph_nlist_build(sph_box *box, int64_t cap, int64_t *iap, int64_t *nip_out)
     hipStream_t st = A.ctx->stream;
     HIPCHK(hipSetDevice(A.ctx->device));
     const int n = A.boxes[box->b].n;
+    A.sched_kept = false;  // lists rebuilt outside a run's schedule
     A.mark(T_PAIRS, true);
-    A.enqueue_build(MASK_ALL);
+    if (int rc = A.build_now(MASK_ALL)) return rc;
     if (!A.d_inv) {
         if (dalloc(&A.d_inv, A.cap) || dalloc(&A.d_cnt, A.cap) || dalloc(&A.d_off, A.cap) ||
             dalloc(&A.d_grand, 1))
@@ -1299,8 +1398,9 @@ static int eval_box(sph_box *box, int grad, int rates)
     HIPCHK(hipSetDevice(A.ctx->device));
     const int f = A.fp32() ? 1 : 0;
     ArenaView V = A.view();
+    A.sched_kept = false;  // lists rebuilt outside a run's schedule
     A.mark(T_SORT, true);
-    A.enqueue_build(MASK_ALL);
+    if (int rc = A.build_now(MASK_ALL)) return rc;
     A.mark(T_SORT, false);
     A.mark(T_DENS, true);
     launch_density(V, f, MASK_ALL, A.unroll, st);
@@ -1381,20 +1481,19 @@ int sph_step(sph_box *box, int64_t nsteps)
     if (A.open_run) {  // continue the open run by nsteps, then close it
         if (int rc = A.advance(box->b, nsteps)) return rc;
         if (int rc = A.close_run(box->b)) return rc;
-        return check_health(A, box->b);
+    } else {
+        BoxDyn D;
+        int64_t plain = 0;
+        if (int rc = A.run_start(box->b, nsteps, D, plain)) return rc;
+        if (int rc = A.write_dyn(box->b, D)) return rc;
+        // nsteps leapfrog steps take nsteps + 1 rate evaluations
+        if (int rc = A.step_single(box->b, nsteps + 1, plain)) return rc;
+        A.steps_done -= 1.0;
+        A.stepped = true;
     }
-    BoxDyn D;
-    memset(&D, 0, sizeof(D));
-    D.state = BOX_RUN;
-    D.k = 0;
-    D.nsteps = nsteps;
-    D.next_rebuild = 0;
-    if (int rc = A.write_dyn(box->b, D)) return rc;
-    // nsteps leapfrog steps take nsteps + 1 rate evaluations
-    if (int rc = A.step_single(box->b, nsteps + 1)) return rc;
-    A.steps_done -= 1.0;
-    A.stepped = true;
-    return check_health(A, box->b);
+    if (int rc = check_health(A, box->b)) return rc;
+    A.sched_kept = true;
+    return SPH_OK;
 }
 
 int sph_box_advance(sph_box *box, int64_t nsteps)
@@ -1466,8 +1565,8 @@ int sph_box_timing(sph_box *box, double *out, int n, int reset)
 
 int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch)
 {
-    if (!box || !us_per_launch || pass < 0 || pass > 2 || reps < 1 || reps > 100000)
-        return fail(SPH_ERR_ARG, "invalid argument (pass 0..2, reps 1..100000)");
+    if (!box || !us_per_launch || pass < 0 || pass > 3 || reps < 1 || reps > 100000)
+        return fail(SPH_ERR_ARG, "invalid argument (pass 0..3, reps 1..100000)");
     if (int rc = settle(box)) return rc;
     Arena &A = box->ar;
     HIPCHK(hipSetDevice(A.ctx->device));
@@ -1475,10 +1574,17 @@ int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch)
     // between sph_step calls the box rests in BOX_DONE, which the stepping
     // masks skip: the passes run under MASK_ALL, as the single evaluations
     // do.  They write only derived fields (never r, v, u, m), which the next
-    // sph_step re-derives at its first evaluation.
+    // sph_step re-derives at its first evaluation.  The build stage (pass 3)
+    // re-sorts the slots into cell order and rebuilds the lists at the
+    // current positions: the particles keep their values, the next run builds
+    // at its first step.
     const ArenaView V = A.view();
     const int f = A.fp32() ? 1 : 0;
     hipStream_t st = A.ctx->stream;
+    if (pass == 3) {
+        A.sched_kept = false;
+        if (int rc = A.build_now(MASK_ALL)) return rc;  // held builds settled before timing
+    }
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipError_t e = hipEventCreate(&ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
@@ -1487,7 +1593,8 @@ int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch)
         for (int i = 0; i < reps; ++i) {
             if (pass == 0) launch_density(V, f, MASK_ALL, A.unroll, st);
             else if (pass == 1) launch_grad(V, f, MASK_ALL, 0, A.unroll, st);
-            else launch_rates(V, f, MASK_ALL, A.rates_unroll(), st);
+            else if (pass == 2) launch_rates(V, f, MASK_ALL, A.rates_unroll(), st);
+            else A.enqueue_build(MASK_ALL);
         }
         e = hipGetLastError();
     }
@@ -1917,6 +2024,57 @@ int sph_box_set_pass_block(sph_box *box, int block)
     if (!box) return fail(SPH_ERR_ARG, "null box");
     if (int rc = set_pass_block(box->ar, block)) return rc;
     return box->ar.prepare_graphs();
+}
+
+// Entry width of the lists that the boxes' next builds write: 16 (slot
+// offsets; a box whose offsets do not fit falls back to 32 by itself) or 32.
+// The lists in memory keep their form until their box's next build.
+static int set_list_bits(Arena &A, int bits)
+{
+    if (bits != 16 && bits != 32) return fail(SPH_ERR_ARG, "list entry bits must be 16 or 32");
+    HIPCHK(hipSetDevice(A.ctx->device));
+    A.list_bits = bits;
+    for (size_t b = 0; b < A.boxes.size(); ++b)
+        if (A.box_live[b]) A.boxes[b].build16 = bits == 16 ? 1 : 0;
+    if (!A.boxes.empty())
+        HIPCHK(hipMemcpyAsync(A.d_boxes, A.boxes.data(), sizeof(BoxDesc) * A.boxes.size(), hipMemcpyHostToDevice,
+                              A.ctx->stream));
+    return SPH_OK;
+}
+
+static int list_bits_of(Arena &A, int b, int *bits)
+{
+    if (!bits) return fail(SPH_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(A.ctx->device));
+    if (int rc = A.read_dyn()) return rc;
+    *bits = A.h_dyn[b].list16 ? 16 : 32;
+    return SPH_OK;
+}
+
+int sph_arena_set_list_bits(sph_arena *ar, int bits)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    return set_list_bits(ar->ar, bits);
+}
+
+int sph_box_set_list_bits(sph_box *box, int bits)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    return set_list_bits(box->ar, bits);
+}
+
+int sph_arena_list_bits(sph_arena *ar, int box_id, int *bits)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    Arena &A = ar->ar;
+    if (box_id < 0 || box_id >= (int)A.boxes.size() || !A.box_live[box_id]) return fail(SPH_ERR_ARG, "no such box");
+    return list_bits_of(A, box_id, bits);
+}
+
+int sph_box_list_bits(sph_box *box, int *bits)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    return list_bits_of(box->ar, box->b, bits);
 }
 
 int sph_arena_set_rates_unroll(sph_arena *ar, int unroll)

@@ -9,11 +9,15 @@
 //                    integer time from the last rates; every pass reads these
 //   last rates       acc (double2), udot
 //   sorted scratch   *_s copies written by the permutation of a build
-//   neighbour list   tiles of 64 slots in groups of 4 entries: position P of
-//                    slot s at ((s/64)*(lcap/4) + P/4)*256 + (s%64)*4 + P%4
-//                    (uint32 slot), cnt[s] = (short, long): short entries
+//   neighbour list   tiles of 64 slots in groups of 4 entries, short entries
 //                    (r < h + skin at build) at P = 0.., long entries
-//                    (h + skin <= r < r_cut + skin) at P = lcap-1 down
+//                    (h + skin <= r < r_cut + skin) at P = lcap-1 down;
+//                    16-bit slot offsets (halfword ((s/64)*(lcap/4)*2 +
+//                    P/4)*256 + (s%64)*4 + P%4) or, for a box whose offsets
+//                    do not fit, 32-bit slots (word ((s/64)*(lcap/4) +
+//                    P/4)*256 + (s%64)*4 + P%4); cnt[s] = (short, long)
+//                    counts, bit 31 of the short count set when an offset is
+//                    taken modulo the box's slot count
 //   derived          rho (double2: rho, rho_lr), T, S (double4: Sxx, Sxy,
 //                    Syy = (p I + Pi)/rho^2, and m), Q (double2: q/rho^2),
 //                    vt (double2: m/rho, T), aco / uco (cohesive rates)
@@ -66,7 +70,8 @@ struct BoxDesc {
     int precision;
     int lolong;                    // FP32C: the long-range walk reads the low position bits too
                                    // (sides over FP32C_HI_ONLY_MAX_L)
-    int pad1, pad2;
+    int build16;                   // builds write 16-bit list entries (0: 32-bit slots)
+    int pad2;
 };
 
 // Dynamic per-box state (device-owned; the host writes single entries
@@ -86,6 +91,10 @@ struct BoxDyn {
     unsigned long long dmax2;  // largest squared displacement of the box's slots in
                                // waves shared with other boxes, gathered on its review
                                // step (double bits; cleared when read)
+    int list16;     // the lists in memory hold 16-bit entries (set by each build)
+    int relist;     // a build met a neighbour offset beyond 16 bits: the box holds
+                    // (no pass, integration or bookkeeping) until the runtime has
+                    // switched it to 32-bit entries and cleared this
 };
 
 // Segment of the slot space: a box or a free range, sorted by offset.

@@ -50,25 +50,74 @@ __device__ __forceinline__ int xcd_block()
 
 __device__ __forceinline__ int wrapi(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
+// Wave-uniform values: a value the lanes agree on, moved to a scalar
+// register (readfirstlane).
+__device__ __forceinline__ double wave_uniform(double x)
+{
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ float wave_uniform(float x)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // Neighbour lists in tiles of 64 slots (one wavefront), entries in groups
 // of four.  Short entry k sits at position P = k, long entry k at
-// P = lcap - 1 - k (lcap a multiple of 4), and position P of slot s at word
-//   ((s / 64) * (lcap / 4) + P / 4) * 256 + (s % 64) * 4 + P % 4.
-// A slot's four entries of a group are one 16-byte word and a wave's 64
-// groups of one row are 1 KB contiguous: the walks fetch up to four entries
-// per lane with one load and the build stores four at a time.  Everything a
-// wave reads or writes lies in one 256 * lcap byte tile.
+// P = lcap - 1 - k (lcap a multiple of 8).  A tile is lcap / 4 rows of 64
+// 16-byte cells, cell (h, s % 64) belonging to slot s, in one of two forms:
+//  * 32-bit: the neighbour's slot, position P in cell P / 4, word P % 4
+//    (word ((s / 64) * (lcap / 4) + P / 4) * 256 + (s % 64) * 4 + P % 4);
+//  * 16-bit (the default): the neighbour's slot offset from the slot
+//    itself, taken modulo the box's slot count n when the plain difference
+//    does not fit (cnt[s].x bit 31 marks a slot holding such an entry),
+//    position P in cell P / 8, halfword P % 8, so a slot's entries fill
+//    the first lcap / 8 of its cells.
+// Either way a slot writes and reads its own cells only, so boxes of both
+// forms may share a tile.
+// Slots are in cell order, so a neighbour lies within about two rows of
+// cells of the slot in slot order, or as far from the box's other end (the
+// periodic image): offsets fit 16 bits unless two rows of cells hold more
+// than ~32k particles.  A build that meets a neighbour whose offset does not
+// fit holds its box for that step (BoxDyn::relist) and the runtime switches
+// the box to the 32-bit form, with the same entries in the same order.
+// The walks fetch up to four entries per lane with one load and the build
+// stores four at a time.
 static constexpr int LIST_TILE = 64;
 static constexpr int LIST_GROUP = 4;
-static constexpr int LIST_ROW = LIST_TILE * LIST_GROUP;  // words per group row of a tile
+static constexpr int LIST_ROW = LIST_TILE * LIST_GROUP;  // entries per group row of a tile
+static constexpr int CNT_WRAP = (int)0x80000000;       // cnt.x: a 16-bit entry is taken modulo n
+static constexpr int CNT_MASK = 0x7fffffff;
+
+__device__ __forceinline__ size_t list_tile_words(const ArenaView &A, int s)
+{
+    return (size_t)(s / LIST_TILE) * (A.lcap / LIST_GROUP) * LIST_ROW;
+}
 
 __device__ __forceinline__ unsigned *list_base(const ArenaView &A, int s)
 {
-    return A.nbr + (size_t)(s / LIST_TILE) * (A.lcap / LIST_GROUP) * LIST_ROW + (s % LIST_TILE) * LIST_GROUP;
+    return A.nbr + list_tile_words(A, s) + (s % LIST_TILE) * LIST_GROUP;
 }
 
+__device__ __forceinline__ unsigned short *list_base16(const ArenaView &A, int s)
+{
+    return (unsigned short *)(A.nbr + list_tile_words(A, s)) + (s % LIST_TILE) * (2 * LIST_GROUP);
+}
+
+// Halfword of position P in a slot's 16-bit list (list_base16): cell P / 8
+// (cells CELL16 halfwords apart: consecutive rows of the tile), halfword P % 8.
+// (Cells on every other row, spread over the tile as the 32-bit form's,
+// measured the same.)
+static constexpr int CELL16 = 2 * LIST_ROW;
+__device__ __forceinline__ size_t pos16(int P) { return (size_t)(P >> 3) * CELL16 + (P & 7); }
+
 // Entries at positions P0 .. P0 + U - 1 of a slot (U = 1, 2 or 4, P0 a
-// multiple of U): one load.
+// multiple of U): one load.  32-bit form: one entry per word of w.
 template <int U>
 __device__ __forceinline__ void load_entries(const unsigned *base, int P0, unsigned (&w)[U])
 {
@@ -84,39 +133,156 @@ __device__ __forceinline__ void load_entries(const unsigned *base, int P0, unsig
     }
 }
 
+// 16-bit form: entry i of the U in the low (i even) or high (i odd) half
+// of w[i / 2].
+template <int U>
+__device__ __forceinline__ void load_entries16(const unsigned short *base, int P0, unsigned (&w)[U])
+{
+    const unsigned short *p = base + pos16(P0);
+    if constexpr (U == 4) {
+        const uint2 g = *(const uint2 *)p;
+        w[0] = g.x; w[1] = g.y;
+    } else if constexpr (U == 2) {
+        w[0] = *(const unsigned *)p;
+    } else {
+        w[0] = *p;
+    }
+}
+
+// Entry form of a wave's lists, one of
+//   LF32: 32-bit slots;
+//   LF16: 16-bit offsets whose plain sum with the slot is the neighbour;
+//   LF16W: 16-bit offsets of which some lane's are taken modulo n (the
+//          sum is wrapped into the box's slot range [lo, lo + n)).
+// The walks are instantiated per form; list_fmt picks the wave's.
+enum : int { LF32 = 0, LF16 = 1, LF16W = 2 };
+
+struct ListFmt {
+    int lo, hi, nb;  // the box's slot range [lo, hi), nb = hi - lo
+    int form;
+};
+
+// The lists of the box bu that the wave's active lanes belong to.
+__device__ __forceinline__ ListFmt list_fmt(const ArenaView &A, int bu, int s)
+{
+    ListFmt f;
+    f.lo = wave_uniform(A.boxes[bu].part_off);
+    f.nb = wave_uniform(A.boxes[bu].n);
+    f.hi = f.lo + f.nb;
+    const bool w16 = wave_uniform(A.dyn[bu].list16) != 0;
+    f.form = !w16 ? LF32 : (__ballot(A.cnt[s].x < 0) != 0ull ? LF16W : LF16);
+    return f;
+}
+
+// Calls f with the wave's form as a compile-time constant.
+template <class Fn>
+__device__ __forceinline__ void with_form(const ListFmt &L, Fn f)
+{
+    if (L.form == LF16) f(std::integral_constant<int, LF16>());
+    else if (L.form == LF16W) f(std::integral_constant<int, LF16W>());
+    else f(std::integral_constant<int, LF32>());
+}
+
 // A slot's short (LONG = false) or long entries, U per round (k0 a multiple
-// of U; long entry k sits at position lcap - 1 - k).  take(k0, t) gives the
-// slots of round k0.  With AHEAD, next(k0) issues the load of the next
-// round's group, so that list fetch is in flight while round k0's neighbour
-// records are gathered and summed; at the last round it re-reads the current
-// group, so every load stays inside the slot's list rows.  Without AHEAD,
-// take() loads its own round's group.  Entries past the count read as s (a
-// valid record whose terms the callers mask off), so the gathers need no
-// branch.  AHEAD pays on the density walks (~15 rounds per slot); on the
-// short walks of the gradient and rate passes (3-4 rounds) its extra load
-// and registers cost more than the overlap gains.
-template <int U, bool LONG, bool AHEAD>
+// of U; long entry k sits at position lcap - 1 - k), in list form LF.
+// take(k0, t) gives the slots of round k0.  With AHEAD, next(k0) issues the
+// load of the next round's group, so that list fetch is in flight while
+// round k0's neighbour records are gathered and summed; at the last round it
+// re-reads the current group, so every load stays inside the slot's list
+// rows.  Without AHEAD, take() loads its own round's group.  Entries past the
+// count read as s (a valid record whose terms the callers mask off), so the
+// gathers need no branch.  AHEAD pays on the density walks (~15 rounds per
+// slot); on the short walks of the gradient and rate passes (3-4 rounds) its
+// extra load and registers cost more than the overlap gains.
+template <int U, bool LONG, bool AHEAD, int LF>
 struct ListRounds {
+    // 16-bit entries four per round: one 16-byte cell (8 entries) serves a
+    // pair of rounds, so a cell load every other round (issued, with AHEAD,
+    // two rounds before its use); c holds the current cell
+    static constexpr bool PAIRS = LF != LF32 && U == 4;
     const unsigned *base;
-    int lcap, n, s;
+    const unsigned short *base16;
+    int lcap, n, s, lo, hi, nb;
     unsigned w[U];
-    __device__ __forceinline__ ListRounds(const unsigned *b, int lcap_, int n_, int s_)
-        : base(b), lcap(lcap_), n(n_), s(s_)
+    unsigned c[PAIRS ? 4 : 1];
+    __device__ __forceinline__ ListRounds(const ArenaView &A, int s_, int n_, const ListFmt &L)
+        : lcap(A.lcap), n(n_), s(s_), lo(L.lo), hi(L.hi), nb(L.nb)
     {
-        if (AHEAD) load_entries<U>(base, pos(0), w);
+        if (LF == LF32) base = list_base(A, s_);
+        else base16 = list_base16(A, s_);
+        if (AHEAD) {
+            if constexpr (PAIRS) load_cell(cell(0), w);
+            else load(pos(0));
+        }
     }
     __device__ __forceinline__ int pos(int k0) const { return LONG ? lcap - k0 - U : k0; }
+    // the cell of round k0's entries (lcap a multiple of 8)
+    __device__ __forceinline__ int cell(int k0) const { return LONG ? (lcap >> 3) - 1 - (k0 >> 3) : (k0 >> 3); }
+    __device__ __forceinline__ void load_cell(int q, unsigned (&d)[4]) const
+    {
+        const uint4 g = *(const uint4 *)(base16 + (size_t)q * CELL16);
+        d[0] = g.x; d[1] = g.y; d[2] = g.z; d[3] = g.w;
+    }
+    __device__ __forceinline__ void load(int P0)
+    {
+        if constexpr (LF == LF32) load_entries<U>(base, P0, w);
+        else load_entries16<U>(base16, P0, w);
+    }
+    __device__ __forceinline__ int decode(unsigned x, int half) const
+    {
+        int e = s + (half ? ((int)x >> 16) : ((int)(x << 16) >> 16));
+        if constexpr (LF == LF16W) {
+            e = e < lo ? e + nb : e;
+            e = e >= hi ? e - nb : e;
+        }
+        return e;
+    }
     __device__ __forceinline__ void take(int k0, int (&t)[U])
     {
-        if (!AHEAD) load_entries<U>(base, pos(k0), w);
+        if constexpr (PAIRS) {
+            if ((k0 & 4) == 0) {  // first round of a cell
+                if constexpr (AHEAD) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) t[u] = k0 + u < n ? (int)w[LONG ? U - 1 - u : u] : s;
+                    for (int i = 0; i < 4; ++i) c[i] = w[i];
+                } else {
+                    load_cell(cell(k0), c);
+                }
+            }
+            // short rounds take offsets 0-3 then 4-7 of the cell, long
+            // rounds 7-4 then 3-0 (the half picked by selects: a register
+            // array indexed at run time would live in scratch memory)
+            const bool hi4 = LONG ? ((k0 & 4) == 0) : ((k0 & 4) != 0);
+            const unsigned x0 = hi4 ? c[2] : c[0], x1 = hi4 ? c[3] : c[1];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int o = LONG ? 3 - u : u;  // offset within the half
+                t[u] = k0 + u < n ? decode(o < 2 ? x0 : x1, o & 1) : s;
+            }
+            return;
+        } else {
+            if (!AHEAD) load(pos(k0));
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = LONG ? U - 1 - u : u;
+                int e;
+                if constexpr (LF == LF32) e = (int)w[i];
+                else e = decode(w[i >> 1], i & 1);
+                t[u] = k0 + u < n ? e : s;
+            }
+        }
     }
     // issued after round k0's gathers, so waiting for them does not wait for
-    // this load too (loads retire in order)
+    // this load too (loads retire in order); at the last round it re-reads
+    // the current group or cell, so every load stays inside the slot's rows
     __device__ __forceinline__ void next(int k0)
     {
-        if (AHEAD) load_entries<U>(base, pos(k0 + U < n ? k0 + U : k0), w);
+        if constexpr (AHEAD) {
+            if constexpr (PAIRS) {
+                if ((k0 & 4) == 0) load_cell(cell(k0 + 8 < n ? k0 + 8 : k0), w);
+            } else {
+                load(pos(k0 + U < n ? k0 + U : k0));
+            }
+        }
     }
 };
 
@@ -206,23 +372,6 @@ __device__ __forceinline__ void fix_rec(double2 r, double2 v, const BoxDesc &B, 
                      (float)v.y);
     lo = (unsigned)(X & 0xffffull) | ((unsigned)(Y & 0xffffull) << 16);
 }
-
-// Wave-uniform values: a value the lanes agree on, moved to a scalar
-// register (readfirstlane), and a waterfall over the boxes of a wave.
-__device__ __forceinline__ double wave_uniform(double x)
-{
-    const long long v = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffll));
-    const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-__device__ __forceinline__ float wave_uniform(float x)
-{
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
-}
-
-__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // Runs f(bu) once for each distinct box bu among the active lanes, on the
 // lanes of that box, with bu wave-uniform (a waterfall over the boxes of the
@@ -614,20 +763,34 @@ __global__ __launch_bounds__(BS_THREADS) void k_box_sort(ArenaView A, int force_
 // cell, in stencil order and slot order within a cell (the list order).
 // Four candidate positions are loaded before any is tested; SINGLE: a box
 // with a single-cell axis, whose candidates need the minimum image.
-// Entries collect in two four-entry buffers (short from position 0 up, long
-// from lcap - 1 down) stored a group at a time; the partial groups are
-// stored at the end, merged when they share a group.  Entries that do not
-// fit (ns + nl = lcap) set lost.  The buffers are the lane's column of an
-// LDS array (buf[row * NT]), so an append is one indexed LDS write on one
-// path for short and long entries; in registers each append took a select
-// chain per buffer, and the wave ran both chains whenever its lanes split.
-// Positions past a count hold stale words, which the walks never read
-// (ListRounds maps them to the slot itself).
-template <bool SINGLE>
+// Entries collect in two G-entry buffers (short from position 0 up, long
+// from lcap - 1 down), each group stored with one 16-byte store when its
+// last offset fills: G = 4 slots, or with L16 G = 8 offsets of the 16-bit
+// form.  The partial groups are stored at the end, merged when they share a
+// group.  Entries that do not fit (ns + nl = lcap) set lost.  The buffers
+// are the lane's column of an LDS array (words buf[w * NT], w = 0-3 short,
+// 4-7 long; halfwords in the 16-bit form), so an append is one indexed LDS
+// write on one path for short and long entries; in registers each append
+// took a select chain per buffer, and the wave ran both chains whenever its
+// lanes split.  Positions past a count hold stale words, which the walks
+// never read (ListRounds maps them to the slot itself).
+//
+// 16-bit offsets: a stencil cell is a contiguous slot range, so one base
+// serves all its candidates: the slot offset t - s when the cell's offsets
+// all fit, else taken modulo n (the box's slot count) to the representative
+// nearest 0 of the cell's middle; the cell's first and last offsets then
+// decide whether all fit (else over), and a cell taken modulo n sets
+// wrapped.  Boxes of up to 32,768 slots never wrap, and a box of n <= 65,536
+// slots never overflows while its cells hold fewer than (65,536 - n) / 2
+// slots each.  The candidate loop adds one constant per entry, as the
+// 32-bit form stores the slot itself.
+template <bool SINGLE, bool L16>
 __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu, int s, double2 ri, int cell,
-                                          unsigned *lb, unsigned *buf, int &ns, int &nl, bool &lost)
+                                          unsigned *buf, int &ns, int &nl, bool &lost, bool &wrapped, bool &over)
 {
+    constexpr int G = L16 ? 8 : 4;  // entries per group store
     const int lcap = A.lcap;
+    const int nb = wave_uniform(Bu.n);
     const double rnl2 = wave_uniform(Bu.rnl2), hsk2 = wave_uniform(Bu.hsk2);
     const double Lx = wave_uniform(Bu.Lx), Ly = wave_uniform(Bu.Ly);
     const double hLx = wave_uniform(Bu.hLx), hLy = wave_uniform(Bu.hLy);
@@ -635,6 +798,14 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
     const int sx = wave_uniform(Bu.sx), sy = wave_uniform(Bu.sy);
     const int *cs = A.cell_start + wave_uniform(Bu.key_base);
     const int cx = cell % ncx, cy = cell / ncx;
+    // the list address is formed at each store (not held across the loop)
+    auto put = [&](int grp, uint4 g) {
+        if (L16) *(uint4 *)(list_base16(A, s) + (size_t)grp * CELL16) = g;
+        else *(uint4 *)(list_base(A, s) + (size_t)grp * LIST_ROW) = g;
+    };
+    auto store = [&](int grp, int w0) {
+        put(grp, make_uint4(buf[w0 * NT], buf[(w0 + 1) * NT], buf[(w0 + 2) * NT], buf[(w0 + 3) * NT]));
+    };
     for (int oy = -sy; oy <= sy; ++oy) {
         const int yr = cy + oy;
         const int yy = yr < 0 ? yr + ncy : (yr >= ncy ? yr - ncy : yr);
@@ -645,6 +816,15 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
             const double rx = ri.x - (xr < 0 ? -Lx : (xr >= ncx ? Lx : 0.0));
             const int c = yy * ncx + xx;
             const int t0 = cs[c], t1 = cs[c + 1];
+            int ks = 0;  // 16-bit entry of candidate t: t + ks
+            if (L16 && t1 > t0) {
+                const int dlo = t0 - s, dhi = t1 - 1 - s, dm = (dlo + dhi) >> 1;
+                int k = 0;  // the plain offsets when they fit, else the nearest modulo n
+                if (dlo < -32768 || dhi > 32767) k = 2 * dm > nb ? -nb : (2 * dm < -nb ? nb : 0);
+                if (dlo + k < -32768 || dhi + k > 32767) over = true;
+                if (k) wrapped = true;
+                ks = k - s;
+            }
             for (int t = t0; t < t1; t += 4) {
                 double2 rj[4];
 #pragma unroll
@@ -665,40 +845,43 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
                         if (ns + nl >= lcap) {
                             lost = true;
                         } else {
-                            // short entry ns at offset ns % 4 of buffer row 0-3, long
-                            // entry nl (position lcap - 1 - nl) at offset 3 - nl % 4
-                            // of rows 4-7; one path for both, the group stored
-                            // when its last offset fills
+                            // short entry ns at offset ns % G (buffer slot
+                            // 0 .. G-1), long entry nl (position lcap - 1 - nl)
+                            // at offset G - 1 - nl % G (slot G .. 2G-1); one
+                            // path for both, the group stored when its last
+                            // offset fills
                             const bool sh = r2 < hsk2;
-                            const int row = sh ? (ns & 3) : 7 - (nl & 3);
-                            buf[row * NT] = (unsigned)(t + u);
+                            const int row = sh ? (ns & (G - 1)) : (2 * G - 1) - (nl & (G - 1));
+                            if (L16) ((unsigned short *)(buf + (row >> 1) * NT))[row & 1] = (unsigned short)(t + u + ks);
+                            else buf[row * NT] = (unsigned)(t + u);
                             ns += sh ? 1 : 0;
                             nl += sh ? 0 : 1;
-                            if (row == 3 || row == 4) {
-                                const int g0 = sh ? 0 : 4;
-                                const int grp = sh ? ((ns - 4) >> 2) : ((lcap - nl) >> 2);
-                                *(uint4 *)(lb + (size_t)grp * LIST_ROW) =
-                                    make_uint4(buf[g0 * NT], buf[(g0 + 1) * NT], buf[(g0 + 2) * NT], buf[(g0 + 3) * NT]);
-                            }
+                            if (row == G - 1 || row == G)
+                                store(sh ? (ns - G) / G : (lcap - nl) / G, sh ? 0 : 4);
                         }
                     }
                 }
             }
         }
     }
-    // partial groups: short offsets [0, ns % 4) of group ns / 4, long
-    // offsets [4 - nl % 4, 4) of group (lcap - nl) / 4; with ns + nl <= lcap
-    // they never overlap, and a shared group is stored once
-    const int ps = ns & 3, pl = nl & 3;
-    const int gs = ns >> 2, gl = (lcap - nl) >> 2;
-    const unsigned s0 = buf[0], s1 = buf[NT], s2 = buf[2 * NT], s3 = buf[3 * NT];
-    const unsigned l0 = buf[4 * NT], l1 = buf[5 * NT], l2 = buf[6 * NT], l3 = buf[7 * NT];
+    // partial groups: short offsets [0, ns % G) of group ns / G, long
+    // offsets [G - nl % G, G) of group (lcap - nl) / G; with ns + nl <= lcap
+    // they never overlap, and a shared group is stored once, each offset
+    // taken from the list that owns it
+    const int ps = ns & (G - 1), pl = nl & (G - 1);
+    const int gs = ns / G, gl = (lcap - nl) / G;
     if (ps && pl && gs == gl) {
-        *(uint4 *)(lb + (size_t)gs * LIST_ROW) =
-            make_uint4(0 < ps ? s0 : l0, 1 < ps ? s1 : l1, 2 < ps ? s2 : l2, 3 < ps ? s3 : l3);
+        unsigned w[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const unsigned sw = buf[i * NT], lw = buf[(4 + i) * NT];
+            if (L16) w[i] = ((2 * i < ps ? sw : lw) & 0xffffu) | ((2 * i + 1 < ps ? sw : lw) & 0xffff0000u);
+            else w[i] = i < ps ? sw : lw;
+        }
+        put(gs, make_uint4(w[0], w[1], w[2], w[3]));
     } else {
-        if (ps) *(uint4 *)(lb + (size_t)gs * LIST_ROW) = make_uint4(s0, s1, s2, s3);
-        if (pl) *(uint4 *)(lb + (size_t)gl * LIST_ROW) = make_uint4(l0, l1, l2, l3);
+        if (ps) store(gs, 0);
+        if (pl) store(gl, 4);
     }
 }
 
@@ -734,19 +917,29 @@ __device__ __forceinline__ void build_slot(const ArenaView &A, int s, int force_
     A.r_ref[s] = ri;
     const int cell = A.pcell[s];
     if (fp32) fix_rec(ri, vpi, B, A.rf[s], A.rlo[s]);
-    unsigned *lb = list_base(A, s);
     int ns = 0, nl = 0;
-    bool lost = false;
+    bool lost = false, wrapped = false, over = false;
     // Box constants wave-uniform (scalar registers); each stencil cell's
     // periodic image is applied to r_i once per cell, so the candidate loop
     // needs no minimum image (axes with a single cell keep it).
     per_box(b, [&](int bu) {
         const BoxDesc &Bu = A.boxes[bu];
-        if (wave_uniform(Bu.sx) == 0 || wave_uniform(Bu.sy) == 0) list_walk<true>(A, Bu, s, ri, cell, lb, buf, ns, nl, lost);
-        else list_walk<false>(A, Bu, s, ri, cell, lb, buf, ns, nl, lost);
+        const bool single = wave_uniform(Bu.sx) == 0 || wave_uniform(Bu.sy) == 0;
+        if (wave_uniform(Bu.build16)) {
+            if (single) list_walk<true, true>(A, Bu, s, ri, cell, buf, ns, nl, lost, wrapped, over);
+            else list_walk<false, true>(A, Bu, s, ri, cell, buf, ns, nl, lost, wrapped, over);
+        } else {
+            if (single) list_walk<true, false>(A, Bu, s, ri, cell, buf, ns, nl, lost, wrapped, over);
+            else list_walk<false, false>(A, Bu, s, ri, cell, buf, ns, nl, lost, wrapped, over);
+        }
     });
     if (lost) flag_health(A.dyn, b, HEALTH_LISTCAP);
-    A.cnt[s] = make_int2(ns, nl);
+    // an offset beyond 16 bits: the box holds this step (every pass, the
+    // integration and the bookkeeping skip it) until the runtime has
+    // switched it to 32-bit entries; its next step builds again
+    if (over) D.relist = 1;
+    if (s == B.part_off) D.list16 = B.build16;  // the form the walks read
+    A.cnt[s] = make_int2(ns | (wrapped ? CNT_WRAP : 0), nl);
     max_bits(&D.vmax2, vpi.x * vpi.x + vpi.y * vpi.y);
     max_bits(&D.amax2, ai.x * ai.x + ai.y * ai.y);
 }
@@ -754,8 +947,9 @@ __device__ __forceinline__ void build_slot(const ArenaView &A, int s, int force_
 // Over the building tiles (for_building_tiles).
 __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask, int fp32)
 {
-    // each lane's two four-entry groups being filled (short, long), one
-    // column per lane: row r of lane t at lbuf[r * NT + t], bank-conflict free
+    // each lane's two groups being filled (short, long: four words each),
+    // one column per lane: word w of lane t at lbuf[w * NT + t],
+    // bank-conflict free
     __shared__ unsigned lbuf[8 * NT];
     for_building_tiles(A, [&](int s) { build_slot(A, s, force_mask, fp32, lbuf + threadIdx.x); });
 }
@@ -853,12 +1047,12 @@ struct Dens64 {
 
 // P1 list walk.  COH: accumulate the cohesive terms (needs velocities);
 // UNIM: all masses equal (no mass fetch).
-template <int U64, bool COH, bool UNIM, bool WRAP>
-__device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK &K, int s)
+template <int U64, bool COH, bool UNIM, bool WRAP, int LF>
+__device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK &K, const ListFmt &L, int s)
 {
     const double2 ri = A.r[s], vi = A.vp[s];
-    const int2 c = A.cnt[s];
-    const unsigned *nb = list_base(A, s);
+    const int2 cr = A.cnt[s];
+    const int2 c = make_int2(cr.x & CNT_MASK, cr.y);
     // the sums leave out the kernel constants (and, with UNIM, the mass);
     // they scale the totals at the end
     Dens64 d;
@@ -866,7 +1060,7 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
     // U64 entries per round: all their loads are issued before any use so
     // several neighbour fetches are in flight per wavefront; one load brings
     // a round's list entries.
-    ListRounds<U64, false, true> rounds(nb, A.lcap, c.x, s);
+    ListRounds<U64, false, true, LF> rounds(A, s, c.x, L);
     for (int k0 = 0; k0 < c.x; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64];
@@ -906,7 +1100,7 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
     }
     // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
     // read the same row at step k (coalesced), whatever their counts
-    ListRounds<U64, true, true> lrounds(nb, A.lcap, c.y, s);
+    ListRounds<U64, true, true, LF> lrounds(A, s, c.y, L);
     for (int k0 = 0; k0 < c.y; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64];
@@ -965,18 +1159,22 @@ __global__ __launch_bounds__(PASS_NT_MAX) void k_density64(ArenaView A, int stat
     const int b = A.slot_box[s];
     if (b < 0) return;
     const int st = A.dyn[b].state;
-    if (!state_in(st, state_mask)) return;
+    if (!state_in(st, state_mask) || A.dyn[b].relist) return;
     Dens64 d;
     per_box(b, [&](int bu) {
         const BoxK K = box_k(A.boxes[bu], true);
-        const bool coh = (K.terms & 2) != 0, unim = K.munif > 0.0;
-        if (wave_interior(A.r[s], K)) {
-            if (coh) d = unim ? density64_walk<U64, true, true, false>(A, K, s) : density64_walk<U64, true, false, false>(A, K, s);
-            else d = unim ? density64_walk<U64, false, true, false>(A, K, s) : density64_walk<U64, false, false, false>(A, K, s);
-        } else {
-            if (coh) d = unim ? density64_walk<U64, true, true, true>(A, K, s) : density64_walk<U64, true, false, true>(A, K, s);
-            else d = unim ? density64_walk<U64, false, true, true>(A, K, s) : density64_walk<U64, false, false, true>(A, K, s);
-        }
+        const ListFmt L = list_fmt(A, bu, s);
+        const bool coh = (K.terms & 2) != 0, unim = K.munif > 0.0, inner = wave_interior(A.r[s], K);
+        with_form(L, [&](auto lf) {
+            constexpr int F = decltype(lf)::value;
+            if (inner) {
+                if (coh) d = unim ? density64_walk<U64, true, true, false, F>(A, K, L, s) : density64_walk<U64, true, false, false, F>(A, K, L, s);
+                else d = unim ? density64_walk<U64, false, true, false, F>(A, K, L, s) : density64_walk<U64, false, false, false, F>(A, K, L, s);
+            } else {
+                if (coh) d = unim ? density64_walk<U64, true, true, true, F>(A, K, L, s) : density64_walk<U64, true, false, true, F>(A, K, L, s);
+                else d = unim ? density64_walk<U64, false, true, true, F>(A, K, L, s) : density64_walk<U64, false, false, true, F>(A, K, L, s);
+            }
+        });
     });
     const BoxDesc &B = A.boxes[b];
     const bool coh = (B.terms & 2u) != 0;
@@ -1011,16 +1209,15 @@ struct Grad64 {
     double g00, g01, g10, g11, gt0, gt1;
 };
 
-template <int U64, bool WRAP>
-__device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K, int s)
+template <int U64, bool WRAP, int LF>
+__device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K, const ListFmt &L, int s)
 {
     const double2 ri = A.r[s], vi = A.vp[s];
     const double Ti = A.T[s];
-    const int ns = A.cnt[s].x;
-    const unsigned *nb = list_base(A, s);
+    const int ns = A.cnt[s].x & CNT_MASK;
     Grad64 g;
     g.g00 = g.g01 = g.g10 = g.g11 = g.gt0 = g.gt1 = 0.0;
-    ListRounds<U64, false, false> rounds(nb, A.lcap, ns, s);
+    ListRounds<U64, false, false, LF> rounds(A, s, ns, L);
     for (int k0 = 0; k0 < ns; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64], vt[U64];
@@ -1068,13 +1265,18 @@ __global__ __launch_bounds__(PASS_NT_MAX) void k_grad64(ArenaView A, int state_m
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
-    if (!state_in(A.dyn[b].state, state_mask)) return;
+    if (!state_in(A.dyn[b].state, state_mask) || A.dyn[b].relist) return;
     const BoxDesc &B = A.boxes[b];
     if (!diag && !(B.terms & 12u)) return;
     Grad64 g;
     per_box(b, [&](int bu) {
         const BoxK K = box_k(A.boxes[bu], true);
-        g = wave_interior(A.r[s], K) ? grad64_walk<U64, false>(A, K, s) : grad64_walk<U64, true>(A, K, s);
+        const ListFmt L = list_fmt(A, bu, s);
+        const bool inner = wave_interior(A.r[s], K);
+        with_form(L, [&](auto lf) {
+            constexpr int F = decltype(lf)::value;
+            g = inner ? grad64_walk<U64, false, F>(A, K, L, s) : grad64_walk<U64, true, F>(A, K, L, s);
+        });
     });
     const double Ti = A.T[s];
     const double rho = A.rho[s].x, r2i = rho * rho;
@@ -1103,16 +1305,15 @@ struct Rate64 {
     double ax, ay, ud;
 };
 
-template <int U64, bool COND, bool WRAP>
-__device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K, int s, int ns)
+template <int U64, bool COND, bool WRAP, int LF>
+__device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K, const ListFmt &L, int s, int ns)
 {
     const double2 ri = A.r[s], vi = A.vp[s];
     const double4 Si = A.S[s];
     const double2 Qi = COND ? A.Q[s] : make_double2(0.0, 0.0);
-    const unsigned *nb = list_base(A, s);
     Rate64 o;
     o.ax = o.ay = o.ud = 0.0;
-    ListRounds<U64, false, false> rounds(nb, A.lcap, ns, s);
+    ListRounds<U64, false, false, LF> rounds(A, s, ns, L);
     for (int k0 = 0; k0 < ns; k0 += U64) {
         int t[U64];
         double2 rj[U64], vv[U64], Qj[U64];
@@ -1162,16 +1363,21 @@ __global__ __launch_bounds__(PASS_NT_MAX) void k_rates64(ArenaView A, int state_
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
-    if (!state_in(A.dyn[b].state, state_mask)) return;
+    if (!state_in(A.dyn[b].state, state_mask) || A.dyn[b].relist) return;
     const BoxDesc &B = A.boxes[b];
-    const int ns = (B.terms & 13u) ? A.cnt[s].x : 0;
+    const int ns = (B.terms & 13u) ? (A.cnt[s].x & CNT_MASK) : 0;
     Rate64 o;
     per_box(b, [&](int bu) {
         const BoxK K = box_k(A.boxes[bu], true);
-        if (wave_interior(A.r[s], K))
-            o = (K.terms & 8) ? rates64_walk<U64, true, false>(A, K, s, ns) : rates64_walk<U64, false, false>(A, K, s, ns);
-        else
-            o = (K.terms & 8) ? rates64_walk<U64, true, true>(A, K, s, ns) : rates64_walk<U64, false, true>(A, K, s, ns);
+        const ListFmt L = list_fmt(A, bu, s);
+        const bool inner = wave_interior(A.r[s], K), cond = (K.terms & 8) != 0;
+        with_form(L, [&](auto lf) {
+            constexpr int F = decltype(lf)::value;
+            if (inner)
+                o = cond ? rates64_walk<U64, true, false, F>(A, K, L, s, ns) : rates64_walk<U64, false, false, F>(A, K, L, s, ns);
+            else
+                o = cond ? rates64_walk<U64, true, true, F>(A, K, L, s, ns) : rates64_walk<U64, false, true, F>(A, K, L, s, ns);
+        });
     });
     const double2 ac = A.aco[s];
     const double ax = o.ax + ac.x, ay = o.ay + ac.y, ud = o.ud + A.uco[s];
@@ -1310,8 +1516,8 @@ struct Dens32 {
 // long entry rf alone (16), or rf and rlo in boxes wider than the side
 // up to which the high words keep the long-range terms within FP32 rounding
 // (LOLONG).  UNIM: all masses equal, none fetched.
-template <int U32, bool COH, bool UNIM, bool LOLONG>
-__device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F, int s, Dens32 &d)
+template <int U32, bool COH, bool UNIM, bool LOLONG, int LF>
+__device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F, const ListFmt &L, int s, Dens32 &d)
 {
     // the kernel constants (and, with UNIM, the mass) scale each round's
     // partial sums instead of every term
@@ -1319,9 +1525,9 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
     const float cfhl = UNIM ? F.fhl * F.munif : F.fhl;
     const float4 ri = A.rf[s];
     const unsigned loi = A.rlo[s];
-    const int2 c = A.cnt[s];
-    const unsigned *nb = list_base(A, s);
-    ListRounds<U32, false, true> rounds(nb, A.lcap, c.x, s);
+    const int2 cr = A.cnt[s];
+    const int2 c = make_int2(cr.x & CNT_MASK, cr.y);
+    ListRounds<U32, false, true, LF> rounds(A, s, c.x, L);
     for (int k0 = 0; k0 < c.x; k0 += U32) {
         int t[U32];
         float4 rj[U32];
@@ -1369,7 +1575,7 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
     }
     // long entries: row lcap-1-k holds the k-th, so all lanes of a wave
     // read the same row at step k (coalesced), whatever their counts
-    ListRounds<U32, true, true> lrounds(nb, A.lcap, c.y, s);
+    ListRounds<U32, true, true, LF> lrounds(A, s, c.y, L);
     for (int k0 = 0; k0 < c.y; k0 += U32) {
         int t[U32];
         float4 rj[U32];
@@ -1422,19 +1628,24 @@ __global__ __launch_bounds__(PASS_NT_MAX) void k_density32(ArenaView A, int stat
     const int b = A.slot_box[s];
     if (b < 0) return;
     const int st = A.dyn[b].state;
-    if (!state_in(st, state_mask)) return;
+    if (!state_in(st, state_mask) || A.dyn[b].relist) return;
     const BoxDesc &B = A.boxes[b];
     const bool coh = (B.terms & 2u) != 0;
     Dens32 d;
     per_box(b, [&](int bu) {
         const BoxDesc &Bu = A.boxes[bu];
         const BoxF F = boxf(Bu, true);
+        const ListFmt L = list_fmt(A, bu, s);
+        const bool lolong = wave_uniform(Bu.lolong) != 0;
         d.rho = 0.0;
         d.rhol = Kahan(F.qhl);
         auto walk = [&](auto coh_t, auto unim_t) {
             constexpr bool C = decltype(coh_t)::value, M = decltype(unim_t)::value;
-            if (wave_uniform(Bu.lolong)) density32_walk<U32, C, M, true>(A, F, s, d);
-            else density32_walk<U32, C, M, false>(A, F, s, d);
+            with_form(L, [&](auto lf) {
+                constexpr int LF = decltype(lf)::value;
+                if (lolong) density32_walk<U32, C, M, true, LF>(A, F, L, s, d);
+                else density32_walk<U32, C, M, false, LF>(A, F, L, s, d);
+            });
         };
         const bool unim = F.munif > 0.f;
         if (F.terms & 2) {
@@ -1485,16 +1696,15 @@ struct Grad32 {
 // difference) left up to 4e-6 of force and heating error on relaxed liquid
 // states, where the terms cancel to a small net rate.  The temperature comes
 // from the FP64 T of the density pass.  Per neighbour: rf, rlo, m/rho, T.
-template <int U32>
-__device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, int s, Grad32 &g)
+template <int U32, int LF>
+__device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, const ListFmt &L, int s, Grad32 &g)
 {
     const float4 ri = A.rf[s];
     const unsigned loi = A.rlo[s];
     const double vix = ri.z, viy = ri.w;
     const double Ti = A.T[s];
-    const int ns = A.cnt[s].x;
-    const unsigned *nb = list_base(A, s);
-    ListRounds<U32, false, false> rounds(nb, A.lcap, ns, s);
+    const int ns = A.cnt[s].x & CNT_MASK;
+    ListRounds<U32, false, false, LF> rounds(A, s, ns, L);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32];
@@ -1542,13 +1752,14 @@ __global__ __launch_bounds__(PASS_NT_MAX) void k_grad32(ArenaView A, int state_m
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
-    if (!state_in(A.dyn[b].state, state_mask)) return;
+    if (!state_in(A.dyn[b].state, state_mask) || A.dyn[b].relist) return;
     const BoxDesc &B = A.boxes[b];
     if (!diag && !(B.terms & 12u)) return;
     Grad32 g = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
-        grad32_walk<U32>(A, F, s, g);
+        const ListFmt L = list_fmt(A, bu, s);
+        with_form(L, [&](auto lf) { grad32_walk<U32, decltype(lf)::value>(A, F, L, s, g); });
     });
     const double G00 = g.g00, G01 = g.g01, G10 = g.g10, G11 = g.g11;
     const double rho = A.rho[s].x, r2i = rho * rho;
@@ -1580,15 +1791,15 @@ struct Rate32 {
 // P3 walk of the FP32C form: FP32 neighbour records, FP64 arithmetic on the
 // exact separations (see grad32_walk).  Per neighbour: rf, rlo, (S, m) and,
 // with conduction, q/rho^2.
-template <int U32, bool COND>
-__device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, int s, int ns, Rate32 &o)
+template <int U32, bool COND, int LF>
+__device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, const ListFmt &L, int s, int ns,
+                                             Rate32 &o)
 {
     const float4 ri = A.rf[s];
     const unsigned loi = A.rlo[s];
     const float2 qi = COND ? A.q32[s] : make_float2(0.f, 0.f);
     const float4 Si = A.S32[s];
-    const unsigned *nb = list_base(A, s);
-    ListRounds<U32, false, false> rounds(nb, A.lcap, ns, s);
+    ListRounds<U32, false, false, LF> rounds(A, s, ns, L);
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32], Sj[U32];
@@ -1636,14 +1847,18 @@ __global__ __launch_bounds__(PASS_NT_MAX) void k_rates32(ArenaView A, int state_
     if (s >= A.n) return;
     const int b = A.slot_box[s];
     if (b < 0) return;
-    if (!state_in(A.dyn[b].state, state_mask)) return;
+    if (!state_in(A.dyn[b].state, state_mask) || A.dyn[b].relist) return;
     const BoxDesc &B = A.boxes[b];
-    const int ns = (B.terms & 13u) ? A.cnt[s].x : 0;
+    const int ns = (B.terms & 13u) ? (A.cnt[s].x & CNT_MASK) : 0;
     Rate32 o = {0.0, 0.0, 0.0};
     per_box(b, [&](int bu) {
         const BoxF F = boxf(A.boxes[bu], true);
-        if (F.terms & 8) rates32_walk<U32, true>(A, F, s, ns, o);
-        else rates32_walk<U32, false>(A, F, s, ns, o);
+        const ListFmt L = list_fmt(A, bu, s);
+        with_form(L, [&](auto lf) {
+            constexpr int LF = decltype(lf)::value;
+            if (F.terms & 8) rates32_walk<U32, true, LF>(A, F, L, s, ns, o);
+            else rates32_walk<U32, false, LF>(A, F, L, s, ns, o);
+        });
     });
     const double2 ac = A.aco[s];
     const double axd = o.ax + ac.x, ayd = o.ay + ac.y, udd = o.ud + A.uco[s];
@@ -1735,7 +1950,7 @@ __global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
     int b = -1;
     if (s < A.n) {
         b = A.slot_box[s];
-        if (b >= 0 && A.dyn[b].state == BOX_RUN) {
+        if (b >= 0 && A.dyn[b].state == BOX_RUN && !A.dyn[b].relist) {
             d2 = fmax(integrate_slot(A, fp32, s, b), 0.0);
             review = list_review(A.dyn[b]);
         }
@@ -1821,6 +2036,7 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
     const int b = blockIdx.x;
     if (b >= nbox) return;
     BoxDyn &D = A.dyn[b];
+    if (D.relist) return;  // held: no step taken, the next one builds again
     const int state = D.state;
     const int64_t k = D.k, K = D.nsteps;
     const bool capture = state == BOX_RUN && (k == 0 || k == K);

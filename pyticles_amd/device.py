@@ -132,16 +132,31 @@ class DeviceBox:
     def snapshot_wait(self):
         _lib.check(_lib.lib.sph_box_snapshot_wait(self._h))
 
-    def set_schedule(self, unroll, rates_unroll=0, block=0):
+    def set_schedule(self, unroll, rates_unroll=0, block=0, list_bits=0):
         """List entries in flight per lane in the pair passes (1, 2, 4; 0 =
-        default), in the rate pass alone (0 = the same) and the pair passes'
-        block size (128, 256, 512; 0 = 256).  unroll may be a schedule dict
-        from autotune.schedule_for."""
+        default), in the rate pass alone (0 = the same), the pair passes'
+        block size (128, 256, 512; 0 = 256) and the list entry width (16,
+        32; 0 = leave it).  unroll may be a schedule dict from
+        autotune.schedule_for."""
         if isinstance(unroll, dict):
-            unroll, rates_unroll, block = unroll["unroll"], unroll.get("rates_unroll", 0), unroll.get("block", 0)
+            unroll, rates_unroll, block, list_bits = (unroll["unroll"], unroll.get("rates_unroll", 0),
+                                                      unroll.get("block", 0), unroll.get("list_bits", 0))
         _lib.check(_lib.lib.sph_box_set_schedule(self._h, int(unroll)))
         _lib.check(_lib.lib.sph_box_set_rates_unroll(self._h, int(rates_unroll)))
         _lib.check(_lib.lib.sph_box_set_pass_block(self._h, int(block)))
+        if list_bits:
+            self.set_list_bits(list_bits)
+
+    def set_list_bits(self, bits):
+        """Entry width of the lists the next builds write: 16 (default;
+        falls back to 32 by itself where offsets do not fit) or 32."""
+        _lib.check(_lib.lib.sph_box_set_list_bits(self._h, int(bits)))
+
+    def list_bits(self):
+        """Entry width of the box's lists in memory (after its first build)."""
+        b = ctypes.c_int()
+        _lib.check(_lib.lib.sph_box_list_bits(self._h, ctypes.byref(b)))
+        return b.value
 
     def timing(self, reset=False):
         out = np.zeros(7)
@@ -150,14 +165,18 @@ class DeviceBox:
                          "integrate_ms"), out))
 
     PASSES = ("density", "gradients", "rates")
+    BUILD = "build"  # time_pass of the list build stage (reorders slots; not one of PASSES)
 
     def time_pass(self, name, reps=50):
         """Average microseconds of one launch of a pair pass ('density',
         'gradients' or 'rates') at the current state: events around reps
         back-to-back launches.  Writes only derived fields (the next step
-        re-derives them); the box must have been stepped."""
+        re-derives them); the box must have been stepped.  'build' times the
+        list build stage (it may reorder the slots; the next run builds
+        first)."""
         us = ctypes.c_double()
-        _lib.check(_lib.lib.sph_box_time_pass(self._h, self.PASSES.index(name), int(reps), ctypes.byref(us)))
+        idx = 3 if name == self.BUILD else self.PASSES.index(name)
+        _lib.check(_lib.lib.sph_box_time_pass(self._h, idx, int(reps), ctypes.byref(us)))
         return us.value
 
     # -- (a) neighbour list ----------------------------------------------

@@ -60,6 +60,29 @@ def test_position_reads_keep_the_run_open():
         _same(_state(b), ref)
 
 
+def test_following_runs_continue_the_list_schedule():
+    # runs that follow each other keep the box's lists and their schedule;
+    # an upload (even of the same values) drops them, so the next run
+    # builds at its first step
+    s = configs.square_system(48, 0.9, 23, 0)
+    with from_system(s) as a, from_system(s) as c:
+        a.step(5)
+        c.step(5)
+        builds_a = builds_c = 0
+        for _ in range(10):
+            a.step(3)
+            builds_a += a.run_stats()["builds"]
+            st = c.download("r", "v", "u")
+            c.upload(r=st["r"], v=st["v"], u=st["u"])
+            c.step(3)
+            builds_c += c.run_stats()["builds"]
+        assert builds_c >= 10 and builds_a < builds_c / 2, (builds_a, builds_c)
+        # the same trajectory up to the summation order of other lists
+        ga, gc = _state(a), _state(c)
+        for k in ("r", "v", "u"):
+            assert np.max(np.abs(ga[k] - gc[k])) <= 1e-9 * np.max(np.abs(gc[k])), k
+
+
 def _surface_system(ns, seed=31):
     rng = np.random.Generator(np.random.PCG64(seed))
     n = ns * ns

@@ -100,8 +100,10 @@ def test_autotune_small_workload(tmp_path):
     s = configs.square_system(32, 0.9, 10, 10)
     best, table = autotune.tune(s, _lib.FP64, "box", steps=3, warm=1, path=p)
     assert autotune.valid(best)
-    assert len(table) == len(autotune.candidates()) + len(autotune.UNROLLS) + len(autotune.BLOCKS)
+    assert len(table) == (len(autotune.candidates()) + len(autotune.UNROLLS) + len(autotune.BLOCKS)
+                          + len(autotune.LIST_BITS))
     assert best["rates_unroll"] in (0,) + autotune.UNROLLS and best["block"] in (0,) + autotune.BLOCKS
+    assert best["list_bits"] in autotune.LIST_BITS
     assert all(e["rate"] > 0 for e in table)
     assert autotune.schedule_for(_lib.FP64, "box", path=p)["source"] == "tuned"
     boxes = [configs.lognormal_box(400 + j, 256, 600, 0.4, 0.9, steps=20) for j in range(4)]

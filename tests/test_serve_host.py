@@ -124,8 +124,11 @@ def test_autotune_cache_roundtrip(tmp_path):
     assert d["source"] == "default" and d["unroll"] == 2
     autotune.save_cache({autotune.key(_lib.FP32C, "batched"): {"best": {"unroll": 1, "skin": 0.3}}}, p)
     got = autotune.schedule_for(_lib.FP32C, "batched", path=p)
-    # a cache without rates_unroll or block: their defaults
-    assert got == dict(unroll=1, skin=0.3, rates_unroll=0, block=0, source="tuned")
+    # a cache without rates_unroll, block or list_bits: their defaults
+    assert got == dict(unroll=1, skin=0.3, rates_unroll=0, block=0, list_bits=0, source="tuned")
+    # the list entry width picks 16 unless 32 is clearly faster
+    assert autotune.pick_list_bits([dict(list_bits=16, rate=1.0), dict(list_bits=32, rate=1.005)]) == 16
+    assert autotune.pick_list_bits([dict(list_bits=16, rate=1.0), dict(list_bits=32, rate=1.05)]) == 32
     # an invalid entry falls back to the default
     autotune.save_cache({autotune.key(_lib.FP32C, "box"): {"best": {"unroll": 3, "skin": 0.3}}}, p)
     assert autotune.schedule_for(_lib.FP32C, "box", path=p)["source"] == "default"
