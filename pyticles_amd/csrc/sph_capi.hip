@@ -140,12 +140,14 @@ static size_t arena_bytes(size_t cap, int prec, int lcap)
     const size_t d2 = sizeof(double2), d1 = sizeof(double), i1 = sizeof(int);
     size_t b = 0;
     b += cap * (4 * d2 + 4 * d1 + i1);          // r v vp acc, u m up udot, id
+    b += cap * 2 * d2;                          // second buffers of r, vp
     b += cap * (4 * d2 + 4 * d1 + i1);          // sorted scratch of the same
     b += cap * (2 * i1 + d2 + sizeof(int2));    // pcell slot_box, r_ref, cnt
     b += (cap + 63) / 64 * 64 * (size_t)lcap * sizeof(unsigned);  // nbr, in tiles of 64 slots
     b += cap * (3 * d2 + 2 * d1 + 2 * sizeof(double4));  // rho Q jq, T e, S gradv
     b += cap * (2 * d2 + d1);                             // vt, aco, uco
-    if (prec == SPH_FP32C) b += cap * (2 * sizeof(float4) + sizeof(unsigned) + 2 * sizeof(float2));  // rf S32, rlo, mt32 q32
+    if (prec == SPH_FP32C)  // rf S32, rlo, mt32 q32, second buffers of rf, rlo
+        b += cap * (2 * sizeof(float4) + sizeof(unsigned) + 2 * sizeof(float2) + sizeof(float4) + sizeof(unsigned));
     b += cap * 5 * sizeof(unsigned);             // sort keys / values
     b += ((cap + 63) / 64 + 1) * i1;             // building tiles
     b += (cap + 63) / 64 * sizeof(double);       // per-wave displacement
@@ -167,6 +169,12 @@ struct Arena {
 
     // device state (cap slots)
     double2 *r = nullptr, *v = nullptr, *vp = nullptr, *acc = nullptr;
+    // second buffers: the step pass writes the stepped r, vp (rf, rlo) there
+    // and swap_state() makes them current after every launched step
+    double2 *r_nx = nullptr, *vp_nx = nullptr;
+    float4 *rf_nx = nullptr;
+    unsigned *rlo_nx = nullptr;
+    int parity = 0;  // which set of buffers is current (swap_state)
     double *u = nullptr, *m = nullptr, *up = nullptr, *udot = nullptr;
     int *id = nullptr;
     double2 *r_s = nullptr, *v_s = nullptr, *vp_s = nullptr, *acc_s = nullptr;
@@ -253,7 +261,7 @@ struct Arena {
     // replays the largest that fit: each graph launch costs ~1-2 us of idle
     // GPU between graphs)
     static constexpr int PLAIN_LOG2 = 3;
-    hipGraphExec_t graph[2 + PLAIN_LOG2] = {};
+    hipGraphExec_t graph[2][2 + PLAIN_LOG2] = {};  // per parity (the buffers a step starts from)
     // What the captured launches bake in: the view (pointers, slot, key and
     // segment counts), the box count, the sort path and the pass depths.  A
     // relayout that leaves it unchanged (a box's terms or constants changed:
@@ -279,6 +287,7 @@ struct Arena {
         ArenaView A;
         memset(&A, 0, sizeof(A));
         A.r = r; A.v = v; A.vp = vp; A.acc = acc; A.u = u; A.m = m; A.up = up; A.udot = udot; A.id = id;
+        A.r_nx = r_nx; A.vp_nx = vp_nx; A.rf_nx = rf_nx; A.rlo_nx = rlo_nx;
         A.r_s = r_s; A.v_s = v_s; A.vp_s = vp_s; A.acc_s = acc_s;
         A.u_s = u_s; A.m_s = m_s; A.up_s = up_s; A.udot_s = udot_s; A.id_s = id_s;
         A.pcell = pcell; A.r_ref = r_ref; A.slot_box = slot_box; A.nbr = nbr; A.cnt = cnt;
@@ -306,6 +315,7 @@ struct Arena {
         if (lcap <= 0 || lcap % 8) return fail(SPH_ERR_ARG, "list entries must be a positive multiple of 8");
         int rc = 0;
         rc |= dalloc(&r, cap); rc |= dalloc(&v, cap); rc |= dalloc(&vp, cap); rc |= dalloc(&acc, cap);
+        rc |= dalloc(&r_nx, cap); rc |= dalloc(&vp_nx, cap);
         rc |= dalloc(&u, cap); rc |= dalloc(&m, cap); rc |= dalloc(&up, cap); rc |= dalloc(&udot, cap);
         rc |= dalloc(&id, cap);
         rc |= dalloc(&r_s, cap); rc |= dalloc(&v_s, cap); rc |= dalloc(&vp_s, cap); rc |= dalloc(&acc_s, cap);
@@ -319,6 +329,7 @@ struct Arena {
         rc |= dalloc(&vt, cap); rc |= dalloc(&aco, cap); rc |= dalloc(&uco, cap);
         if (prec == SPH_FP32C) {
             rc |= dalloc(&rf, cap); rc |= dalloc(&rlo, cap); rc |= dalloc(&S32, cap);
+            rc |= dalloc(&rf_nx, cap); rc |= dalloc(&rlo_nx, cap);
             rc |= dalloc(&mt32, cap); rc |= dalloc(&q32, cap);
         }
         rc |= dalloc(&keys_in, cap); rc |= dalloc(&k_out, cap); rc |= dalloc(&v_out, cap);
@@ -333,6 +344,7 @@ struct Arena {
         // zero-fill the state so that unused slots hold finite values
         size_t z2 = sizeof(double2) * cap, z1 = sizeof(double) * cap;
         if (hipMemset(r, 0, z2) || hipMemset(v, 0, z2) || hipMemset(vp, 0, z2) || hipMemset(u, 0, z1) ||
+            hipMemset(r_nx, 0, z2) || hipMemset(vp_nx, 0, z2) ||
             hipMemset(up, 0, z1) || hipMemset(m, 0, z1) || hipMemset(id, 0, sizeof(int) * cap) ||
             hipMemset(rho, 0, z2) || hipMemset(T, 0, z1) || hipMemset(S, 0, sizeof(double4) * cap) ||
             hipMemset(Q, 0, z2) || hipMemset(acc, 0, z2) || hipMemset(udot, 0, z1) ||
@@ -351,6 +363,7 @@ struct Arena {
     {
         invalidate_graph();
         dfree(r); dfree(v); dfree(vp); dfree(acc); dfree(u); dfree(m); dfree(up); dfree(udot); dfree(id);
+        dfree(r_nx); dfree(vp_nx); dfree(rf_nx); dfree(rlo_nx);
         dfree(r_s); dfree(v_s); dfree(vp_s); dfree(acc_s); dfree(u_s); dfree(m_s); dfree(up_s); dfree(udot_s);
         dfree(id_s); dfree(pcell); dfree(slot_box); dfree(r_ref); dfree(nbr); dfree(cnt);
         dfree(rho); dfree(Q); dfree(jq); dfree(T); dfree(e); dfree(S); dfree(gradv);
@@ -415,13 +428,15 @@ struct Arena {
     {
         graph_sig_set = false;
         bool any = false;
-        for (auto g : graph) any = any || g;
+        for (auto &gp : graph)
+            for (auto g : gp) any = any || g;
         if (!any) return;
         (void)hipStreamSynchronize(ctx->stream);  // no replay of them is in flight
-        for (auto &g : graph) {
-            if (g) (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
+        for (auto &gp : graph)
+            for (auto &g : gp) {
+                if (g) (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
     }
 
     GraphSig signature() const
@@ -429,6 +444,12 @@ struct Arena {
         GraphSig sg;
         memset(&sg, 0, sizeof(sg));
         sg.view = view();
+        if (parity) {  // the same graphs serve both parities: compare one order
+            std::swap(sg.view.r, sg.view.r_nx);
+            std::swap(sg.view.vp, sg.view.vp_nx);
+            std::swap(sg.view.rf, sg.view.rf_nx);
+            std::swap(sg.view.rlo, sg.view.rlo_nx);
+        }
         sg.nbox = (int)boxes.size();
         sg.small = small_boxes ? 1 : 0;
         sg.unroll = unroll;
@@ -447,18 +468,35 @@ struct Arena {
         graph_sig_set = true;
     }
 
-    // Capture and instantiate step graph idx: [0] one step without the build
-    // stage, [1] one step with it, [1 + lg] 2^lg steps without it.
+    // The stepped state becomes the current one: swap the buffer pairs the
+    // step pass reads and writes (host pointers only; stream order puts every
+    // later launch after the step).
+    void swap_state()
+    {
+        std::swap(r, r_nx);
+        std::swap(vp, vp_nx);
+        std::swap(rf, rf_nx);
+        std::swap(rlo, rlo_nx);
+        parity ^= 1;
+    }
+
+    // Capture and instantiate step graph idx for the current parity: [0] one
+    // step without the build stage, [1] one step with it, [1 + lg] 2^lg steps
+    // without it (the buffers swapping between the captured steps).
     int capture_graph(int idx)
     {
-        hipGraphExec_t &g = graph[idx];
+        hipGraphExec_t &g = graph[parity][idx];
         if (g) return SPH_OK;
         check_graph_sig();
         const bool with_build = idx == 1;
         const int per = idx >= 2 ? 1 << (idx - 1) : 1;
         hipGraph_t gr;
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int j = 0; j < per; ++j) enqueue_step(with_build);
+        for (int j = 0; j < per; ++j) {
+            enqueue_step(with_build);
+            swap_state();
+        }
+        if (per & 1) swap_state();  // nothing ran: back to the buffers that hold the state
         HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
         hipError_t e1 = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
@@ -475,8 +513,17 @@ struct Arena {
     {
         if (!use_graph || ctx->timing) return SPH_OK;  // timed runs launch directly
         HIPCHK(hipSetDevice(ctx->device));
-        for (int idx = 0; idx < 2 + PLAIN_LOG2; ++idx)
-            if (int rc = capture_graph(idx)) return rc;
+        // both parities: a step may start from either set of buffers (the
+        // swaps around the second set move pointers only)
+        for (int pass = 0; pass < 2; ++pass) {
+            int rc = SPH_OK;
+            for (int idx = 0; idx < 2 + PLAIN_LOG2 && !rc; ++idx) rc = capture_graph(idx);
+            swap_state();
+            if (rc) {
+                if (pass == 0) swap_state();
+                return rc;
+            }
+        }
         return SPH_OK;
     }
 
@@ -634,11 +681,8 @@ struct Arena {
         launch_grad(A, f, MASK_RUN, 0, unroll, st);
         mark(T_GRAD, false);
         mark(T_RATES, true);
-        launch_rates(A, f, MASK_RUN, rates_unroll(), st);
+        launch_rates(A, f, MASK_RUN, rates_unroll(), 1, st);  // with the leapfrog step
         mark(T_RATES, false);
-        mark(T_INT, true);
-        launch_integrate(A, f, st);
-        mark(T_INT, false);
         launch_box_step(A, (int)boxes.size(), st);
     }
 
@@ -656,9 +700,11 @@ struct Arena {
                 // single boxes capture at set-up (prepare_graphs); arenas
                 // capture here, at the first step after a relayout
                 if (int rc = capture_graph(idx)) return rc;
-                HIPCHK(hipGraphLaunch(graph[idx], ctx->stream));
+                HIPCHK(hipGraphLaunch(graph[parity][idx], ctx->stream));
+                if (per & 1) swap_state();
             } else {
                 enqueue_step(with_build);
+                swap_state();
             }
         }
         HIPCHK(hipGetLastError());
@@ -1412,7 +1458,7 @@ static int eval_box(sph_box *box, int grad, int rates)
     }
     if (rates) {
         A.mark(T_RATES, true);
-        launch_rates(V, f, MASK_ALL, A.rates_unroll(), st);
+        launch_rates(V, f, MASK_ALL, A.rates_unroll(), 0, st);
         A.mark(T_RATES, false);
     }
     HIPCHK(hipGetLastError());
@@ -1593,7 +1639,7 @@ int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch)
         for (int i = 0; i < reps; ++i) {
             if (pass == 0) launch_density(V, f, MASK_ALL, A.unroll, st);
             else if (pass == 1) launch_grad(V, f, MASK_ALL, 0, A.unroll, st);
-            else if (pass == 2) launch_rates(V, f, MASK_ALL, A.rates_unroll(), st);
+            else if (pass == 2) launch_rates(V, f, MASK_ALL, A.rates_unroll(), 0, st);
             else A.enqueue_build(MASK_ALL);
         }
         e = hipGetLastError();

@@ -7,6 +7,9 @@
 //                    half-step values between steps (leapfrog)
 //   predicted        vp, up: velocity / energy predicted to the current
 //                    integer time from the last rates; every pass reads these
+//   second buffers   r_nx, vp_nx (and FP32C rf_nx, rlo_nx): the step pass
+//                    writes the stepped positions, predicted velocities and
+//                    records there; the two sets swap roles every step
 //   last rates       acc (double2), udot
 //   sorted scratch   *_s copies written by the permutation of a build
 //   neighbour list   tiles of 64 slots in groups of 4 entries, short entries
@@ -107,6 +110,13 @@ struct SegDesc {
 struct ArenaView {
     // state (slot order of the last build)
     double2 *r, *v, *vp, *acc;
+    // the second buffers of what other slots read during the step pass:
+    // the leapfrog step writes the new position, predicted velocity and
+    // FP32C record there (the runtime swaps them with r, vp, rf, rlo after
+    // every step)
+    double2 *r_nx, *vp_nx;
+    float4 *rf_nx;
+    unsigned *rlo_nx;
     double *u, *m, *up, *udot;
     int *id;
     // sorted scratch written by the permutation of a build

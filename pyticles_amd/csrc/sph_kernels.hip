@@ -1352,40 +1352,6 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
     return o;
 }
 
-// P3: rates over the short entries, plus the cohesive part from P1.
-//   f   = (S_i + S_j) . dwdx_h           (S = (p I + Pi) / rho^2)
-//   a_i = sum_j m_j f + a_co,i;   udot_i = sum_j m_j (dv . f / 2 + (Q_i + Q_j) . dwdx_h) + du_co,i
-// Neighbour record: position, velocity, (Sxx, Sxy, Syy, m), Q.
-template <int U64>
-__global__ __launch_bounds__(PASS_NT_MAX) void k_rates64(ArenaView A, int state_mask)
-{
-    const int s = xcd_block() * (int)blockDim.x + threadIdx.x;
-    if (s >= A.n) return;
-    const int b = A.slot_box[s];
-    if (b < 0) return;
-    if (!state_in(A.dyn[b].state, state_mask) || A.dyn[b].relist) return;
-    const BoxDesc &B = A.boxes[b];
-    const int ns = (B.terms & 13u) ? (A.cnt[s].x & CNT_MASK) : 0;
-    Rate64 o;
-    per_box(b, [&](int bu) {
-        const BoxK K = box_k(A.boxes[bu], true);
-        const ListFmt L = list_fmt(A, bu, s);
-        const bool inner = wave_interior(A.r[s], K), cond = (K.terms & 8) != 0;
-        with_form(L, [&](auto lf) {
-            constexpr int F = decltype(lf)::value;
-            if (inner)
-                o = cond ? rates64_walk<U64, true, false, F>(A, K, L, s, ns) : rates64_walk<U64, false, false, F>(A, K, L, s, ns);
-            else
-                o = cond ? rates64_walk<U64, true, true, F>(A, K, L, s, ns) : rates64_walk<U64, false, true, F>(A, K, L, s, ns);
-        });
-    });
-    const double2 ac = A.aco[s];
-    const double ax = o.ax + ac.x, ay = o.ay + ac.y, ud = o.ud + A.uco[s];
-    if (!isfinite(ax) || !isfinite(ay) || !isfinite(ud)) flag_health(A.dyn, b, HEALTH_NONFINITE);
-    A.acc[s] = make_double2(ax, ay);
-    A.udot[s] = ud;
-}
-
 // ===========================================================================
 // FP32-compensated form: FP32 neighbour records (48-bit box-relative fixed
 // point positions), FP32 long-range pair arithmetic with Kahan-compensated
@@ -1839,36 +1805,8 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
     }
 }
 
-// P3 of the FP32C form over the short entries; adds the cohesive part from P1.
-template <int U32>
-__global__ __launch_bounds__(PASS_NT_MAX) void k_rates32(ArenaView A, int state_mask)
-{
-    const int s = xcd_block() * (int)blockDim.x + threadIdx.x;
-    if (s >= A.n) return;
-    const int b = A.slot_box[s];
-    if (b < 0) return;
-    if (!state_in(A.dyn[b].state, state_mask) || A.dyn[b].relist) return;
-    const BoxDesc &B = A.boxes[b];
-    const int ns = (B.terms & 13u) ? (A.cnt[s].x & CNT_MASK) : 0;
-    Rate32 o = {0.0, 0.0, 0.0};
-    per_box(b, [&](int bu) {
-        const BoxF F = boxf(A.boxes[bu], true);
-        const ListFmt L = list_fmt(A, bu, s);
-        with_form(L, [&](auto lf) {
-            constexpr int LF = decltype(lf)::value;
-            if (F.terms & 8) rates32_walk<U32, true, LF>(A, F, L, s, ns, o);
-            else rates32_walk<U32, false, LF>(A, F, L, s, ns, o);
-        });
-    });
-    const double2 ac = A.aco[s];
-    const double axd = o.ax + ac.x, ayd = o.ay + ac.y, udd = o.ud + A.uco[s];
-    if (!isfinite(axd) || !isfinite(ayd) || !isfinite(udd)) flag_health(A.dyn, b, HEALTH_NONFINITE);
-    A.acc[s] = make_double2(axd, ayd);
-    A.udot[s] = udd;
-}
-
 // ===========================================================================
-// P4 leapfrog, shared by both forms (FP64 state)
+// P3 rates with the P4 leapfrog fused, shared by both forms (FP64 state)
 // ===========================================================================
 
 // Running boxes kick by dt/2 at k = 0 and k = nsteps and by dt otherwise,
@@ -1876,52 +1814,74 @@ __global__ __launch_bounds__(PASS_NT_MAX) void k_rates32(ArenaView A, int state_
 // half-step values; vp / up get the prediction to the next integer time.
 // The energy sum m (v^2/2 + u) of each slot is captured at the first and
 // last synchronised state.
-// Leapfrog step of slot s of running box b; returns the squared displacement
-// since the box's last build (-1 at the closing half kick, which moves
-// nothing).
-__device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, int s, int b)
+//
+// The kick and drift run in the rate pass, right after a slot's rates,
+// while other slots of the same launch still gather this slot's position
+// and predicted velocity (and FP32C record): those go to the second state
+// buffers (r_nx, vp_nx, rf_nx, rlo_nx), which the runtime makes the current
+// ones after the step; v, u and up are read by their own slot only and are
+// updated in place.  A live slot of a box that takes no step in this launch
+// (initialising, done, held) is carried over into the second buffers.
+//
+// Leapfrog step of slot s of running box b with this evaluation's rates
+// (a, ud); returns the squared displacement since the box's last build (-1
+// at the closing half kick, which moves nothing).
+__device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, int s, int b, double2 a, double ud)
 {
     const BoxDyn &D = A.dyn[b];
     const BoxDesc &B = A.boxes[b];
     const int64_t k = D.k, K = D.nsteps;
     const double kick = (k == 0 || k == K) ? 0.5 * B.dt : B.dt;
-    const double2 v = A.v[s], a = A.acc[s];
-    const double u = A.u[s], ud = A.udot[s], m = A.m[s];
+    const double2 v = A.v[s];
+    const double u = A.u[s], m = A.m[s];
     if (k == 0) A.e[s] = m * (0.5 * (v.x * v.x + v.y * v.y) + u);
     const double vx = v.x + kick * a.x, vy = v.y + kick * a.y;
     const double un = u + kick * ud;
     A.v[s] = make_double2(vx, vy);
     A.u[s] = un;
+    const double2 r = A.r[s];
     if (k == K) {
         A.e[s] = m * (0.5 * (vx * vx + vy * vy) + un);
-        A.vp[s] = make_double2(vx, vy);
+        A.r_nx[s] = r;
+        A.vp_nx[s] = make_double2(vx, vy);
         A.up[s] = un;
-        if (fp32) fix_rec(A.r[s], make_double2(vx, vy), B, A.rf[s], A.rlo[s]);
+        if (fp32) fix_rec(r, make_double2(vx, vy), B, A.rf_nx[s], A.rlo_nx[s]);
         return -1.0;
     }
-    const double2 r = A.r[s];
     double2 rn;
     rn.x = wrap_coord(r.x + B.dt * vx, B.Lx);
     rn.y = wrap_coord(r.y + B.dt * vy, B.Ly);
-    A.r[s] = rn;
+    A.r_nx[s] = rn;
     // a non-finite velocity (or energy) of a particle without neighbours
     // reaches no rate; catch it in the state
     if (!isfinite(rn.x) || !isfinite(rn.y) || !isfinite(un)) flag_health(A.dyn, b, HEALTH_NONFINITE);
     const double hdt = 0.5 * B.dt;  // predictor to the next integer time
     const double2 vp = make_double2(vx + hdt * a.x, vy + hdt * a.y);
-    A.vp[s] = vp;
+    A.vp_nx[s] = vp;
     A.up[s] = un + hdt * ud;
     const double2 r0 = A.r_ref[s];
     double ddx = rn.x - r0.x, ddy = rn.y - r0.y;
     min_image(ddx, ddy, B);
     const double d2 = ddx * ddx + ddy * ddy;
     if (d2 > B.hskin2) flag_health(A.dyn, b, HEALTH_SKIN);
-    if (fp32) fix_rec(rn, vp, B, A.rf[s], A.rlo[s]);
+    if (fp32) fix_rec(rn, vp, B, A.rf_nx[s], A.rlo_nx[s]);
     return d2;
 }
 
+// A live slot whose box takes no step in this launch: its state, unchanged,
+// into the second buffers.
+__device__ __forceinline__ void carry_slot(const ArenaView &A, int fp32, int s)
+{
+    A.r_nx[s] = A.r[s];
+    A.vp_nx[s] = A.vp[s];
+    if (fp32) {
+        A.rf_nx[s] = A.rf[s];
+        A.rlo_nx[s] = A.rlo[s];
+    }
+}
+
 // A running box re-examines its list at the step before a scheduled build
-// (k_box_step, extend_steps); on that step k_integrate gathers the largest
+// (k_box_step, extend_steps); on that step the step pass gathers the largest
 // squared displacement of the box's slots: per wave in wdisp for the waves
 // wholly inside the box, in its dyn.dmax2 for the waves it shares.
 __device__ __forceinline__ bool list_review(const BoxDyn &D)
@@ -1932,27 +1892,28 @@ __device__ __forceinline__ bool list_review(const BoxDyn &D)
     return D.state == BOX_RUN && !building(D, 0) && D.k < D.nsteps && D.k + 1 >= D.next_rebuild;
 }
 
-// Integrate every running slot; on a box's review step also record the
-// largest squared displacement of its lanes.  A wave holds 64 consecutive
-// slots and a box a contiguous slot range, so a wave whose first and last
-// lanes share a box lies wholly inside it: it stores its maximum in wdisp.
-// A wave shared by several boxes (at most two per box) folds each reviewing
-// box's lanes into that box's dyn.dmax2 with one atomic.  Each box thus sees
-// only its own slots, so its schedule, and with it its results, do not depend
-// on the boxes beside it.  (One atomic per wave into a box field made every
-// review step of a large box slow, the waves queueing on one address.)  No
-// early return before the shuffles, so every lane takes part.
-__global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
+// The step pass's tail for slot s of box b (-1: no box): the leapfrog step
+// (on: the box runs and took its rates in this launch) or the carry-over,
+// then, on a box's review step, the largest squared displacement of its
+// lanes.  A wave holds 64 consecutive slots and a box a contiguous slot
+// range, so a wave whose first and last lanes share a box lies wholly inside
+// it: it stores its maximum in wdisp.  A wave shared by several boxes (at
+// most two per box) folds each reviewing box's lanes into that box's
+// dyn.dmax2 with one atomic.  Each box thus sees only its own slots, so its
+// schedule, and with it its results, do not depend on the boxes beside it.
+// (One atomic per wave into a box field made every review step of a large
+// box slow, the waves queueing on one address.)  Every lane of the wave
+// calls this (no early return before the shuffles).
+__device__ __forceinline__ void step_tail(const ArenaView &A, int fp32, int s, int b, bool on, double2 a, double ud)
 {
-    const int s = xcd_block() * NT + threadIdx.x;
     double d2 = 0.0;
     bool review = false;
-    int b = -1;
-    if (s < A.n) {
-        b = A.slot_box[s];
-        if (b >= 0 && A.dyn[b].state == BOX_RUN && !A.dyn[b].relist) {
-            d2 = fmax(integrate_slot(A, fp32, s, b), 0.0);
+    if (s < A.n && b >= 0) {
+        if (on) {
+            d2 = fmax(integrate_slot(A, fp32, s, b, a, ud), 0.0);
             review = list_review(A.dyn[b]);
+        } else {
+            carry_slot(A, fp32, s);
         }
     }
     uint64_t pending = __ballot(review);
@@ -1975,6 +1936,87 @@ __global__ __launch_bounds__(NT) void k_integrate(ArenaView A, int fp32)
         if (lane == leader) max_bits(&A.dyn[bu].dmax2, m);
         pending &= ~__ballot(mine);
     }
+}
+
+// P3: rates over the short entries, plus the cohesive part from P1.
+//   f   = (S_i + S_j) . dwdx_h           (S = (p I + Pi) / rho^2)
+//   a_i = sum_j m_j f + a_co,i;   udot_i = sum_j m_j (dv . f / 2 + (Q_i + Q_j) . dwdx_h) + du_co,i
+// Neighbour record: position, velocity, (Sxx, Sxy, Syy, m), Q.  step: the
+// stepping launch, which then takes the leapfrog step (step_tail); the
+// stand-alone evaluations only store the rates.
+template <int U64>
+__global__ __launch_bounds__(PASS_NT_MAX) void k_rates64(ArenaView A, int state_mask, int step)
+{
+    const int s = xcd_block() * (int)blockDim.x + threadIdx.x;
+    int b = -1;
+    bool on = false;
+    if (s < A.n) {
+        b = A.slot_box[s];
+        on = b >= 0 && state_in(A.dyn[b].state, state_mask) && !A.dyn[b].relist;
+    }
+    double2 a = make_double2(0.0, 0.0);
+    double ud = 0.0;
+    if (on) {
+        const BoxDesc &B = A.boxes[b];
+        const int ns = (B.terms & 13u) ? (A.cnt[s].x & CNT_MASK) : 0;
+        Rate64 o;
+        per_box(b, [&](int bu) {
+            const BoxK K = box_k(A.boxes[bu], true);
+            const ListFmt L = list_fmt(A, bu, s);
+            const bool inner = wave_interior(A.r[s], K), cond = (K.terms & 8) != 0;
+            with_form(L, [&](auto lf) {
+                constexpr int F = decltype(lf)::value;
+                if (inner)
+                    o = cond ? rates64_walk<U64, true, false, F>(A, K, L, s, ns) : rates64_walk<U64, false, false, F>(A, K, L, s, ns);
+                else
+                    o = cond ? rates64_walk<U64, true, true, F>(A, K, L, s, ns) : rates64_walk<U64, false, true, F>(A, K, L, s, ns);
+            });
+        });
+        const double2 ac = A.aco[s];
+        a = make_double2(o.ax + ac.x, o.ay + ac.y);
+        ud = o.ud + A.uco[s];
+        if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(ud)) flag_health(A.dyn, b, HEALTH_NONFINITE);
+        A.acc[s] = a;
+        A.udot[s] = ud;
+    }
+    if (step) step_tail(A, 0, s, b, on, a, ud);
+}
+
+// P3 of the FP32C form over the short entries; adds the cohesive part from
+// P1; step as in k_rates64.
+template <int U32>
+__global__ __launch_bounds__(PASS_NT_MAX) void k_rates32(ArenaView A, int state_mask, int step)
+{
+    const int s = xcd_block() * (int)blockDim.x + threadIdx.x;
+    int b = -1;
+    bool on = false;
+    if (s < A.n) {
+        b = A.slot_box[s];
+        on = b >= 0 && state_in(A.dyn[b].state, state_mask) && !A.dyn[b].relist;
+    }
+    double2 a = make_double2(0.0, 0.0);
+    double ud = 0.0;
+    if (on) {
+        const BoxDesc &B = A.boxes[b];
+        const int ns = (B.terms & 13u) ? (A.cnt[s].x & CNT_MASK) : 0;
+        Rate32 o = {0.0, 0.0, 0.0};
+        per_box(b, [&](int bu) {
+            const BoxF F = boxf(A.boxes[bu], true);
+            const ListFmt L = list_fmt(A, bu, s);
+            with_form(L, [&](auto lf) {
+                constexpr int LF = decltype(lf)::value;
+                if (F.terms & 8) rates32_walk<U32, true, LF>(A, F, L, s, ns, o);
+                else rates32_walk<U32, false, LF>(A, F, L, s, ns, o);
+            });
+        });
+        const double2 ac = A.aco[s];
+        a = make_double2(o.ax + ac.x, o.ay + ac.y);
+        ud = o.ud + A.uco[s];
+        if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(ud)) flag_health(A.dyn, b, HEALTH_NONFINITE);
+        A.acc[s] = a;
+        A.udot[s] = ud;
+    }
+    if (step) step_tail(A, 1, s, b, on, a, ud);
 }
 
 // ===========================================================================
@@ -2005,7 +2047,7 @@ __device__ __forceinline__ int64_t reuse_steps(const BoxDesc &B, const BoxDyn &D
 // the speed measured at the last build plus what that acceleration adds over
 // the list's age (reuse_steps' bound applied to the margin that is left),
 // capped so that a list lives at most 64 steps; 0 = build now.  Exceeding
-// half the skin anyway raises HEALTH_SKIN in k_integrate, as before.
+// half the skin anyway raises HEALTH_SKIN in the step pass (integrate_slot).
 __device__ __forceinline__ int64_t extend_steps(const BoxDesc &B, const BoxDyn &D, double dmax2)
 {
     // a list built before any rates existed (acceleration 0, as reuse_steps
@@ -2093,7 +2135,7 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
         D.amax2 = 0ull;
     } else if (review) {
         // the next step would build: postpone while the measured displacement
-        // (gathered by k_integrate this step) leaves room, by the same bound
+        // (gathered by the step pass this step) leaves room, by the same bound
         // as reuse_steps on what is left
         D.dmax2 = 0ull;
         const int64_t c = extend_steps(A.boxes[b], D, dmax2);
@@ -2539,13 +2581,9 @@ void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, int unr
 {
     SPH_DISPATCH(k_grad64, k_grad32, A, state_mask, diag);
 }
-void launch_rates(const ArenaView &A, int fp32, int state_mask, int unroll, hipStream_t st)
+void launch_rates(const ArenaView &A, int fp32, int state_mask, int unroll, int step, hipStream_t st)
 {
-    SPH_DISPATCH(k_rates64, k_rates32, A, state_mask);
-}
-void launch_integrate(const ArenaView &A, int fp32, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_integrate, grid_for(A.n), dim3(NT), 0, st, A, fp32);
+    SPH_DISPATCH(k_rates64, k_rates32, A, state_mask, step);
 }
 void launch_box_step(const ArenaView &A, int nbox, hipStream_t st)
 {

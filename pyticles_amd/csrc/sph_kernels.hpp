@@ -25,8 +25,10 @@ bool box_sort_fits(int ncells, int n);
 void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_t st);
 void launch_density(const ArenaView &A, int fp32, int state_mask, int unroll, hipStream_t st);
 void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, int unroll, hipStream_t st);
-void launch_rates(const ArenaView &A, int fp32, int state_mask, int unroll, hipStream_t st);
-void launch_integrate(const ArenaView &A, int fp32, hipStream_t st);
+// step != 0: the stepping launch, which also takes each running slot's
+// leapfrog step into the second state buffers (and carries the other live
+// slots over); the runtime swaps the buffers after it
+void launch_rates(const ArenaView &A, int fp32, int state_mask, int unroll, int step, hipStream_t st);
 void launch_box_step(const ArenaView &A, int nbox, hipStream_t st);
 void launch_upload(const ArenaView &A, int off, int n, int field, const double *src, hipStream_t st);
 void launch_init_slots(const ArenaView &A, int off, int n, hipStream_t st);
