@@ -518,8 +518,11 @@ def main():
     value = s.n * args.steps / el
     evals = args.steps + 1
     kevals = int(tm["steps"]) + 1  # rate evaluations of the kernel-timing region
-    # the rate pass takes the leapfrog step in the same launch
+    # the rate pass takes the leapfrog step in the same launch, unless the
+    # schedule runs it as a launch of its own ("leapfrog")
     per = {k: tm[k + "_ms"] / kevals for k in ("build", "density", "gradients", "rates")}
+    if tm["integrate_ms"] > 0:
+        per["leapfrog"] = tm["integrate_ms"] / kevals
     # the dominant pair-pass kernel by measured time, timed alone
     lus = tm["launch_us"]
     dom = max(("density", "gradients", "rates"), key=lambda k: lus[k])

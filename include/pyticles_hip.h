@@ -299,6 +299,12 @@ int sph_box_set_pass_block(sph_box *box, int block);
  * sph_*_list_bits reports the width of a box's lists in memory (after its
  * first build).  No counterpart in the reference (its lists are Python
  * arrays, neighbour_list.py:32-35). */
+/* Where the leapfrog step runs: inside the rate pass's launch (1, the
+ * default: no re-read of the rates, one launch fewer per step) or as a
+ * launch of its own after it (0); the same results.  A launch schedule the
+ * autotuner picks per workload class.  No counterpart in the reference. */
+int sph_arena_set_fused_step(sph_arena *ar, int on);
+int sph_box_set_fused_step(sph_box *box, int on);
 int sph_arena_set_list_bits(sph_arena *ar, int bits);
 int sph_box_set_list_bits(sph_box *box, int bits);
 int sph_arena_list_bits(sph_arena *ar, int box_id, int *bits);

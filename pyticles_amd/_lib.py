@@ -101,6 +101,8 @@ _SIGS = {
     "sph_box_set_rates_unroll": [_P, ctypes.c_int],
     "sph_arena_set_pass_block": [_P, ctypes.c_int],
     "sph_box_set_pass_block": [_P, ctypes.c_int],
+    "sph_arena_set_fused_step": [_P, ctypes.c_int],
+    "sph_box_set_fused_step": [_P, ctypes.c_int],
     "sph_arena_set_list_bits": [_P, ctypes.c_int],
     "sph_box_set_list_bits": [_P, ctypes.c_int],
     "sph_arena_list_bits": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],

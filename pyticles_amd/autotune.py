@@ -17,7 +17,9 @@ rounding; the list entry width not even that):
     list_bits
             neighbour-list entry width (16: slot offsets, half the list
             bytes, one 16-byte load per eight entries; 32: slots); same
-            results either way; tuned last
+            results either way
+    fused   the leapfrog step inside the rate pass (1) or as a launch of
+            its own (0); same results; tuned last
     skin    neighbour-list skin: lists hold pairs within r_cut + skin and are
             reused while no particle can have crossed half of it; a larger
             skin rebuilds less often and walks longer lists
@@ -43,6 +45,7 @@ from . import _lib
 UNROLLS = (1, 2, 4)
 BLOCKS = (128, 256, 512)
 LIST_BITS = (16, 32)
+FUSED = (1, 0)
 SKINS = (0.3, 0.5, 0.75, 1.0, 1.25)
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "schedules.json")
 
@@ -92,7 +95,8 @@ def save_cache(data, path=CACHE):
 def valid(s):
     return (isinstance(s, dict) and s.get("unroll") in UNROLLS and isinstance(s.get("skin"), (int, float))
             and 0.0 < float(s["skin"]) <= 2.0 and s.get("rates_unroll", 0) in (0,) + UNROLLS
-            and s.get("block", 0) in (0,) + BLOCKS and s.get("list_bits", 0) in (0,) + LIST_BITS)
+            and s.get("block", 0) in (0,) + BLOCKS and s.get("list_bits", 0) in (0,) + LIST_BITS
+            and s.get("fused", 1) in FUSED)
 
 
 def schedule_for(precision, cls, path=CACHE):
@@ -102,9 +106,10 @@ def schedule_for(precision, cls, path=CACHE):
     if entry and valid(entry.get("best")):
         b = entry["best"]
         return dict(unroll=int(b["unroll"]), skin=float(b["skin"]), rates_unroll=int(b.get("rates_unroll", 0)),
-                    block=int(b.get("block", 0)), list_bits=int(b.get("list_bits", 0)), source="tuned")
+                    block=int(b.get("block", 0)), list_bits=int(b.get("list_bits", 0)), fused=int(b.get("fused", 1)),
+                    source="tuned")
     d = DEFAULT[int(precision)]
-    return dict(d, rates_unroll=0, block=0, list_bits=0, source="default")
+    return dict(d, rates_unroll=0, block=0, list_bits=0, fused=1, source="default")
 
 
 def candidates():
@@ -227,6 +232,20 @@ def tune(systems, precision, cls=None, steps=None, warm=100, save=True, path=CAC
                 key(precision, cls), c["unroll"], c["skin"], c["rates_unroll"], c["block"], lb, rate))
     best["list_bits"] = pick_list_bits(ltable)
     table += ltable
+    # then where the leapfrog step runs
+    ftable = []
+    for fu in FUSED:
+        c = dict(best, fused=fu)
+        if single:
+            rate = _time_box(systems[0], c, precision, steps, warm)
+        else:
+            rate = _time_batched(systems, c, precision, steps, warm)
+        ftable.append(dict(c, rate=rate))
+        if log:
+            log("tune %s unroll=%d skin=%.2f rates_unroll=%d block=%d list_bits=%d fused=%d  %.3e particle-steps/s" % (
+                key(precision, cls), c["unroll"], c["skin"], c["rates_unroll"], c["block"], c["list_bits"], fu, rate))
+    best["fused"] = pick_fused(ftable)
+    table += ftable
     if save:
         data = load_cache(path)
         data[key(precision, cls)] = dict(
@@ -279,3 +298,12 @@ def pick_list_bits(ltable, within=0.01):
     if 16 in r and 32 in r and r[32] > (1.0 + within) * r[16]:
         return 32
     return 16
+
+
+def pick_fused(ftable, within=0.01):
+    """The leapfrog step inside the rate pass unless a launch of its own is
+    more than `within` faster."""
+    r = {e["fused"]: e["rate"] for e in ftable}
+    if 1 in r and 0 in r and r[0] > (1.0 + within) * r[1]:
+        return 0
+    return 1

@@ -268,7 +268,7 @@ struct Arena {
     // the kernels read those from the box table) keeps the graphs.
     struct GraphSig {
         ArenaView view;
-        int nbox, small, unroll, unroll_rates;
+        int nbox, small, unroll, unroll_rates, fuse_step;
     };
     GraphSig graph_sig;
     bool graph_sig_set = false;
@@ -278,6 +278,7 @@ struct Arena {
     int unroll = 0;        // list entries in flight per lane in the pair passes (0 = default)
     int pass_block = 256;  // work-items per block of the pair passes (128, 256, 512)
     int unroll_rates = 0;  // the same for the rate pass alone (0 = unroll)
+    int fuse_step = 1;     // the leapfrog step inside the rate pass (0: a launch of its own)
     int rates_unroll() const { return unroll_rates ? unroll_rates : unroll; }
 
     bool fp32() const { return precision == SPH_FP32C; }
@@ -454,6 +455,7 @@ struct Arena {
         sg.small = small_boxes ? 1 : 0;
         sg.unroll = unroll;
         sg.unroll_rates = unroll_rates;
+        sg.fuse_step = fuse_step;
         return sg;
     }
 
@@ -681,8 +683,13 @@ struct Arena {
         launch_grad(A, f, MASK_RUN, 0, unroll, st);
         mark(T_GRAD, false);
         mark(T_RATES, true);
-        launch_rates(A, f, MASK_RUN, rates_unroll(), 1, st);  // with the leapfrog step
+        launch_rates(A, f, MASK_RUN, rates_unroll(), fuse_step, st);  // with the leapfrog step when fused
         mark(T_RATES, false);
+        if (!fuse_step) {
+            mark(T_INT, true);
+            launch_leapfrog(A, f, st);
+            mark(T_INT, false);
+        }
         launch_box_step(A, (int)boxes.size(), st);
     }
 
@@ -800,24 +807,23 @@ struct Arena {
             open_plain = plain;
         }
         int64_t done = 0;
-        int held = 0;
         while (done < nsteps) {
-            if (open_plain == 0) {
-                // a step with the build stage, then the schedule read back
-                const int64_t rel = released;
-                if (int rc = launch_steps(1, true)) return rc;
-                if (int rc = read_dyn()) return rc;
-                if (released != rel) {  // held build: again, in 32-bit entries
-                    if (++held > 2) return fail(SPH_ERR_HIP, "list build held repeatedly");
-                    continue;
-                }
-                ++done;
+            if (open_plain < 0) {
+                // the schedule after the last step with the build stage, read
+                // when it is needed (at this call, not right after that
+                // launch, so the host's work between calls overlaps the step)
+                if (int rc = retake_held()) return rc;
                 const BoxDyn &D = h_dyn[b];
                 if (D.health) {
                     open_run = false;
                     return SPH_OK;  // the caller's health check reports it
                 }
                 open_plain = D.next_rebuild > D.k ? D.next_rebuild - D.k : 0;
+            }
+            if (open_plain == 0) {
+                if (int rc = launch_steps(1, true)) return rc;
+                open_plain = -1;
+                ++done;
             } else {
                 const int64_t p = std::min(open_plain, nsteps - done);
                 if (int rc = launch_steps(p, false)) return rc;
@@ -836,7 +842,7 @@ struct Arena {
     {
         if (!open_run) return SPH_OK;
         open_run = false;
-        if (int rc = read_dyn()) return rc;  // also waits for the run's steps
+        if (int rc = retake_held()) return rc;  // also waits for the run's steps
         BoxDyn D = h_dyn[b];
         if (D.health || D.state != BOX_RUN) return SPH_OK;
         D.nsteps = D.k;
@@ -849,6 +855,20 @@ struct Arena {
             if (++held > 2) return fail(SPH_ERR_HIP, "list build held repeatedly");
         }
         return collect_timing();
+    }
+
+    // Read the box states; while the last launched step was a build that
+    // held its box (released by the read, now in 32-bit entries), launch
+    // that step again, so the steps a caller asked for all take effect.
+    int retake_held()
+    {
+        for (int held = 0;;) {
+            const int64_t rel = released;
+            if (int rc = read_dyn()) return rc;
+            if (released == rel) return SPH_OK;
+            if (++held > 2) return fail(SPH_ERR_HIP, "list build held repeatedly");
+            if (int rc = launch_steps(1, true)) return rc;
+        }
     }
 
     // The box states, read back (waits for the stream); boxes held by a build
@@ -2121,6 +2141,29 @@ int sph_box_list_bits(sph_box *box, int *bits)
 {
     if (!box) return fail(SPH_ERR_ARG, "null box");
     return list_bits_of(box->ar, box->b, bits);
+}
+
+// Where the leapfrog step runs: inside the rate pass (1, the default) or as
+// a launch of its own after it (0); the same results.
+static int set_fused_step(Arena &A, int on)
+{
+    if (on != 0 && on != 1) return fail(SPH_ERR_ARG, "fused step must be 0 or 1");
+    A.fuse_step = on;
+    A.check_graph_sig();
+    return SPH_OK;
+}
+
+int sph_arena_set_fused_step(sph_arena *ar, int on)
+{
+    if (!ar) return fail(SPH_ERR_ARG, "null arena");
+    return set_fused_step(ar->ar, on);
+}
+
+int sph_box_set_fused_step(sph_box *box, int on)
+{
+    if (!box) return fail(SPH_ERR_ARG, "null box");
+    if (int rc = set_fused_step(box->ar, on)) return rc;
+    return box->ar.prepare_graphs();
 }
 
 int sph_arena_set_rates_unroll(sph_arena *ar, int unroll)

@@ -2019,6 +2019,27 @@ __global__ __launch_bounds__(PASS_NT_MAX) void k_rates32(ArenaView A, int state_
     if (step) step_tail(A, 1, s, b, on, a, ud);
 }
 
+// The step pass as a launch of its own (the schedule's unfused form): the
+// same leapfrog step and carry-over as step_tail in the rate pass, from the
+// rates that pass stored.  The same bits either way.
+__global__ __launch_bounds__(NT) void k_leapfrog(ArenaView A, int fp32)
+{
+    const int s = xcd_block() * NT + threadIdx.x;
+    int b = -1;
+    bool on = false;
+    double2 a = make_double2(0.0, 0.0);
+    double ud = 0.0;
+    if (s < A.n) {
+        b = A.slot_box[s];
+        on = b >= 0 && A.dyn[b].state == BOX_RUN && !A.dyn[b].relist;
+        if (on) {
+            a = A.acc[s];
+            ud = A.udot[s];
+        }
+    }
+    step_tail(A, fp32, s, b, on, a, ud);
+}
+
 // ===========================================================================
 // per-box bookkeeping
 // ===========================================================================
@@ -2584,6 +2605,10 @@ void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, int unr
 void launch_rates(const ArenaView &A, int fp32, int state_mask, int unroll, int step, hipStream_t st)
 {
     SPH_DISPATCH(k_rates64, k_rates32, A, state_mask, step);
+}
+void launch_leapfrog(const ArenaView &A, int fp32, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_leapfrog, grid_for(A.n), dim3(NT), 0, st, A, fp32);
 }
 void launch_box_step(const ArenaView &A, int nbox, hipStream_t st)
 {

@@ -29,6 +29,8 @@ void launch_grad(const ArenaView &A, int fp32, int state_mask, int diag, int unr
 // leapfrog step into the second state buffers (and carries the other live
 // slots over); the runtime swaps the buffers after it
 void launch_rates(const ArenaView &A, int fp32, int state_mask, int unroll, int step, hipStream_t st);
+// the step pass as a launch of its own, after launch_rates(step = 0)
+void launch_leapfrog(const ArenaView &A, int fp32, hipStream_t st);
 void launch_box_step(const ArenaView &A, int nbox, hipStream_t st);
 void launch_upload(const ArenaView &A, int off, int n, int field, const double *src, hipStream_t st);
 void launch_init_slots(const ArenaView &A, int off, int n, hipStream_t st);

@@ -132,20 +132,23 @@ class DeviceBox:
     def snapshot_wait(self):
         _lib.check(_lib.lib.sph_box_snapshot_wait(self._h))
 
-    def set_schedule(self, unroll, rates_unroll=0, block=0, list_bits=0):
+    def set_schedule(self, unroll, rates_unroll=0, block=0, list_bits=0, fused=1):
         """List entries in flight per lane in the pair passes (1, 2, 4; 0 =
         default), in the rate pass alone (0 = the same), the pair passes'
-        block size (128, 256, 512; 0 = 256) and the list entry width (16,
-        32; 0 = leave it).  unroll may be a schedule dict from
-        autotune.schedule_for."""
+        block size (128, 256, 512; 0 = 256), the list entry width (16,
+        32; 0 = leave it) and whether the leapfrog step runs inside the rate
+        pass (1) or as a launch of its own (0).  unroll may be a schedule
+        dict from autotune.schedule_for."""
         if isinstance(unroll, dict):
-            unroll, rates_unroll, block, list_bits = (unroll["unroll"], unroll.get("rates_unroll", 0),
-                                                      unroll.get("block", 0), unroll.get("list_bits", 0))
+            unroll, rates_unroll, block, list_bits, fused = (
+                unroll["unroll"], unroll.get("rates_unroll", 0), unroll.get("block", 0),
+                unroll.get("list_bits", 0), unroll.get("fused", 1))
         _lib.check(_lib.lib.sph_box_set_schedule(self._h, int(unroll)))
         _lib.check(_lib.lib.sph_box_set_rates_unroll(self._h, int(rates_unroll)))
         _lib.check(_lib.lib.sph_box_set_pass_block(self._h, int(block)))
         if list_bits:
             self.set_list_bits(list_bits)
+        _lib.check(_lib.lib.sph_box_set_fused_step(self._h, 1 if fused else 0))
 
     def set_list_bits(self, bits):
         """Entry width of the lists the next builds write: 16 (default;

@@ -233,6 +233,26 @@ def test_step_parity(cfg, nsteps):
     assert e[2] == pytest.approx(E[-1], rel=1e-11)
 
 
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32C])
+def test_fused_and_separate_leapfrog_step_give_the_same_bits(precision):
+    # the leapfrog step inside the rate pass or as a launch of its own: the
+    # same arithmetic on the same rates, with the state buffers swapping
+    # every step (an odd and an even number of steps, through both the step
+    # graphs and direct launches)
+    s = configs.square_system(40, 0.9, 7, 0, precision=precision)
+    outs = []
+    for fused in (1, 0):
+        with from_system(s) as b:
+            b.set_schedule(0, fused=fused)
+            b.step(37)
+            b.step(24)
+            b.set_graph(False)
+            b.step(5)
+            outs.append(b.download("r", "v", "u", "rho"))
+    for k in ("r", "v", "u", "rho"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
 @pytest.mark.parametrize("nsteps", [8, 150])
 def test_step_graph_equals_direct_launches(nsteps):
     # 150 steps: runs of plain steps between builds long enough to replay the

@@ -101,7 +101,7 @@ def test_autotune_small_workload(tmp_path):
     best, table = autotune.tune(s, _lib.FP64, "box", steps=3, warm=1, path=p)
     assert autotune.valid(best)
     assert len(table) == (len(autotune.candidates()) + len(autotune.UNROLLS) + len(autotune.BLOCKS)
-                          + len(autotune.LIST_BITS))
+                          + len(autotune.LIST_BITS) + len(autotune.FUSED))
     assert best["rates_unroll"] in (0,) + autotune.UNROLLS and best["block"] in (0,) + autotune.BLOCKS
     assert best["list_bits"] in autotune.LIST_BITS
     assert all(e["rate"] > 0 for e in table)

@@ -125,7 +125,9 @@ def test_autotune_cache_roundtrip(tmp_path):
     autotune.save_cache({autotune.key(_lib.FP32C, "batched"): {"best": {"unroll": 1, "skin": 0.3}}}, p)
     got = autotune.schedule_for(_lib.FP32C, "batched", path=p)
     # a cache without rates_unroll, block or list_bits: their defaults
-    assert got == dict(unroll=1, skin=0.3, rates_unroll=0, block=0, list_bits=0, source="tuned")
+    assert got == dict(unroll=1, skin=0.3, rates_unroll=0, block=0, list_bits=0, fused=1, source="tuned")
+    assert autotune.pick_fused([dict(fused=1, rate=1.0), dict(fused=0, rate=1.005)]) == 1
+    assert autotune.pick_fused([dict(fused=1, rate=1.0), dict(fused=0, rate=1.05)]) == 0
     # the list entry width picks 16 unless 32 is clearly faster
     assert autotune.pick_list_bits([dict(list_bits=16, rate=1.0), dict(list_bits=32, rate=1.005)]) == 16
     assert autotune.pick_list_bits([dict(list_bits=16, rate=1.0), dict(list_bits=32, rate=1.05)]) == 32
