@@ -160,6 +160,8 @@ struct ArenaView {
     int *tile_list;
     double *wdisp;       // per 64-slot wave wholly inside one box: its largest squared
                          // displacement since the build, stored on the box's review step
+    double2 *wbuild;     // per 64-slot wave wholly inside one building box: its largest
+                         // squared predicted speed and acceleration, stored at the build
     // boxes
     const BoxDesc *boxes;
     BoxDyn *dyn;

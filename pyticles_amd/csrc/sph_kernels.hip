@@ -455,6 +455,10 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_build_plan(ArenaView A, int fo
         if (sg.box >= 0 && building(A.dyn[sg.box], force_mask)) {
             A.box_coff[sg.box] = pos;
             pos += sg.len;
+            // the shared waves' part of this build's maxima (build_tail)
+            // starts from zero, as the whole waves' wbuild entries do
+            A.dyn[sg.box].vmax2 = 0ull;
+            A.dyn[sg.box].amax2 = 0ull;
         }
     }
     if (tid == PLAN_THREADS - 1) A.plan_n[1] = ss[tid];
@@ -888,18 +892,21 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
 // Neighbour list of a building slot: every other slot of the 3x3 cell
 // stencil with r^2 < (r_cut + skin)^2, short entries (r^2 < (h + skin)^2)
 // from the front, the rest from the back.  The sorted state is copied back
-// into the state arrays, the build position kept for the skin check, and the
-// box's largest speed and acceleration recorded for its next schedule.
+// into the state arrays and the build position kept for the skin check.
+// Returns the slot's box (-1: not building) and, in v2 / a2, its squared
+// predicted speed and acceleration, which build_tail folds into the box's
+// maxima for its next schedule.
 //
 // A wave covers one 64-slot list tile, so every tile is written by one wave.
 // (Over compact positions, which are not tile-aligned, two waves, often on
 // two XCDs with their own L2, wrote most tiles and the kernel ran 2.4x
 // slower.)
-__device__ __forceinline__ void build_slot(const ArenaView &A, int s, int force_mask, int fp32, unsigned *buf)
+__device__ __forceinline__ int build_slot(const ArenaView &A, int s, int force_mask, int fp32, unsigned *buf,
+                                          double &v2, double &a2)
 {
-    if (s >= A.n) return;
+    if (s >= A.n) return -1;
     const int b = building_box(A, s, force_mask);
-    if (b < 0) return;
+    if (b < 0) return -1;
     BoxDyn &D = A.dyn[b];
     const BoxDesc &B = A.boxes[b];
     const double2 ri = A.r_s[s];
@@ -940,8 +947,50 @@ __device__ __forceinline__ void build_slot(const ArenaView &A, int s, int force_
     if (over) D.relist = 1;
     if (s == B.part_off) D.list16 = B.build16;  // the form the walks read
     A.cnt[s] = make_int2(ns | (wrapped ? CNT_WRAP : 0), nl);
-    max_bits(&D.vmax2, vpi.x * vpi.x + vpi.y * vpi.y);
-    max_bits(&D.amax2, ai.x * ai.x + ai.y * ai.y);
+    v2 = vpi.x * vpi.x + vpi.y * vpi.y;
+    a2 = ai.x * ai.x + ai.y * ai.y;
+    return b;
+}
+
+// The build's largest squared speed and acceleration per box, as step_tail
+// gathers the review displacement: a wave wholly inside one building box
+// stores its maxima in wbuild (k_box_step folds the box's waves); a wave
+// shared by several boxes folds each building box's lanes into that box's
+// dyn.vmax2 / amax2 with one atomic each.  (A max_bits per lane put up to
+// 2 x 64 atomics per wave on the box's two words, and while the running
+// maxima were still low most lanes issued theirs: the in-run builds of a
+// 262,144-slot box ran 10-200 us over the 76 us of one at a settled
+// maximum.)  Every lane of the wave calls this.
+__device__ __forceinline__ void build_tail(const ArenaView &A, int s, int b, double v2, double a2)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int bfirst = __shfl(b, 0);
+    if (bfirst >= 0 && bfirst == __shfl(b, 63)) {  // wholly inside one box
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            v2 = fmax(v2, __shfl_xor(v2, off));
+            a2 = fmax(a2, __shfl_xor(a2, off));
+        }
+        if (lane == 0) A.wbuild[s >> 6] = make_double2(v2, a2);
+        return;
+    }
+    uint64_t pending = __ballot(b >= 0);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int bu = __shfl(b, leader);
+        const bool mine = b == bu;
+        double mv = mine ? v2 : 0.0, ma = mine ? a2 : 0.0;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            mv = fmax(mv, __shfl_xor(mv, off));
+            ma = fmax(ma, __shfl_xor(ma, off));
+        }
+        if (lane == leader) {
+            max_bits(&A.dyn[bu].vmax2, mv);
+            max_bits(&A.dyn[bu].amax2, ma);
+        }
+        pending &= ~__ballot(mine);
+    }
 }
 
 // Over the building tiles (for_building_tiles).
@@ -951,7 +1000,11 @@ __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask,
     // one column per lane: word w of lane t at lbuf[w * NT + t],
     // bank-conflict free
     __shared__ unsigned lbuf[8 * NT];
-    for_building_tiles(A, [&](int s) { build_slot(A, s, force_mask, fp32, lbuf + threadIdx.x); });
+    for_building_tiles(A, [&](int s) {
+        double v2 = 0.0, a2 = 0.0;
+        const int b = build_slot(A, s, force_mask, fp32, lbuf + threadIdx.x, v2, a2);
+        build_tail(A, s, b, v2, a2);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -2046,11 +2099,10 @@ __global__ __launch_bounds__(NT) void k_leapfrog(ArenaView A, int fp32)
 
 // Steps a box may reuse the list it just built: the largest c with
 //   c dt vmax + (c dt)^2 amax <= 0.4 skin    (twice the kinematic bound on
-// the acceleration term, 80 % of the half skin), plus the build step.
-__device__ __forceinline__ int64_t reuse_steps(const BoxDesc &B, const BoxDyn &D)
+// the acceleration term, 80 % of the half skin), plus the build step; vmax,
+// amax the largest speed and acceleration at the build.
+__device__ __forceinline__ int64_t reuse_steps(const BoxDesc &B, double vmax, double amax)
 {
-    const double vmax = sqrt(__longlong_as_double((long long)D.vmax2));
-    const double amax = sqrt(__longlong_as_double((long long)D.amax2));
     const double C = 0.4 * B.skin, Bq = B.dt * vmax, Aq = B.dt * B.dt * amax;
     double c;
     if (Aq > 0.0) c = (-Bq + sqrt(Bq * Bq + 4.0 * Aq * C)) / (2.0 * Aq);
@@ -2104,8 +2156,35 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
     const int64_t k = D.k, K = D.nsteps;
     const bool capture = state == BOX_RUN && (k == 0 || k == K);
     const bool review = list_review(D);
+    const bool built = building(D, 0);
     __shared__ double red[NT_BOX];
-    double dmax2 = 0.0;
+    double dmax2 = 0.0, vmax2 = 0.0, amax2 = 0.0;
+    if (built) {  // the build's largest squared speed and acceleration: the box's whole waves, then its shared ones
+        const BoxDesc &B = A.boxes[b];
+        const int w0 = (B.part_off + 63) >> 6, w1 = ((B.part_off + B.n) >> 6) - 1;
+        double mv = 0.0, ma = 0.0;
+        for (int w = w0 + (int)threadIdx.x; w <= w1; w += NT_BOX) {
+            const double2 x = A.wbuild[w];
+            mv = fmax(mv, x.x);
+            ma = fmax(ma, x.y);
+        }
+        red[threadIdx.x] = mv;
+        __syncthreads();
+        for (int off = NT_BOX / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+            __syncthreads();
+        }
+        vmax2 = fmax(red[0], __longlong_as_double((long long)D.vmax2));
+        __syncthreads();
+        red[threadIdx.x] = ma;
+        __syncthreads();
+        for (int off = NT_BOX / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+            __syncthreads();
+        }
+        amax2 = fmax(red[0], __longlong_as_double((long long)D.amax2));
+        __syncthreads();  // red is reused below
+    }
     if (review) {  // largest squared displacement: the box's whole waves, then its shared ones
         const BoxDesc &B = A.boxes[b];
         const int w0 = (B.part_off + 63) >> 6, w1 = ((B.part_off + B.n) >> 6) - 1;
@@ -2145,13 +2224,13 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
         if (k == 0) D.E0 = red[0];
         else D.E1 = red[0];
     }
-    if (building(D, 0)) {
-        const int64_t nK = reuse_steps(A.boxes[b], D);
+    if (built) {
+        D.vlast = sqrt(vmax2);
+        D.alast = sqrt(amax2);
+        const int64_t nK = reuse_steps(A.boxes[b], D.vlast, D.alast);
         D.next_rebuild = (state == BOX_INIT ? 0 : k) + nK;
         D.builds += 1;
         D.built_at = state == BOX_INIT ? 0 : k;
-        D.vlast = sqrt(__longlong_as_double((long long)D.vmax2));
-        D.alast = sqrt(__longlong_as_double((long long)D.amax2));
         D.vmax2 = 0ull;
         D.amax2 = 0ull;
     } else if (review) {

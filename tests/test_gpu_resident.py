@@ -138,24 +138,30 @@ def test_host_changes_are_uploaded_between_updates():
 
 
 def test_update_costs_what_run_costs():
-    # one box of C2's size: 100 update() calls against one run(100)
+    # one box of C2's size: 100 update() calls against one run(100), three
+    # times each, alternating; the best of each (a single pair of ~20 ms
+    # windows is within a few ms of the bound on host noise alone)
     n_steps = 100
     a = _surface_system(512, seed=35)
     a.run(5, 0.005)  # device box, graphs and schedule set up
-    t = time.perf_counter()
-    a.run(n_steps, 0.005)
-    t_run = time.perf_counter() - t
     b = _surface_system(512, seed=35)
     b.run(5, 0.005)
-    # the fetch after run() pondered the host list (never built, so flagged):
-    # the reference's update() would rebuild it first (particles.py:471-472);
-    # do that before the timed loop, which then holds device steps only
-    b.rebuild_lists()
-    assert not b.nlists[0].rebuild_list
-    t = time.perf_counter()
-    for _ in range(n_steps):
-        b.update(0.005)
-    _ = b.v  # closes the run and fetches the state, as run() does
-    t_upd = time.perf_counter() - t
-    print("[resident] run(100) %.2f ms, 100 x update() %.2f ms" % (1e3 * t_run, 1e3 * t_upd), flush=True)
-    assert t_upd < 1.2 * t_run, (t_upd, t_run)
+    t_run, t_upd = [], []
+    for _ in range(3):
+        t = time.perf_counter()
+        a.run(n_steps, 0.005)
+        t_run.append(time.perf_counter() - t)
+        # the fetch after run() or a read pondered the host list (never
+        # built, so flagged): the reference's update() would rebuild it
+        # first (particles.py:471-472); do that before the timed loop, which
+        # then holds device steps only
+        b.rebuild_lists()
+        assert not b.nlists[0].rebuild_list
+        t = time.perf_counter()
+        for _ in range(n_steps):
+            b.update(0.005)
+        _ = b.v  # closes the run and fetches the state, as run() does
+        t_upd.append(time.perf_counter() - t)
+    print("[resident] run(100) %s ms, 100 x update() %s ms"
+          % (["%.2f" % (1e3 * x) for x in t_run], ["%.2f" % (1e3 * x) for x in t_upd]), flush=True)
+    assert min(t_upd) < 1.2 * min(t_run), (t_upd, t_run)
