@@ -316,6 +316,9 @@ struct Arena {
         precision = prec;
         lcap = list_entries;
         if (lcap <= 0 || lcap % 8) return fail(SPH_ERR_ARG, "list entries must be a positive multiple of 8");
+        // the pair walks gather neighbour records (up to 32 bytes) through
+        // 32-bit byte offsets
+        if (cap > (1 << 27)) return fail(SPH_ERR_ARG, "arena capacity above 2^27 slots");
         int rc = 0;
         rc |= dalloc(&r, cap); rc |= dalloc(&v, cap); rc |= dalloc(&vp, cap); rc |= dalloc(&acc, cap);
         rc |= dalloc(&r_nx, cap); rc |= dalloc(&vp_nx, cap);

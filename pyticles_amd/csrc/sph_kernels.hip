@@ -286,6 +286,18 @@ struct ListRounds {
     }
 };
 
+// Record j of array p through a 32-bit byte offset from the array's base,
+// which the walks hold in scalar registers: the gather is then a
+// global_load with a scalar base and a 32-bit vector offset, one shift per
+// array, where a 64-bit index took a sign extension, a 64-bit shift and a
+// 64-bit add.  Slot numbers stay below 2^32 / sizeof(T): the arena holds
+// at most 2^27 slots, and the largest record gathered is 32 bytes.
+template <class T>
+__device__ __forceinline__ const T &at32(const T *p, int j)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (uint32_t)j * (uint32_t)sizeof(T));
+}
+
 // Minimum image exactly as neighbour_list.py:110-123 (all '>' tests, then '<').
 template <class Box>
 __device__ __forceinline__ void min_image(double &dx, double &dy, const Box &B)
@@ -1121,9 +1133,9 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
-            rj[u] = A.r[t[u]];
-            if (!UNIM) mj[u] = A.m[t[u]];
-            if (COH) vv[u] = A.vp[t[u]];
+            rj[u] = at32(A.r, t[u]);
+            if (!UNIM) mj[u] = at32(A.m, t[u]);
+            if (COH) vv[u] = at32(A.vp, t[u]);
         }
         rounds.next(k0);
 #pragma unroll
@@ -1161,9 +1173,9 @@ __device__ __forceinline__ Dens64 density64_walk(const ArenaView &A, const BoxK 
         lrounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
-            rj[u] = A.r[t[u]];
-            if (!UNIM) mj[u] = A.m[t[u]];
-            if (COH) vv[u] = A.vp[t[u]];
+            rj[u] = at32(A.r, t[u]);
+            if (!UNIM) mj[u] = at32(A.m, t[u]);
+            if (COH) vv[u] = at32(A.vp, t[u]);
         }
         lrounds.next(k0);
 #pragma unroll
@@ -1277,9 +1289,9 @@ __device__ __forceinline__ Grad64 grad64_walk(const ArenaView &A, const BoxK &K,
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
-            rj[u] = A.r[t[u]];
-            vv[u] = A.vp[t[u]];
-            vt[u] = A.vt[t[u]];
+            rj[u] = at32(A.r, t[u]);
+            vv[u] = at32(A.vp, t[u]);
+            vt[u] = at32(A.vt, t[u]);
         }
         rounds.next(k0);
 #pragma unroll
@@ -1374,10 +1386,10 @@ __device__ __forceinline__ Rate64 rates64_walk(const ArenaView &A, const BoxK &K
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U64; ++u) {
-            rj[u] = A.r[t[u]];
-            vv[u] = A.vp[t[u]];
-            Sj[u] = A.S[t[u]];
-            if (COND) Qj[u] = A.Q[t[u]];
+            rj[u] = at32(A.r, t[u]);
+            vv[u] = at32(A.vp, t[u]);
+            Sj[u] = at32(A.S, t[u]);
+            if (COND) Qj[u] = at32(A.Q, t[u]);
         }
         rounds.next(k0);
 #pragma unroll
@@ -1555,9 +1567,9 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
-            rj[u] = A.rf[t[u]];
-            lj[u] = A.rlo[t[u]];
-            if (!UNIM) mm[u] = (float)A.m[t[u]];
+            rj[u] = at32(A.rf, t[u]);
+            lj[u] = at32(A.rlo, t[u]);
+            if (!UNIM) mm[u] = (float)at32(A.m, t[u]);
         }
         rounds.next(k0);
         float grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
@@ -1603,9 +1615,9 @@ __device__ __forceinline__ void density32_walk(const ArenaView &A, const BoxF &F
         lrounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
-            rj[u] = A.rf[t[u]];
-            if (LOLONG) lj[u] = A.rlo[t[u]];
-            if (!UNIM) mm[u] = (float)A.m[t[u]];
+            rj[u] = at32(A.rf, t[u]);
+            if (LOLONG) lj[u] = at32(A.rlo, t[u]);
+            if (!UNIM) mm[u] = (float)at32(A.m, t[u]);
         }
         lrounds.next(k0);
         float grhol = 0.f, ggx = 0.f, ggy = 0.f, gev = 0.f;
@@ -1733,10 +1745,10 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, c
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
-            rj[u] = A.rf[t[u]];
-            lj[u] = A.rlo[t[u]];
-            Vj[u] = A.mt32[t[u]].x;
-            Tj[u] = A.T[t[u]];
+            rj[u] = at32(A.rf, t[u]);
+            lj[u] = at32(A.rlo, t[u]);
+            Vj[u] = at32(A.mt32, t[u]).x;
+            Tj[u] = at32(A.T, t[u]);
         }
         rounds.next(k0);
 #pragma unroll
@@ -1827,10 +1839,10 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
-            rj[u] = A.rf[t[u]];
-            lj[u] = A.rlo[t[u]];
-            Sj[u] = A.S32[t[u]];
-            qj[u] = COND ? A.q32[t[u]] : make_float2(0.f, 0.f);
+            rj[u] = at32(A.rf, t[u]);
+            lj[u] = at32(A.rlo, t[u]);
+            Sj[u] = at32(A.S32, t[u]);
+            qj[u] = COND ? at32(A.q32, t[u]) : make_float2(0.f, 0.f);
         }
         rounds.next(k0);
 #pragma unroll
