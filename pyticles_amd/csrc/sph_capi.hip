@@ -77,7 +77,7 @@ int bits_for(int nkeys)
 namespace {
 constexpr int64_t NOALLOC_MAX_PARTICLES = 65536;
 constexpr int NOALLOC_MAX_CELLS = 65536;
-constexpr size_t NOALLOC_SORT_BYTES = 64 * 1024;  // radix-sort counters of 65,536 keys: 33 KB
+constexpr size_t NOALLOC_SORT_BYTES = radix_sort_temp_bytes(NOALLOC_MAX_PARTICLES);  // 129 KB
 constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 constexpr size_t NOALLOC_REC_BYTES = align256(sizeof(double4) * NOALLOC_MAX_PARTICLES);
 constexpr size_t NOALLOC_KEY_BYTES = align256(sizeof(unsigned) * 5 * NOALLOC_MAX_PARTICLES);

@@ -12,8 +12,9 @@ namespace sph {
 
 static constexpr int RS_THREADS = 256;           // 4 waves of 64
 static constexpr int RS_WAVES = RS_THREADS / 64;
-static constexpr int RS_ITEMS = 8;               // items per thread per tile
+static constexpr int RS_ITEMS = RADIX_TILE_KEYS / RS_THREADS;  // items per thread per tile
 static constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+static_assert(RS_TILE == RADIX_TILE_KEYS, "a tile is whole items per thread");
 static constexpr int RS_BINS = 256;
 
 __device__ __forceinline__ uint64_t lanemask_lt()
@@ -159,12 +160,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
         }
         __syncthreads();
     }
-}
-
-size_t radix_sort_temp_bytes(int n)
-{
-    const int nb = (n + RS_TILE - 1) / RS_TILE;
-    return (size_t)RS_BINS * (nb > 0 ? nb : 1) * sizeof(unsigned) + RS_BINS * sizeof(unsigned);
 }
 
 // Sorts (keys_in, implicit identity values) by the low key_bits bits.
