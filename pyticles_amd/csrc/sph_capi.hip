@@ -663,9 +663,8 @@ struct Arena {
         } else {
             launch_keys(A, force_mask, st);
             radix_sort_pairs(keys_in, k_out, v_out, k_alt, v_alt, hwm, bits_for(nkeys), sort_tmp, st, plan_n + 1);
-            launch_cell_start(A, force_mask, st);
         }
-        launch_permute(A, force_mask, st);
+        launch_permute(A, force_mask, small_boxes ? 0 : 1, st);  // with the cell starts after the radix sort
         launch_build_lists(A, force_mask, fp32() ? 1 : 0, st);
     }
 

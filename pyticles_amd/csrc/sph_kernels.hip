@@ -442,6 +442,34 @@ __device__ __forceinline__ int seg_tiles(const ArenaView &A, int g, int force_ma
     return t1 >= t0 ? t1 - t0 + 1 : 0;
 }
 
+// Inclusive scan of x over the block into ss[tid]: shuffle scans within
+// the waves, then one wave scans the wave totals (three barriers; a
+// Hillis-Steele scan in LDS took twenty).
+__device__ __forceinline__ void plan_scan(int x, int *ss, int *wsum)
+{
+    constexpr int NW = PLAN_THREADS / 64;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        int v = lane < NW ? wsum[lane] : 0;
+#pragma unroll
+        for (int off = 1; off < NW; off <<= 1) {
+            const int y = __shfl_up(v, off);
+            if (lane >= off) v += y;
+        }
+        if (lane < NW) wsum[lane] = v;
+    }
+    __syncthreads();
+    ss[tid] = x + (w > 0 ? wsum[w - 1] : 0);
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(PLAN_THREADS) void k_build_plan(ArenaView A, int force_mask)
 {
     __shared__ int ss[PLAN_THREADS];
@@ -453,14 +481,8 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_build_plan(ArenaView A, int fo
         const SegDesc sg = A.segs[g];
         if (sg.box >= 0 && building(A.dyn[sg.box], force_mask)) sz += sg.len;
     }
-    ss[tid] = sz;
-    __syncthreads();
-    for (int off = 1; off < PLAN_THREADS; off <<= 1) {  // inclusive scan
-        const int c = tid >= off ? ss[tid - off] : 0;
-        __syncthreads();
-        ss[tid] += c;
-        __syncthreads();
-    }
+    __shared__ int wsum[PLAN_THREADS / 64];
+    plan_scan(sz, ss, wsum);
     int pos = ss[tid] - sz;
     for (int g = g0; g < g1; ++g) {
         const SegDesc sg = A.segs[g];
@@ -488,14 +510,7 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_build_plan(ArenaView A, int fo
         }
         nt += c;
     }
-    ss[tid] = nt;
-    __syncthreads();
-    for (int off = 1; off < PLAN_THREADS; off <<= 1) {
-        const int c = tid >= off ? ss[tid - off] : 0;
-        __syncthreads();
-        ss[tid] += c;
-        __syncthreads();
-    }
+    plan_scan(nt, ss, wsum);
     const int T = ss[PLAN_THREADS - 1];
     if (cached) {
         int run = ss[tid] - nt;
@@ -632,11 +647,6 @@ __device__ __forceinline__ void permute_slot(const ArenaView &A, int s, int forc
     A.pcell[s] = (cell < 0 || cell >= B.ncx * B.ncy) ? 0 : cell;
 }
 
-__global__ __launch_bounds__(NT) void k_permute(ArenaView A, int force_mask)
-{
-    for_building_tiles(A, [&](int s) { permute_slot(A, s, force_mask); });
-}
-
 // cell_start over the key range of each building box: cell_start[c] = first
 // slot whose key >= c, and the end marker key_base + ncx*ncy = the slot past
 // the box (the next segment's first slot: a building next box writes the same
@@ -657,14 +667,20 @@ __device__ __forceinline__ void cell_start_slot(const ArenaView &A, int s, int f
     }
 }
 
-__global__ __launch_bounds__(NT) void k_cell_start(ArenaView A, int force_mask)
+// The permutation, and with cells the cell starts from the sorted keys
+// too (the radix-sort path; the per-box sort writes its own), in one
+// launch: both walk the building slots and read only the sort's output.
+__global__ __launch_bounds__(NT) void k_permute(ArenaView A, int force_mask, int cells)
 {
-    for_building_tiles(A, [&](int s) { cell_start_slot(A, s, force_mask); });
+    for_building_tiles(A, [&](int s) {
+        if (cells) cell_start_slot(A, s, force_mask);
+        permute_slot(A, s, force_mask);
+    });
 }
 
 // Per-box counting sort for arenas of small boxes (every box with at most
 // BS_MAX_CELLS cells and BS_MAX_N slots): one block per building box does
-// what k_keys, the radix sort and k_cell_start do for it, in LDS.  The order
+// what k_keys, the radix sort and the cell starts do for it, in LDS.  The order
 // is the same stable one (by cell, then by compact position), so the two
 // pipelines give identical lists.  Ranks come from wave ballots and per-wave
 // counters (each wave owns one counter row and walks its slice of the box in
@@ -2650,9 +2666,9 @@ void launch_keys(const ArenaView &A, int force_mask, hipStream_t st)
 {
     hipLaunchKernelGGL(k_keys, tile_grid(A), dim3(NT), 0, st, A, force_mask);
 }
-void launch_permute(const ArenaView &A, int force_mask, hipStream_t st)
+void launch_permute(const ArenaView &A, int force_mask, int cells, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_permute, tile_grid(A), dim3(NT), 0, st, A, force_mask);
+    hipLaunchKernelGGL(k_permute, tile_grid(A), dim3(NT), 0, st, A, force_mask, cells);
 }
 void launch_box_sort(const ArenaView &A, int force_mask, int nbox, hipStream_t st)
 {
@@ -2661,10 +2677,6 @@ void launch_box_sort(const ArenaView &A, int force_mask, int nbox, hipStream_t s
 // one block sorts a box: beyond ~16k slots its sequential walk outlasts the
 // radix sort (measured on the served workload, boxes up to 40k slots)
 bool box_sort_fits(int ncells, int n) { return ncells <= BS_MAX_CELLS && n <= 16384; }
-void launch_cell_start(const ArenaView &A, int force_mask, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_cell_start, tile_grid(A), dim3(NT), 0, st, A, force_mask);
-}
 void launch_build_lists(const ArenaView &A, int force_mask, int fp32, hipStream_t st)
 {
     hipLaunchKernelGGL(k_build_lists, tile_grid(A), dim3(NT), 0, st, A, force_mask, fp32);

@@ -16,8 +16,8 @@ constexpr int MASK_ALL = (1 << BOX_INIT) | (1 << BOX_RUN) | (1 << BOX_DONE);
 void launch_slot_box(const ArenaView &A, hipStream_t st);
 void launch_build_plan(const ArenaView &A, int force_mask, hipStream_t st);
 void launch_keys(const ArenaView &A, int force_mask, hipStream_t st);
-void launch_permute(const ArenaView &A, int force_mask, hipStream_t st);
-void launch_cell_start(const ArenaView &A, int force_mask, hipStream_t st);
+// cells != 0: also the cell starts from the sorted keys (radix-sort path)
+void launch_permute(const ArenaView &A, int force_mask, int cells, hipStream_t st);
 // arenas whose boxes all fit box_sort_fits: per-box counting sort in place of
 // keys + radix sort + cell starts (same order)
 void launch_box_sort(const ArenaView &A, int force_mask, int nbox, hipStream_t st);
