@@ -793,19 +793,29 @@ __global__ __launch_bounds__(BS_THREADS) void k_box_sort(ArenaView A, int force_
 
 // Candidate scan of one building slot over the stencil cells of its build
 // cell, in stencil order and slot order within a cell (the list order).
-// Four candidate positions are loaded before any is tested; SINGLE: a box
-// with a single-cell axis, whose candidates need the minimum image.
-// Entries collect in two G-entry buffers (short from position 0 up, long
-// from lcap - 1 down), each group stored with one 16-byte store when its
-// last offset fills: G = 4 slots, or with L16 G = 8 offsets of the 16-bit
-// form.  The partial groups are stored at the end, merged when they share a
-// group.  Entries that do not fit (ns + nl = lcap) set lost.  The buffers
-// are the lane's column of an LDS array (words buf[w * NT], w = 0-3 short,
-// 4-7 long; halfwords in the 16-bit form), so an append is one indexed LDS
-// write on one path for short and long entries; in registers each append
-// took a select chain per buffer, and the wave ran both chains whenever its
-// lanes split.  Positions past a count hold stale words, which the walks
-// never read (ListRounds maps them to the slot itself).
+// SINGLE: a box with a single-cell axis, whose candidates need the minimum
+// image.  Four candidate positions are loaded before any is tested.
+// (Scanning a stencil row's consecutive cells as one slot range, in fewer
+// and fuller rounds, measured slower: its range set-up cost more registers
+// and instructions than the part-empty rounds it saved.)
+//
+// Entries collect in a ring of two G-entry cells per side (short entries
+// from position 0 up, long ones from lcap - 1 down; G = 4 slots, or with L16
+// G = 8 offsets of the 16-bit form): ring index sh ? ns % 2G : 4G-1 - nl % 2G,
+// so a long cell sits in its half in list order.  A round appends at most
+// four entries per side, so at most one cell per side completes in it, and
+// the round's end stores the completed cells, one 16-byte store each, while
+// the other half fills.  The partial cells are stored at the end, merged
+// when they share a cell.  Entries that do not fit (ns + nl = lcap) set
+// lost.  The ring is the lane's column of an LDS array (words buf[w * NT],
+// 0-7 short, 8-15 long), so an append is one indexed LDS write on one path
+// for short and long entries.  (Stored from inside the candidate loop, as
+// each cell completed, the store path ran for almost every candidate, some
+// lane of the wave completing a cell at nearly every one: 20% more VALU
+// instructions per round, and the C2 build stage took 122 us against 115.)
+// Positions past a
+// count hold stale words, which the walks never read (ListRounds maps them
+// to the slot itself).
 //
 // 16-bit offsets: a stencil cell is a contiguous slot range, so one base
 // serves all its candidates: the slot offset t - s when the cell's offsets
@@ -820,7 +830,7 @@ template <bool SINGLE, bool L16>
 __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu, int s, double2 ri, int cell,
                                           unsigned *buf, int &ns, int &nl, bool &lost, bool &wrapped, bool &over)
 {
-    constexpr int G = L16 ? 8 : 4;  // entries per group store
+    constexpr int G = L16 ? 8 : 4;  // entries per cell (one 16-byte store)
     const int lcap = A.lcap;
     const int nb = wave_uniform(Bu.n);
     const double rnl2 = wave_uniform(Bu.rnl2), hsk2 = wave_uniform(Bu.hsk2);
@@ -830,13 +840,67 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
     const int sx = wave_uniform(Bu.sx), sy = wave_uniform(Bu.sy);
     const int *cs = A.cell_start + wave_uniform(Bu.key_base);
     const int cx = cell % ncx, cy = cell / ncx;
+    int fs = 0, fl = 0;  // cells stored per side
     // the list address is formed at each store (not held across the loop)
-    auto put = [&](int grp, uint4 g) {
-        if (L16) *(uint4 *)(list_base16(A, s) + (size_t)grp * CELL16) = g;
-        else *(uint4 *)(list_base(A, s) + (size_t)grp * LIST_ROW) = g;
+    auto put = [&](int q, uint4 g) {
+        if (L16) *(uint4 *)(list_base16(A, s) + (size_t)q * CELL16) = g;
+        else *(uint4 *)(list_base(A, s) + (size_t)q * LIST_ROW) = g;
     };
-    auto store = [&](int grp, int w0) {
-        put(grp, make_uint4(buf[w0 * NT], buf[(w0 + 1) * NT], buf[(w0 + 2) * NT], buf[(w0 + 3) * NT]));
+    auto ring = [&](int w0) { return make_uint4(buf[w0 * NT], buf[(w0 + 1) * NT], buf[(w0 + 2) * NT], buf[(w0 + 3) * NT]); };
+    // 16-bit base of the cell's candidates (0 in the 32-bit form)
+    auto base16 = [&](int t0, int t1) {
+        int ks = 0;
+        if (L16 && t1 > t0) {
+            const int dlo = t0 - s, dhi = t1 - 1 - s, dm = (dlo + dhi) >> 1;
+            int k = 0;  // the plain offsets when they fit, else the nearest modulo n
+            if (dlo < -32768 || dhi > 32767) k = 2 * dm > nb ? -nb : (2 * dm < -nb ? nb : 0);
+            if (dlo + k < -32768 || dhi + k > 32767) over = true;
+            if (k) wrapped = true;
+            ks = k - s;
+        }
+        return ks;
+    };
+    // candidates t0 .. t1 - 1 at image offset (rx, ry) from r_i
+    auto scan = [&](int t0, int t1, double rx, double ry, int ks) {
+        for (int t = t0; t < t1; t += 4) {
+            double2 rj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rj[u] = at32(A.r_s, min(t + u, t1 - 1));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                double dx = rj[u].x - rx, dy = rj[u].y - ry;
+                if (SINGLE && sx == 0) {
+                    dx = dx > hLx ? dx - Lx : dx;
+                    dx = dx < -hLx ? dx + Lx : dx;
+                }
+                if (SINGLE && sy == 0) {
+                    dy = dy > hLy ? dy - Ly : dy;
+                    dy = dy < -hLy ? dy + Ly : dy;
+                }
+                const double r2 = rsq_exact(dx, dy);
+                if (t + u < t1 && r2 < rnl2 && t + u != s) {
+                    if (ns + nl >= lcap) {
+                        lost = true;
+                    } else {
+                        const bool sh = r2 < hsk2;
+                        const int row = sh ? (ns & (2 * G - 1)) : (4 * G - 1) - (nl & (2 * G - 1));
+                        if (L16) ((unsigned short *)(buf + (row >> 1) * NT))[row & 1] = (unsigned short)(t + u + ks);
+                        else buf[row * NT] = (unsigned)(t + u);
+                        ns += sh ? 1 : 0;
+                        nl += sh ? 0 : 1;
+                    }
+                }
+            }
+            // the cells completed in this round (at most one per side)
+            if (ns / G > fs) {
+                put(fs, ring(4 * (fs & 1)));
+                ++fs;
+            }
+            if (nl / G > fl) {
+                put(lcap / G - 1 - fl, ring(12 - 4 * (fl & 1)));
+                ++fl;
+            }
+        }
     };
     for (int oy = -sy; oy <= sy; ++oy) {
         const int yr = cy + oy;
@@ -848,72 +912,28 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
             const double rx = ri.x - (xr < 0 ? -Lx : (xr >= ncx ? Lx : 0.0));
             const int c = yy * ncx + xx;
             const int t0 = cs[c], t1 = cs[c + 1];
-            int ks = 0;  // 16-bit entry of candidate t: t + ks
-            if (L16 && t1 > t0) {
-                const int dlo = t0 - s, dhi = t1 - 1 - s, dm = (dlo + dhi) >> 1;
-                int k = 0;  // the plain offsets when they fit, else the nearest modulo n
-                if (dlo < -32768 || dhi > 32767) k = 2 * dm > nb ? -nb : (2 * dm < -nb ? nb : 0);
-                if (dlo + k < -32768 || dhi + k > 32767) over = true;
-                if (k) wrapped = true;
-                ks = k - s;
-            }
-            for (int t = t0; t < t1; t += 4) {
-                double2 rj[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) rj[u] = A.r_s[min(t + u, t1 - 1)];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    double dx = rj[u].x - rx, dy = rj[u].y - ry;
-                    if (SINGLE && sx == 0) {
-                        dx = dx > hLx ? dx - Lx : dx;
-                        dx = dx < -hLx ? dx + Lx : dx;
-                    }
-                    if (SINGLE && sy == 0) {
-                        dy = dy > hLy ? dy - Ly : dy;
-                        dy = dy < -hLy ? dy + Ly : dy;
-                    }
-                    const double r2 = rsq_exact(dx, dy);
-                    if (t + u < t1 && r2 < rnl2 && t + u != s) {
-                        if (ns + nl >= lcap) {
-                            lost = true;
-                        } else {
-                            // short entry ns at offset ns % G (buffer slot
-                            // 0 .. G-1), long entry nl (position lcap - 1 - nl)
-                            // at offset G - 1 - nl % G (slot G .. 2G-1); one
-                            // path for both, the group stored when its last
-                            // offset fills
-                            const bool sh = r2 < hsk2;
-                            const int row = sh ? (ns & (G - 1)) : (2 * G - 1) - (nl & (G - 1));
-                            if (L16) ((unsigned short *)(buf + (row >> 1) * NT))[row & 1] = (unsigned short)(t + u + ks);
-                            else buf[row * NT] = (unsigned)(t + u);
-                            ns += sh ? 1 : 0;
-                            nl += sh ? 0 : 1;
-                            if (row == G - 1 || row == G)
-                                store(sh ? (ns - G) / G : (lcap - nl) / G, sh ? 0 : 4);
-                        }
-                    }
-                }
-            }
+            scan(t0, t1, rx, ry, base16(t0, t1));
         }
     }
-    // partial groups: short offsets [0, ns % G) of group ns / G, long
-    // offsets [G - nl % G, G) of group (lcap - nl) / G; with ns + nl <= lcap
-    // they never overlap, and a shared group is stored once, each offset
-    // taken from the list that owns it
-    const int ps = ns & (G - 1), pl = nl & (G - 1);
-    const int gs = ns / G, gl = (lcap - nl) / G;
-    if (ps && pl && gs == gl) {
+    // partial cells: short positions [0, ns % G) of cell ns / G, long
+    // positions [G - nl % G, G) of cell lcap / G - 1 - nl / G; with ns + nl
+    // <= lcap they never overlap, and a shared cell is stored once, each
+    // position taken from the side that owns it
+    const int ps = ns % G, pl = nl % G;
+    const int qs = fs, ql = lcap / G - 1 - fl;
+    const uint4 ws = ring(4 * (fs & 1)), wl = ring(12 - 4 * (fl & 1));
+    if (ps && pl && qs == ql) {
+        const unsigned sw[4] = {ws.x, ws.y, ws.z, ws.w}, lw[4] = {wl.x, wl.y, wl.z, wl.w};
         unsigned w[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const unsigned sw = buf[i * NT], lw = buf[(4 + i) * NT];
-            if (L16) w[i] = ((2 * i < ps ? sw : lw) & 0xffffu) | ((2 * i + 1 < ps ? sw : lw) & 0xffff0000u);
-            else w[i] = i < ps ? sw : lw;
+            if (L16) w[i] = ((2 * i < ps ? sw[i] : lw[i]) & 0xffffu) | ((2 * i + 1 < ps ? sw[i] : lw[i]) & 0xffff0000u);
+            else w[i] = i < ps ? sw[i] : lw[i];
         }
-        put(gs, make_uint4(w[0], w[1], w[2], w[3]));
+        put(qs, make_uint4(w[0], w[1], w[2], w[3]));
     } else {
-        if (ps) store(gs, 0);
-        if (pl) store(gl, 4);
+        if (ps) put(qs, ws);
+        if (pl) put(ql, wl);
     }
 }
 
@@ -1024,10 +1044,10 @@ __device__ __forceinline__ void build_tail(const ArenaView &A, int s, int b, dou
 // Over the building tiles (for_building_tiles).
 __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask, int fp32)
 {
-    // each lane's two groups being filled (short, long: four words each),
-    // one column per lane: word w of lane t at lbuf[w * NT + t],
+    // each lane's entry ring (two cells per side: eight words short, eight
+    // long), one column per lane: word w of lane t at lbuf[w * NT + t],
     // bank-conflict free
-    __shared__ unsigned lbuf[8 * NT];
+    __shared__ unsigned lbuf[16 * NT];
     for_building_tiles(A, [&](int s) {
         double v2 = 0.0, a2 = 0.0;
         const int b = build_slot(A, s, force_mask, fp32, lbuf + threadIdx.x, v2, a2);
