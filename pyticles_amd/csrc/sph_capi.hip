@@ -152,6 +152,7 @@ static size_t arena_bytes(size_t cap, int prec, int lcap)
     b += ((cap + 63) / 64 + 1) * i1;             // building tiles
     b += (cap + 63) / 64 * sizeof(double);       // per-wave displacement
     b += (cap + 63) / 64 * sizeof(double2);      // per-wave build speed, acceleration
+    b += (cap + 63) / 64 * sizeof(double);       // per-wave energy
     b += radix_sort_temp_bytes((int)cap);
     return b;
 }
@@ -197,6 +198,7 @@ struct Arena {
     int *tile_list = nullptr;                     // building tiles, one per 64 slots
     double *wdisp = nullptr;                      // one per 64 slots
     double2 *wbuild = nullptr;                    // one per 64 slots
+    double *wenergy = nullptr;                    // one per 64 slots
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int *cell_start = nullptr;
@@ -299,7 +301,7 @@ struct Arena {
         A.vt = vt; A.aco = aco; A.uco = uco;
         A.cell_start = cell_start;
         A.keys = k_out; A.vals = v_out; A.keys_in = keys_in;
-        A.plan_n = plan_n; A.box_coff = box_coff; A.tile_list = tile_list; A.wdisp = wdisp; A.wbuild = wbuild;
+        A.plan_n = plan_n; A.box_coff = box_coff; A.tile_list = tile_list; A.wdisp = wdisp; A.wbuild = wbuild; A.wenergy = wenergy;
         A.boxes = d_boxes; A.dyn = d_dyn; A.segs = d_segs;
         A.nseg = (int)segs.size();
         A.n = hwm;
@@ -347,6 +349,7 @@ struct Arena {
         rc |= dalloc(&tile_list, (size_t)(cap + 63) / 64 + 1);
         rc |= dalloc(&wdisp, (size_t)(cap + 63) / 64);
         rc |= dalloc(&wbuild, (size_t)(cap + 63) / 64);
+        rc |= dalloc(&wenergy, (size_t)(cap + 63) / 64);
         if (rc) return SPH_ERR_NOMEM;
         // zero-fill the state so that unused slots hold finite values
         size_t z2 = sizeof(double2) * cap, z1 = sizeof(double) * cap;
@@ -377,7 +380,7 @@ struct Arena {
         dfree(rf); dfree(rlo); dfree(S32); dfree(mt32); dfree(q32);
         dfree(vt); dfree(aco); dfree(uco);
         dfree(keys_in); dfree(k_out); dfree(v_out); dfree(k_alt); dfree(v_alt);
-        dfree(plan_n); dfree(box_coff); dfree(tile_list); dfree(wdisp); dfree(wbuild);
+        dfree(plan_n); dfree(box_coff); dfree(tile_list); dfree(wdisp); dfree(wbuild); dfree(wenergy);
         if (sort_tmp) (void)hipFree(sort_tmp);
         sort_tmp = nullptr;
         if (snap_pending) (void)hipEventSynchronize(ev_snap_done);

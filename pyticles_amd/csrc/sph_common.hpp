@@ -162,6 +162,8 @@ struct ArenaView {
                          // displacement since the build, stored on the box's review step
     double2 *wbuild;     // per 64-slot wave wholly inside one building box: its largest
                          // squared predicted speed and acceleration, stored at the build
+    double *wenergy;     // per 64-slot wave wholly inside one box: the sum of its slots'
+                         // energies, stored on the box's first and last step
     // boxes
     const BoxDesc *boxes;
     BoxDyn *dyn;

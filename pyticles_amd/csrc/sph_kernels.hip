@@ -1926,8 +1926,10 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
 //
 // Leapfrog step of slot s of running box b with this evaluation's rates
 // (a, ud); returns the squared displacement since the box's last build (-1
-// at the closing half kick, which moves nothing).
-__device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, int s, int b, double2 a, double ud)
+// at the closing half kick, which moves nothing).  On the box's first and
+// last step, cap is set and e holds the slot's energy.
+__device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, int s, int b, double2 a, double ud,
+                                                 double &e, bool &cap)
 {
     const BoxDyn &D = A.dyn[b];
     const BoxDesc &B = A.boxes[b];
@@ -1935,14 +1937,20 @@ __device__ __forceinline__ double integrate_slot(const ArenaView &A, int fp32, i
     const double kick = (k == 0 || k == K) ? 0.5 * B.dt : B.dt;
     const double2 v = A.v[s];
     const double u = A.u[s], m = A.m[s];
-    if (k == 0) A.e[s] = m * (0.5 * (v.x * v.x + v.y * v.y) + u);
+    if (k == 0) {
+        e = m * (0.5 * (v.x * v.x + v.y * v.y) + u);
+        A.e[s] = e;
+        cap = true;
+    }
     const double vx = v.x + kick * a.x, vy = v.y + kick * a.y;
     const double un = u + kick * ud;
     A.v[s] = make_double2(vx, vy);
     A.u[s] = un;
     const double2 r = A.r[s];
     if (k == K) {
-        A.e[s] = m * (0.5 * (vx * vx + vy * vy) + un);
+        e = m * (0.5 * (vx * vx + vy * vy) + un);
+        A.e[s] = e;
+        cap = true;
         A.r_nx[s] = r;
         A.vp_nx[s] = make_double2(vx, vy);
         A.up[s] = un;
@@ -2007,21 +2015,30 @@ __device__ __forceinline__ bool list_review(const BoxDyn &D)
 // calls this (no early return before the shuffles).
 __device__ __forceinline__ void step_tail(const ArenaView &A, int fp32, int s, int b, bool on, double2 a, double ud)
 {
-    double d2 = 0.0;
-    bool review = false;
+    double d2 = 0.0, e = 0.0;
+    bool review = false, cap = false;
     if (s < A.n && b >= 0) {
         if (on) {
-            d2 = fmax(integrate_slot(A, fp32, s, b, a, ud), 0.0);
+            d2 = fmax(integrate_slot(A, fp32, s, b, a, ud, e, cap), 0.0);
             review = list_review(A.dyn[b]);
         } else {
             carry_slot(A, fp32, s);
         }
     }
-    uint64_t pending = __ballot(review);
-    if (pending == 0ull) return;
     const int lane = (int)(threadIdx.x & 63);
     const int bfirst = __shfl(b, 0);
-    if (bfirst >= 0 && bfirst == __shfl(b, 63)) {  // wholly inside one box
+    const bool whole = bfirst >= 0 && bfirst == __shfl(b, 63);  // wholly inside one box
+    // a box's first or last step: a whole wave's energy sum (fixed shuffle
+    // order, every lane the same total) for k_box_step; the box's lanes in
+    // shared waves it sums from e itself
+    if (whole && __ballot(cap) != 0ull) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) e += __shfl_xor(e, off);
+        if (lane == 0) A.wenergy[s >> 6] = e;
+    }
+    uint64_t pending = __ballot(review);
+    if (pending == 0ull) return;
+    if (whole) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off));
         if (lane == 0) A.wdisp[s >> 6] = d2;
@@ -2248,17 +2265,19 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
         __syncthreads();  // red is reused below
     }
     if (capture) {
+        // the step pass's sums of the box's whole waves (wenergy), then the
+        // box's slots in waves it shares (at most 63 at each end)
         const BoxDesc &B = A.boxes[b];
-        const double *e = A.e + B.part_off;
+        const int w0 = (B.part_off + 63) >> 6, w1 = ((B.part_off + B.n) >> 6) - 1;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int i = threadIdx.x;
-        for (; i + 3 * NT_BOX < B.n; i += 4 * NT_BOX) {
-            a0 += e[i];
-            a1 += e[i + NT_BOX];
-            a2 += e[i + 2 * NT_BOX];
-            a3 += e[i + 3 * NT_BOX];
+        int w = w0 + (int)threadIdx.x;
+        for (; w + 3 * NT_BOX <= w1; w += 4 * NT_BOX) {
+            a0 += A.wenergy[w];
+            a1 += A.wenergy[w + NT_BOX];
+            a2 += A.wenergy[w + 2 * NT_BOX];
+            a3 += A.wenergy[w + 3 * NT_BOX];
         }
-        for (; i < B.n; i += NT_BOX) a0 += e[i];
+        for (; w <= w1; w += NT_BOX) a0 += A.wenergy[w];
         red[threadIdx.x] = (a0 + a1) + (a2 + a3);
         __syncthreads();
         for (int off = NT_BOX / 2; off > 0; off >>= 1) {
@@ -2269,8 +2288,15 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
     __syncthreads();  // every thread has read D before thread 0 updates it
     if (threadIdx.x != 0) return;
     if (capture) {
-        if (k == 0) D.E0 = red[0];
-        else D.E1 = red[0];
+        const BoxDesc &B = A.boxes[b];
+        const int s0 = B.part_off, s1 = B.part_off + B.n;
+        const int h1 = min(((s0 + 63) >> 6) << 6, s1), t0 = max((s1 >> 6) << 6, h1);
+        double ends = 0.0;
+        for (int s = s0; s < h1; ++s) ends += A.e[s];
+        for (int s = t0; s < s1; ++s) ends += A.e[s];
+        const double tot = red[0] + ends;
+        if (k == 0) D.E0 = tot;
+        else D.E1 = tot;
     }
     if (built) {
         D.vlast = sqrt(vmax2);
