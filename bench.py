@@ -497,8 +497,8 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (boxes shard by whole box "
-              "across GPUs, which no run here exercises)" % args.gpus, file=sys.stderr)
+        print("bench.py: the stepping path runs on one GPU; --gpus %d is not supported (boxes would shard by "
+              "whole box across GPUs, which is not implemented)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.tune:
