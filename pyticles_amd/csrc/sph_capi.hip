@@ -776,23 +776,48 @@ struct Arena {
             if (int rc = launch_steps(done, false)) return rc;
         }
         int held = 0;
+        // The list review in the step before a scheduled build may postpone
+        // it; a step with the build stage then builds nothing, yet its ten
+        // launches cost ~45 us.  Once a build step of this run has come up
+        // empty, the schedule is read before each build step instead
+        // (postponements recur while the particles move slowly, and are
+        // rare in a start-up transient, where the extra read would only
+        // delay the build).
+        bool ask = false, reviewed = plain0 > 0;
+        int64_t builds = -1;
         while (done < evals) {
+            if (ask && reviewed) {
+                if (int rc = read_dyn()) return rc;
+                const BoxDyn &R = h_dyn[b];
+                if (R.health) break;
+                builds = R.builds;
+                if (R.k < R.next_rebuild) {  // postponed: more steps without the build stage
+                    const int64_t plain = std::min<int64_t>(R.next_rebuild - R.k, evals - done);
+                    if (int rc = launch_steps(plain, false)) return rc;
+                    done += plain;
+                    continue;
+                }
+            }
             const int64_t rel = released;
             if (int rc = launch_steps(1, true)) return rc;
             if (int rc = read_dyn()) return rc;
             if (released != rel) {  // the build held the box; it builds again in 32-bit entries
                 if (++held > 2) return fail(SPH_ERR_HIP, "list build held repeatedly");
+                reviewed = false;
                 continue;
             }
             ++done;
-            if (done >= evals) break;
             const BoxDyn &D = h_dyn[b];
+            if (builds >= 0 && D.builds == builds) ask = true;  // that build step found its build postponed
+            builds = D.builds;
+            if (done >= evals) break;
             if (D.health) break;
             int64_t plain = D.next_rebuild - D.k;
             if (plain < 0) plain = 0;
             if (plain > evals - done) plain = evals - done;
             if (int rc = launch_steps(plain, false)) return rc;
             done += plain;
+            reviewed = plain > 0;
         }
         steps_done += (double)evals;
         return collect_timing();
