@@ -146,8 +146,9 @@ static size_t arena_bytes(size_t cap, int prec, int lcap)
     b += (cap + 63) / 64 * 64 * (size_t)lcap * sizeof(unsigned);  // nbr, in tiles of 64 slots
     b += cap * (3 * d2 + 2 * d1 + 2 * sizeof(double4));  // rho Q jq, T e, S gradv
     b += cap * (2 * d2 + d1);                             // vt, aco, uco
-    if (prec == SPH_FP32C)  // rf S32, rlo, mt32 q32, second buffers of rf, rlo
-        b += cap * (2 * sizeof(float4) + sizeof(unsigned) + 2 * sizeof(float2) + sizeof(float4) + sizeof(unsigned));
+    if (prec == SPH_FP32C)  // rf S32, rlo, g32 q32, second buffers of rf, rlo
+        b += cap * (2 * sizeof(float4) + sizeof(unsigned) + sizeof(GRec32) + sizeof(QRec32) + sizeof(float4) +
+                    sizeof(unsigned));
     b += cap * 5 * sizeof(unsigned);             // sort keys / values
     b += ((cap + 63) / 64 + 1) * i1;             // building tiles
     b += (cap + 63) / 64 * sizeof(double);       // per-wave displacement
@@ -190,7 +191,8 @@ struct Arena {
     double4 *S = nullptr, *gradv = nullptr;
     float4 *rf = nullptr, *S32 = nullptr;
     unsigned *rlo = nullptr;
-    float2 *mt32 = nullptr, *q32 = nullptr;
+    GRec32 *g32 = nullptr;
+    QRec32 *q32 = nullptr;
     double2 *vt = nullptr, *aco = nullptr;
     double *uco = nullptr;
     unsigned *keys_in = nullptr, *k_out = nullptr, *v_out = nullptr, *k_alt = nullptr, *v_alt = nullptr;
@@ -297,7 +299,7 @@ struct Arena {
         A.u_s = u_s; A.m_s = m_s; A.up_s = up_s; A.udot_s = udot_s; A.id_s = id_s;
         A.pcell = pcell; A.r_ref = r_ref; A.slot_box = slot_box; A.nbr = nbr; A.cnt = cnt;
         A.rho = rho; A.T = T; A.S = S; A.Q = Q; A.gradv = gradv; A.jq = jq; A.e = e;
-        A.rf = rf; A.rlo = rlo; A.S32 = S32; A.mt32 = mt32; A.q32 = q32;
+        A.rf = rf; A.rlo = rlo; A.S32 = S32; A.g32 = g32; A.q32 = q32;
         A.vt = vt; A.aco = aco; A.uco = uco;
         A.cell_start = cell_start;
         A.keys = k_out; A.vals = v_out; A.keys_in = keys_in;
@@ -338,7 +340,7 @@ struct Arena {
         if (prec == SPH_FP32C) {
             rc |= dalloc(&rf, cap); rc |= dalloc(&rlo, cap); rc |= dalloc(&S32, cap);
             rc |= dalloc(&rf_nx, cap); rc |= dalloc(&rlo_nx, cap);
-            rc |= dalloc(&mt32, cap); rc |= dalloc(&q32, cap);
+            rc |= dalloc(&g32, cap); rc |= dalloc(&q32, cap);
         }
         rc |= dalloc(&keys_in, cap); rc |= dalloc(&k_out, cap); rc |= dalloc(&v_out, cap);
         rc |= dalloc(&k_alt, cap); rc |= dalloc(&v_alt, cap);
@@ -377,7 +379,7 @@ struct Arena {
         dfree(r_s); dfree(v_s); dfree(vp_s); dfree(acc_s); dfree(u_s); dfree(m_s); dfree(up_s); dfree(udot_s);
         dfree(id_s); dfree(pcell); dfree(slot_box); dfree(r_ref); dfree(nbr); dfree(cnt);
         dfree(rho); dfree(Q); dfree(jq); dfree(T); dfree(e); dfree(S); dfree(gradv);
-        dfree(rf); dfree(rlo); dfree(S32); dfree(mt32); dfree(q32);
+        dfree(rf); dfree(rlo); dfree(S32); dfree(g32); dfree(q32);
         dfree(vt); dfree(aco); dfree(uco);
         dfree(keys_in); dfree(k_out); dfree(v_out); dfree(k_alt); dfree(v_alt);
         dfree(plan_n); dfree(box_coff); dfree(tile_list); dfree(wdisp); dfree(wbuild); dfree(wenergy);

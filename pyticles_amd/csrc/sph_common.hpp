@@ -26,8 +26,8 @@
 //                    vt (double2: m/rho, T), aco / uco (cohesive rates)
 //   FP32 records     rf: the high 32 bits of x, y as 48-bit box-relative
 //                    fixed point (2^48 units per side) and float v; rlo:
-//                    their low 16 bits; S32 (float4: S, m), mt32 (float2:
-//                    m/rho, T), q32 (float2: q/rho^2)
+//                    their low 16 bits; S32 (float4: S, m), g32 (rlo,
+//                    m/rho, T), q32 (rlo, q/rho^2)
 // A box is a contiguous slot range [part_off, part_off + n) plus a key range
 // [key_base, key_base + ncx*ncy) of cells (key_base + ncx*ncy, the next
 // segment's first key, ends its cell_start range); free slot ranges own one
@@ -107,6 +107,22 @@ struct SegDesc {
     int key;        // key base (box) or the segment's single key (free)
 };
 
+// FP32C records that a pass gathers per neighbour in one 16-byte load: the
+// gradient pass's (the neighbour's low position bits, m / rho and T) and the
+// rate pass's conduction record (low position bits and q / rho^2).  The
+// position bits are those of rlo at the time the record is written, which
+// no launch between the writer and the reader changes.
+struct alignas(16) GRec32 {
+    unsigned lo;
+    float v;   // m / rho
+    double T;
+};
+struct alignas(16) QRec32 {
+    unsigned lo;
+    float qx, qy;  // q / rho^2
+    float pad;
+};
+
 struct ArenaView {
     // state (slot order of the last build)
     double2 *r, *v, *vp, *acc;
@@ -144,8 +160,8 @@ struct ArenaView {
     float4 *rf;          // high 32 bits of the 48-bit box-relative x, y; predicted v
     unsigned *rlo;       // low 16 bits of x (bits 0-15) and y (bits 16-31)
     float4 *S32;         // (Sxx, Sxy, Syy, m): the rate pass's stress record
-    float2 *mt32;        // (m / rho, T): the gradient pass reads m / rho
-    float2 *q32;         // q / rho^2 for the rate pass
+    GRec32 *g32;         // the gradient pass's record (rlo, m / rho, T), written by the density pass
+    QRec32 *q32;         // the rate pass's conduction record (rlo, q / rho^2), written by the gradient pass
     // cells and sort
     int *cell_start;
     unsigned *keys;      // sorted keys (final buffer of the radix sort)

@@ -1744,8 +1744,9 @@ __global__ __launch_bounds__(PASS_NT_MAX) void k_density32(ArenaView A, int stat
         const double ps = (B.terms & 1u) ? p / (rd * rd) : 0.0;
         A.S32[s] = make_float4((float)ps, 0.f, (float)ps, (float)mi);
     }
-    if (!(B.terms & 12u)) A.q32[s] = make_float2(0.f, 0.f);
-    A.mt32[s] = make_float2((float)(mi / rd), (float)T);
+    const unsigned lo = A.rlo[s];
+    if (!(B.terms & 12u)) A.q32[s] = QRec32{lo, 0.f, 0.f, 0.f};
+    A.g32[s] = GRec32{lo, (float)(mi / rd), T};
     A.aco[s] = coh ? make_double2(-2.0 * B.a * d.gx.value(), -2.0 * B.a * d.gy.value()) : make_double2(0.0, 0.0);
     A.uco[s] = coh ? -B.a * d.ev.value() : 0.0;
     if (!(rd * B.b < 1.0)) flag_health(A.dyn, b, HEALTH_EOS_POLE);
@@ -1762,7 +1763,11 @@ struct Grad32 {
 // conversions; FP32 rounding of these terms (and of the temperature
 // difference) left up to 4e-6 of force and heating error on relaxed liquid
 // states, where the terms cancel to a small net rate.  The temperature comes
-// from the FP64 T of the density pass.  Per neighbour: rf, rlo, m/rho, T.
+// from the FP64 T of the density pass.  Per neighbour two 16-byte loads:
+// rf and the density pass's record (rlo, m/rho, T); as three records (rlo,
+// m/rho with a float T, the FP64 T) the pass was no faster than the FP64
+// form's, every gather costing the same texture-path time whatever its
+// width.
 template <int U32, int LF>
 __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, const ListFmt &L, int s, Grad32 &g)
 {
@@ -1775,30 +1780,26 @@ __device__ __forceinline__ void grad32_walk(const ArenaView &A, const BoxF &F, c
     for (int k0 = 0; k0 < ns; k0 += U32) {
         int t[U32];
         float4 rj[U32];
-        unsigned lj[U32];
-        float Vj[U32];
-        double Tj[U32];
+        GRec32 gj[U32];
         rounds.take(k0, t);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = at32(A.rf, t[u]);
-            lj[u] = at32(A.rlo, t[u]);
-            Vj[u] = at32(A.mt32, t[u]).x;
-            Tj[u] = at32(A.T, t[u]);
+            gj[u] = at32(A.g32, t[u]);
         }
         rounds.next(k0);
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             double dx, dy;
-            sep_fix<true>(ri, loi, rj[u], lj[u], F, dx, dy);
+            sep_fix<true>(ri, loi, rj[u], gj[u].lo, F, dx, dy);
             const double r2 = rsq_exact(dx, dy);
             if (k0 + u < ns && r2 < F.cutsd) {
                 const double rr = sqrt_pair(r2), w = fma(-rr, F.ihd, 1.0);
                 const double Fk = F.fhd * w * w;
                 const double gx = -Fk * dx, gy = -Fk * dy;
-                const double vj = Vj[u];
+                const double vj = gj[u].v;
                 const double dvx = (double)rj[u].z - vix, dvy = (double)rj[u].w - viy;
-                const double dT = Tj[u] - Ti;
+                const double dT = gj[u].T - Ti;
                 g.g00 += vj * dvx * gx;
                 g.g01 += vj * dvx * gy;
                 g.g10 += vj * dvy * gx;
@@ -1844,7 +1845,7 @@ __global__ __launch_bounds__(PASS_NT_MAX) void k_grad32(ArenaView A, int state_m
         qx = -B.kcond * g.gt0;
         qy = -B.kcond * g.gt1;
     }
-    A.q32[s] = make_float2((float)(qx / r2i), (float)(qy / r2i));
+    A.q32[s] = QRec32{A.rlo[s], (float)(qx / r2i), (float)(qy / r2i), 0.f};
     if (diag) {
         A.gradv[s] = make_double4(G00, G01, G10, G11);
         A.jq[s] = make_double2(qx, qy);
@@ -1856,15 +1857,19 @@ struct Rate32 {
 };
 
 // P3 walk of the FP32C form: FP32 neighbour records, FP64 arithmetic on the
-// exact separations (see grad32_walk).  Per neighbour: rf, rlo, (S, m) and,
-// with conduction, q/rho^2.
+// exact separations (see grad32_walk).  Per neighbour: rf, (S, m) and rlo,
+// the latter, with conduction, in one record with q/rho^2.
 template <int U32, bool COND, int LF>
 __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, const ListFmt &L, int s, int ns,
                                              Rate32 &o)
 {
     const float4 ri = A.rf[s];
     const unsigned loi = A.rlo[s];
-    const float2 qi = COND ? A.q32[s] : make_float2(0.f, 0.f);
+    float2 qi = make_float2(0.f, 0.f);
+    if (COND) {
+        const QRec32 q = A.q32[s];
+        qi = make_float2(q.qx, q.qy);
+    }
     const float4 Si = A.S32[s];
     ListRounds<U32, false, false, LF> rounds(A, s, ns, L);
     for (int k0 = 0; k0 < ns; k0 += U32) {
@@ -1876,9 +1881,15 @@ __device__ __forceinline__ void rates32_walk(const ArenaView &A, const BoxF &F, 
 #pragma unroll
         for (int u = 0; u < U32; ++u) {
             rj[u] = at32(A.rf, t[u]);
-            lj[u] = at32(A.rlo, t[u]);
             Sj[u] = at32(A.S32, t[u]);
-            qj[u] = COND ? at32(A.q32, t[u]) : make_float2(0.f, 0.f);
+            if (COND) {  // the low position bits ride in the conduction record
+                const QRec32 q = at32(A.q32, t[u]);
+                lj[u] = q.lo;
+                qj[u] = make_float2(q.qx, q.qy);
+            } else {
+                lj[u] = at32(A.rlo, t[u]);
+                qj[u] = make_float2(0.f, 0.f);
+            }
         }
         rounds.next(k0);
 #pragma unroll
