@@ -38,22 +38,26 @@ __device__ __forceinline__ uint64_t wave_match8(unsigned d, bool valid)
 }
 
 // Exclusive scan of one value per thread over a 256-thread block; returns the
-// prefix and writes the block total to *total.  Two barriers' worth of LDS.
+// prefix and writes the block total to *total.  Shuffle scans within the
+// four waves, then each thread adds the totals of the waves below it (two
+// barriers; an LDS Hillis-Steele scan took sixteen).
 __device__ unsigned block_exscan256(unsigned x, unsigned *lds /*[256]*/, unsigned *total)
 {
-    const int t = threadIdx.x;
-    lds[t] = x;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        unsigned y = (t >= off) ? lds[t - off] : 0u;
-        __syncthreads();
-        lds[t] += y;
-        __syncthreads();
+    static_assert(RS_THREADS == 256, "four waves");
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    unsigned inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
     }
-    const unsigned incl = lds[t];
-    *total = lds[255];
+    if (lane == 63) lds[w] = inc;
     __syncthreads();
-    return incl - x;
+    unsigned below = 0;
+    for (int ww = 0; ww < w; ++ww) below += lds[ww];
+    *total = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    __syncthreads();  // the caller may reuse lds
+    return below + inc - x;
 }
 
 // hist[d * nblocks + blk] = count of digit d in tile blk.
