@@ -216,6 +216,13 @@ struct Arena {
     int box_cap = 0, seg_cap = 0;
     int nkeys = 1;
     BoxDyn *h_dyn = nullptr;  // pinned mirror for polling
+    // single boxes: the states k_box_step leaves in host-coherent memory at
+    // the end of every step, and their device address; host_dyn_ok while
+    // they equal d_dyn (the last launch that can change a box state was a
+    // step: launch_steps sets it, every other writer of d_dyn clears it)
+    bool keep_host_dyn = false;
+    BoxDyn *h_stepped = nullptr, *d_stepped = nullptr;
+    bool host_dyn_ok = false;
 
     // staging
     double *d_stage = nullptr;
@@ -304,7 +311,7 @@ struct Arena {
         A.cell_start = cell_start;
         A.keys = k_out; A.vals = v_out; A.keys_in = keys_in;
         A.plan_n = plan_n; A.box_coff = box_coff; A.tile_list = tile_list; A.wdisp = wdisp; A.wbuild = wbuild; A.wenergy = wenergy;
-        A.boxes = d_boxes; A.dyn = d_dyn; A.segs = d_segs;
+        A.boxes = d_boxes; A.dyn = d_dyn; A.dyn_host = d_stepped; A.segs = d_segs;
         A.nseg = (int)segs.size();
         A.n = hwm;
         A.nkeys = nkeys;
@@ -403,6 +410,8 @@ struct Arena {
         scan_tmp = nullptr;
         if (h_dyn) (void)hipHostFree(h_dyn);
         h_dyn = nullptr;
+        if (h_stepped) (void)hipHostFree(h_stepped);
+        h_stepped = d_stepped = nullptr;
         for (auto &ev : ev_pool) (void)hipEventDestroy(ev);
         ev_pool.clear();
     }
@@ -410,6 +419,7 @@ struct Arena {
     int ensure_tables(int nbox)
     {
         if (nbox > box_cap) {
+            host_dyn_ok = false;
             int nc = std::max(nbox, box_cap * 2);
             BoxDesc *nb = nullptr;
             BoxDyn *nd = nullptr;
@@ -430,6 +440,18 @@ struct Arena {
             HIPCHK(hipHostMalloc((void **)&nh, sizeof(BoxDyn) * nc, hipHostMallocDefault));
             if (h_dyn) (void)hipHostFree(h_dyn);
             h_dyn = nh;
+            if (keep_host_dyn) {
+                BoxDyn *ns = nullptr, *nsd = nullptr;
+                HIPCHK(hipHostMalloc((void **)&ns, sizeof(BoxDyn) * nc, hipHostMallocCoherent));
+                memset(ns, 0, sizeof(BoxDyn) * nc);
+                if (hipHostGetDevicePointer((void **)&nsd, ns, 0) != hipSuccess) {
+                    (void)hipHostFree(ns);
+                    return fail(SPH_ERR_HIP, "hipHostGetDevicePointer failed");
+                }
+                if (h_stepped) (void)hipHostFree(h_stepped);
+                h_stepped = ns;
+                d_stepped = nsd;
+            }
             box_cap = nc;
             invalidate_graph();
         }
@@ -657,6 +679,7 @@ struct Arena {
     // (see building() in the kernels); the others keep their order.
     void enqueue_build(int force_mask)
     {
+        host_dyn_ok = false;  // the build stage writes box states
         hipStream_t st = ctx->stream;
         ArenaView A = view();
         if (hwm == 0) return;
@@ -705,6 +728,7 @@ struct Arena {
 
     int launch_steps(int64_t count, bool with_build)
     {
+        host_dyn_ok = false;  // until every step is launched
         const bool graphs = use_graph && !ctx->timing;
         for (int64_t i = 0; i < count;) {
             int lg = 0;  // runs of plain steps: the largest chunk graph that fits
@@ -725,6 +749,7 @@ struct Arena {
             }
         }
         HIPCHK(hipGetLastError());
+        host_dyn_ok = count > 0;  // the last launch is a step's k_box_step
         return SPH_OK;
     }
 
@@ -907,12 +932,20 @@ struct Arena {
     }
 
     // The box states, read back (waits for the stream); boxes held by a build
-    // are released on the way (release_held).
+    // are released on the way (release_held).  After a step (host_dyn_ok)
+    // the step's k_box_step has left them in host memory and the wait is
+    // the whole read; otherwise a device-to-host copy.
     int read_dyn()
     {
         if (boxes.empty()) return SPH_OK;
-        HIPCHK(hipMemcpyAsync(h_dyn, d_dyn, sizeof(BoxDyn) * boxes.size(), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (h_stepped && host_dyn_ok) {
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            memcpy(h_dyn, h_stepped, sizeof(BoxDyn) * boxes.size());
+        } else {
+            HIPCHK(hipMemcpyAsync(h_dyn, d_dyn, sizeof(BoxDyn) * boxes.size(), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+        }
         return release_held();
     }
 
@@ -925,6 +958,7 @@ struct Arena {
         static const int zero = 0;
         for (size_t b = 0; b < boxes.size(); ++b) {
             if (!box_live[b] || !h_dyn[b].relist) continue;
+            host_dyn_ok = false;
             boxes[b].build16 = 0;
             HIPCHK(hipMemcpyAsync(d_boxes + b, &boxes[b], sizeof(BoxDesc), hipMemcpyHostToDevice, ctx->stream));
             HIPCHK(hipMemcpyAsync((char *)(d_dyn + b) + offsetof(BoxDyn, relist), &zero, sizeof(int),
@@ -955,6 +989,7 @@ struct Arena {
     int write_dyn(int b, const BoxDyn &D)
     {
         h_dyn[b] = D;
+        host_dyn_ok = false;
         HIPCHK(hipMemcpyAsync(d_dyn + b, h_dyn + b, sizeof(BoxDyn), hipMemcpyHostToDevice, ctx->stream));
         return SPH_OK;
     }
@@ -1234,6 +1269,7 @@ int sph_box_create(sph_ctx *ctx, int64_t n, const sph_params *prm, sph_box **out
     HIPCHK(hipSetDevice(ctx->device));
     sph_box *bx = new sph_box();
     bx->ar.ctx = ctx;
+    bx->ar.keep_host_dyn = true;
     bx->prm = *prm;
     int rc = bx->ar.alloc((int)n, prm->precision, list_entries_for(prm, n));
     if (!rc) rc = bx->ar.add_box(n, prm, &bx->b);
@@ -1500,6 +1536,7 @@ static int eval_box(sph_box *box, int grad, int rates)
     const int f = A.fp32() ? 1 : 0;
     ArenaView V = A.view();
     A.sched_kept = false;  // lists rebuilt outside a run's schedule
+    A.host_dyn_ok = false;  // the passes below may flag box health
     A.mark(T_SORT, true);
     if (int rc = A.build_now(MASK_ALL)) return rc;
     A.mark(T_SORT, false);
@@ -1682,6 +1719,7 @@ int sph_box_time_pass(sph_box *box, int pass, int reps, double *us_per_launch)
     const ArenaView V = A.view();
     const int f = A.fp32() ? 1 : 0;
     hipStream_t st = A.ctx->stream;
+    A.host_dyn_ok = false;  // the passes may flag box health
     if (pass == 3) {
         A.sched_kept = false;
         if (int rc = A.build_now(MASK_ALL)) return rc;  // held builds settled before timing

@@ -183,6 +183,8 @@ struct ArenaView {
     // boxes
     const BoxDesc *boxes;
     BoxDyn *dyn;
+    BoxDyn *dyn_host;    // single boxes: host-coherent pinned memory where k_box_step leaves
+                         // every box's state at the end of each step (null: not kept)
     const SegDesc *segs;
     int nseg;
     int n;               // slots in use (high-water mark)

@@ -2220,6 +2220,8 @@ __device__ __forceinline__ int64_t extend_steps(const BoxDesc &B, const BoxDyn &
 // INIT -> RUN (k = 0); RUN: k++ until k == nsteps, then DONE.  Boxes that
 // built this step get their next build step from their own speed and
 // acceleration, so a box's schedule never depends on the other boxes.
+// With dyn_host set, thread 0 leaves the box's state there last: after a
+// step the host reads it without a device-to-host copy.
 static constexpr int NT_BOX = 1024;
 
 __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
@@ -2227,7 +2229,10 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
     const int b = blockIdx.x;
     if (b >= nbox) return;
     BoxDyn &D = A.dyn[b];
-    if (D.relist) return;  // held: no step taken, the next one builds again
+    if (D.relist) {  // held: no step taken, the next one builds again
+        if (threadIdx.x == 0 && A.dyn_host) A.dyn_host[b] = D;
+        return;
+    }
     const int state = D.state;
     const int64_t k = D.k, K = D.nsteps;
     const bool capture = state == BOX_RUN && (k == 0 || k == K);
@@ -2336,6 +2341,7 @@ __global__ __launch_bounds__(NT_BOX) void k_box_step(ArenaView A, int nbox)
         if (k >= K) D.state = BOX_DONE;
         else D.k = k + 1;
     }
+    if (A.dyn_host) A.dyn_host[b] = D;
 }
 
 // ===========================================================================
