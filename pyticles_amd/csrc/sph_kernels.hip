@@ -806,10 +806,11 @@ __global__ __launch_bounds__(BS_THREADS) void k_box_sort(ArenaView A, int force_
 // four entries per side, so at most one cell per side completes in it, and
 // the round's end stores the completed cells, one 16-byte store each, while
 // the other half fills.  The partial cells are stored at the end, merged
-// when they share a cell.  Entries that do not fit (ns + nl = lcap) set
-// lost.  The ring is the lane's column of an LDS array (words buf[w * NT],
-// 0-7 short, 8-15 long), so an append is one indexed LDS write on one path
-// for short and long entries.  (Stored from inside the candidate loop, as
+// when they share a cell.  A list that would pass lcap entries sets lost
+// and ends with no entries.  The ring is the lane's column of an LDS array
+// (words buf[w * NT], 0-7 short, 8-15 long, 16 the spare that rejected
+// candidates write), so an append is one indexed LDS write on one path
+// for short and long entries, taken by every candidate.  (Stored from inside the candidate loop, as
 // each cell completed, the store path ran for almost every candidate, some
 // lane of the wave completing a cell at nearly every one: 20% more VALU
 // instructions per round, and the C2 build stage took 122 us against 115.)
@@ -878,25 +879,27 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
                     dy = dy < -hLy ? dy + Ly : dy;
                 }
                 const double r2 = rsq_exact(dx, dy);
-                if (t + u < t1 && r2 < rnl2 && t + u != s) {
-                    if (ns + nl >= lcap) {
-                        lost = true;
-                    } else {
-                        const bool sh = r2 < hsk2;
-                        const int row = sh ? (ns & (2 * G - 1)) : (4 * G - 1) - (nl & (2 * G - 1));
-                        if (L16) ((unsigned short *)(buf + (row >> 1) * NT))[row & 1] = (unsigned short)(t + u + ks);
-                        else buf[row * NT] = (unsigned)(t + u);
-                        ns += sh ? 1 : 0;
-                        nl += sh ? 0 : 1;
-                    }
-                }
+                // every candidate writes one ring entry, a rejected one to
+                // the spare word 2G (row 4G), so the append takes no branch
+                // (bitwise, so that the compiler selects instead of branching)
+                const int take = (int)(t + u < t1) & (int)(r2 < rnl2) & (int)(t + u != s);
+                const int sh = (int)(r2 < hsk2);
+                const int rs = ns & (2 * G - 1), rl = (4 * G - 1) - (nl & (2 * G - 1));
+                int row = sh ? rs : rl;
+                row = take ? row : 4 * G;
+                if (L16) ((unsigned short *)(buf + (row >> 1) * NT))[row & 1] = (unsigned short)(t + u + ks);
+                else buf[row * NT] = (unsigned)(t + u);
+                ns += take & sh;
+                nl += take & (sh ^ 1);
             }
-            // the cells completed in this round (at most one per side)
-            if (ns / G > fs) {
+            // the cells completed in this round (at most one per side);
+            // past lcap entries nothing more is stored
+            lost = lost || ns + nl > lcap;
+            if (!lost && ns / G > fs) {
                 put(fs, ring(4 * (fs & 1)));
                 ++fs;
             }
-            if (nl / G > fl) {
+            if (!lost && nl / G > fl) {
                 put(lcap / G - 1 - fl, ring(12 - 4 * (fl & 1)));
                 ++fl;
             }
@@ -914,6 +917,12 @@ __device__ __forceinline__ void list_walk(const ArenaView &A, const BoxDesc &Bu,
             const int t0 = cs[c], t1 = cs[c + 1];
             scan(t0, t1, rx, ry, base16(t0, t1));
         }
+    }
+    // a list that overflowed keeps no entries (its box is flagged
+    // HEALTH_LISTCAP and its run ends)
+    if (lost) {
+        ns = nl = 0;
+        return;
     }
     // partial cells: short positions [0, ns % G) of cell ns / G, long
     // positions [G - nl % G, G) of cell lcap / G - 1 - nl / G; with ns + nl
@@ -1045,9 +1054,9 @@ __device__ __forceinline__ void build_tail(const ArenaView &A, int s, int b, dou
 __global__ __launch_bounds__(NT) void k_build_lists(ArenaView A, int force_mask, int fp32)
 {
     // each lane's entry ring (two cells per side: eight words short, eight
-    // long), one column per lane: word w of lane t at lbuf[w * NT + t],
-    // bank-conflict free
-    __shared__ unsigned lbuf[16 * NT];
+    // long, and a spare word for rejected candidates), one column per lane:
+    // word w of lane t at lbuf[w * NT + t], bank-conflict free
+    __shared__ unsigned lbuf[17 * NT];
     for_building_tiles(A, [&](int s) {
         double v2 = 0.0, a2 = 0.0;
         const int b = build_slot(A, s, force_mask, fp32, lbuf + threadIdx.x, v2, a2);
